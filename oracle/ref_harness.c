@@ -1,0 +1,188 @@
+/*
+ * Reference harness — TEST INFRASTRUCTURE ONLY.
+ *
+ * A main() of this project's own, linked against the reference's hot-path
+ * translation units compiled in place from the reference tree by
+ * oracle/Makefile (nothing of the reference is copied here).  It runs the
+ * reference's own start-up sequence (src/HPhiMain.c:222-288: ReadDefFileNInt,
+ * setmem_def, ReadDefFileIdxPara, SetConvergenceFactor, HPhiTrans, check,
+ * setmem_large, sz, diagonalcalc) on an expert-mode input and then calls the
+ * reference mltply (src/mltply.c:56).
+ *
+ * usage: ref_harness <def-dir> <mode> <out-dir> [seed] [reps]
+ *   mode tables : write list_1.bin list_2_1.bin list_2_2.bin list_Diagonal.bin
+ *                 (raw little-endian arrays exactly as the reference holds them)
+ *                 and meta.txt
+ *   mode hv     : v1 = synthetic vector(seed) (same SplitMix64 generator as the
+ *                 oracle), v0 = 0, one mltply; write v0.bin (dim complex128,
+ *                 0-based), prdct.txt
+ *   mode sums   : as hv but write only checksums (sums.txt) — for large dims
+ *   mode bench  : reps timed mltply calls; print one JSON line
+ * The input directory is entered with chdir, as the reference opens its def
+ * files by relative name and writes ./output/.
+ */
+#include <errno.h>
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "Common.h"
+#include "HPhiTrans.h"
+#include "check.h"
+#include "diagonalcalc.h"
+#include "global.h"
+#include "mltply.h"
+#include "readdef.h"
+#include "sz.h"
+#include "wrapperMPI.h"
+#include "xsetmem.h"
+
+static uint64_t sm64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+static double unit_rng(uint64_t seed, uint64_t c) {
+  return (double)(sm64((seed * 0x9E3779B97F4A7C15ULL) ^ c) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static int dump(const char *dir, const char *name, const void *p, size_t bytes) {
+  char path[4096];
+  snprintf(path, sizeof path, "%s/%s", dir, name);
+  FILE *f = fopen(path, "wb");
+  if (!f) return -1;
+  size_t w = fwrite(p, 1, bytes, f);
+  fclose(f);
+  return w == bytes ? 0 : -1;
+}
+
+static double now(void) { return omp_get_wtime(); }
+
+int main(int argc, char *argv[]) {
+  if (argc < 4) {
+    fprintf(stderr, "usage: %s <def-dir> tables|hv|sums|bench <out-dir> [seed] [reps]\n", argv[0]);
+    return 2;
+  }
+  char outdir[4096];
+  if (argv[3][0] == '/') snprintf(outdir, sizeof outdir, "%s", argv[3]);
+  else {
+    char cwd[3000];
+    if (!getcwd(cwd, sizeof cwd)) return 2;
+    snprintf(outdir, sizeof outdir, "%s/%s", cwd, argv[3]);
+  }
+  const char *mode = argv[2];
+  uint64_t seed = argc > 4 ? strtoull(argv[4], NULL, 10) : 0;
+  int reps = argc > 5 ? atoi(argv[5]) : 3;
+  mkdir(outdir, 0777);
+  if (chdir(argv[1]) != 0) {
+    perror("chdir");
+    return 2;
+  }
+  mkdir("output", 0777);
+
+  /* start-up sequence of src/HPhiMain.c:222-288 */
+  InitializeMPI(argc, argv);
+  InitTimer();
+  char nl[] = "namelist.def";
+  setmem_HEAD(&X.Bind);
+  if (ReadDefFileNInt(nl, &(X.Bind.Def), &(X.Bind.Boost)) != 0) return 3;
+  setmem_def(&X.Bind, &X.Bind.Boost);
+  if (ReadDefFileIdxPara(&(X.Bind.Def), &(X.Bind.Boost)) != 0) return 3;
+  SetConvergenceFactor(&(X.Bind.Def));
+  if (HPhiTrans(&(X.Bind)) != 0) return 3;
+  if (check(&(X.Bind)) == MPIFALSE) return 3;
+  if (setmem_large(&X.Bind) != 0) return 3;
+  double t0 = now();
+  if (sz(&(X.Bind), list_1, list_2_1, list_2_2) != 0) return 3;
+  double t_sz = now() - t0;
+  t0 = now();
+  diagonalcalc(&(X.Bind));
+  double t_diag = now() - t0;
+
+  unsigned long dim = X.Bind.Check.idim_max;
+  int model = X.Bind.Def.iCalcModel;
+  int canon = (model == Spin || model == Hubbard);
+
+  if (strcmp(mode, "tables") == 0) {
+    FILE *f;
+    char path[4200];
+    snprintf(path, sizeof path, "%s/meta.txt", outdir);
+    f = fopen(path, "w");
+    fprintf(f, "dim %lu\nsdim %lu\nsize21 %lu\nsize22 %lu\nmodel %d\n", dim, X.Bind.Check.sdim,
+            (unsigned long)X.Bind.Large.SizeOflist_2_1, (unsigned long)X.Bind.Large.SizeOflist_2_2, model);
+    fclose(f);
+    if (canon) {
+      if (dump(outdir, "list_1.bin", list_1, (dim + 1) * sizeof(long unsigned int))) return 4;
+      if (dump(outdir, "list_2_1.bin", list_2_1, X.Bind.Large.SizeOflist_2_1 * sizeof(long unsigned int))) return 4;
+      if (dump(outdir, "list_2_2.bin", list_2_2, X.Bind.Large.SizeOflist_2_2 * sizeof(long unsigned int))) return 4;
+    }
+    if (dump(outdir, "list_Diagonal.bin", list_Diagonal, (dim + 1) * sizeof(double))) return 4;
+    return 0;
+  }
+
+  /* synthetic input, element j (1-based) from counter 2(j-1), 2(j-1)+1 */
+#pragma omp parallel for
+  for (unsigned long j = 1; j <= dim; j++) {
+    v1[j] = (2.0 * unit_rng(seed, 2 * (j - 1)) - 1.0) + I * (2.0 * unit_rng(seed, 2 * (j - 1) + 1) - 1.0);
+    v0[j] = 0.0;
+  }
+
+  if (strcmp(mode, "bench") == 0) {
+    double best = 1e300, tot = 0.0;
+    for (int r = 0; r < reps; r++) {
+#pragma omp parallel for
+      for (unsigned long j = 1; j <= dim; j++) v0[j] = 0.0;
+      t0 = now();
+      mltply(&(X.Bind), v0, v1);
+      double dt = now() - t0;
+      tot += dt;
+      if (dt < best) best = dt;
+    }
+    printf("{\"dim\": %lu, \"threads\": %d, \"reps\": %d, \"hv_s_best\": %.6f, \"hv_s_mean\": %.6f, "
+           "\"sz_s\": %.6f, \"diag_s\": %.6f}\n",
+           dim, nthreads, reps, best, tot / reps, t_sz, t_diag);
+    return 0;
+  }
+
+  t0 = now();
+  if (mltply(&(X.Bind), v0, v1) != 0) return 5;
+  double t_hv = now() - t0;
+  double complex pr = X.Bind.Large.prdct;
+
+  char path[4200];
+  FILE *f;
+  if (strcmp(mode, "hv") == 0) {
+    if (dump(outdir, "v0.bin", v0 + 1, dim * sizeof(double complex))) return 4;
+  } else if (strcmp(mode, "sums") == 0) {
+    /* checksums: S1 = sum v0, S2 = sum |v0|^2, S3 = sum w_j v0[j] with w_j = (j mod 97) + 1
+     * and 64 sampled entries at j = 1 + (k * 2654435761) mod dim */
+    double s1r = 0, s1i = 0, s2 = 0, s3r = 0, s3i = 0;
+#pragma omp parallel for reduction(+ : s1r, s1i, s2, s3r, s3i)
+    for (unsigned long j = 1; j <= dim; j++) {
+      double a = creal(v0[j]), b = cimag(v0[j]);
+      double w = (double)((j % 97) + 1);
+      s1r += a; s1i += b; s2 += a * a + b * b; s3r += w * a; s3i += w * b;
+    }
+    snprintf(path, sizeof path, "%s/sums.txt", outdir);
+    f = fopen(path, "w");
+    fprintf(f, "S1 %.17g %.17g\nS2 %.17g\nS3 %.17g %.17g\n", s1r, s1i, s2, s3r, s3i);
+    for (unsigned long k = 0; k < 64; k++) {
+      unsigned long j = 1 + (unsigned long)((k * 2654435761ULL) % dim);
+      fprintf(f, "E %lu %.17g %.17g\n", j, creal(v0[j]), cimag(v0[j]));
+    }
+    fclose(f);
+  } else {
+    fprintf(stderr, "unknown mode %s\n", mode);
+    return 2;
+  }
+  snprintf(path, sizeof path, "%s/prdct.txt", outdir);
+  f = fopen(path, "w");
+  fprintf(f, "%.17g %.17g\n%lu\n%.6f %.6f %.6f\n", creal(pr), cimag(pr), dim, t_sz, t_diag, t_hv);
+  fclose(f);
+  return 0;
+}
