@@ -1,0 +1,274 @@
+"""ctypes binding of libhphi_amd.so (the C ABI in include/hphi_amd.h).
+
+The shared library is the product; this module only marshals arguments.  It
+raises when the library is missing: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhphi_amd.so")
+
+_i32p = C.POINTER(C.c_int)
+_f64p = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+
+# every symbol include/hphi_amd.h declares
+EXPORTED = [
+    "hphi_last_error", "hphi_engine_create", "hphi_engine_create_from_def", "hphi_engine_destroy",
+    "hphi_engine_info", "hphi_engine_device_bytes", "hphi_sz", "hphi_diagonalcalc", "hphi_mltply",
+    "hphi_mltply_dev", "hphi_mltply_block_dev", "hphi_dev_alloc", "hphi_dev_free", "hphi_memcpy_h2d",
+    "hphi_memcpy_d2h", "hphi_dev_zero", "hphi_synchronize", "hphi_fill_random",
+    "hphi_vec_lanczos_beta", "hphi_vec_lanczos_update", "hphi_vec_norm2", "hphi_vec_dot",
+    "hphi_vec_axpby", "hphi_engine_sync", "hphi_time_mltply_dev", "hphi_lanczos",
+]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("model", C.c_int), ("nsite", C.c_int), ("nup", C.c_int), ("ndown", C.c_int),
+                ("n_exchange", C.c_int), ("exchange_ij", _i32p), ("exchange_J", _f64p),
+                ("n_pairlift", C.c_int), ("pairlift_ij", _i32p), ("pairlift_J", _f64p),
+                ("n_transfer", C.c_int), ("transfer_idx", _i32p), ("transfer_t", _f64p),
+                ("n_chemi", C.c_int), ("chemi_site", _i32p), ("chemi_spin", _i32p), ("chemi_mu", _f64p),
+                ("n_intra", C.c_int), ("intra_site", _i32p), ("intra_U", _f64p),
+                ("n_inter", C.c_int), ("inter_ij", _i32p), ("inter_V", _f64p),
+                ("n_hund", C.c_int), ("hund_ij", _i32p), ("hund_J", _f64p)]
+
+
+class ModPara(C.Structure):
+    _fields_ = [("calc_type", C.c_int), ("lanczos_max", C.c_int), ("initial_iv", C.c_int),
+                ("exct", C.c_int), ("lanczos_eps", C.c_int), ("lanczos_target", C.c_int),
+                ("num_ave", C.c_int), ("expec_interval", C.c_int), ("large_value", C.c_double)]
+
+
+class LanczosParams(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("n_eigen", C.c_int), ("eps", C.c_double),
+                ("target", C.c_int), ("seed", C.c_uint64), ("check_every", C.c_int)]
+
+
+class LanczosResult(C.Structure):
+    _fields_ = [("eigenvalues", C.c_double * 16), ("n_found", C.c_int), ("iterations", C.c_int),
+                ("converged", C.c_int), ("seconds", C.c_double), ("hv_seconds", C.c_double)]
+
+
+_lib = None
+
+
+class HphiError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libhphi_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HphiError(f"{LIB_PATH} is missing: build it with `python -m hphi_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.hphi_last_error.restype = C.c_char_p
+        L.hphi_engine_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(_vp)]
+        L.hphi_engine_create_from_def.argtypes = [C.c_char_p, C.POINTER(_vp), C.POINTER(ModPara)]
+        L.hphi_engine_destroy.argtypes = [_vp]
+        L.hphi_engine_destroy.restype = None
+        L.hphi_engine_info.argtypes = [_vp] + [_u64p] * 6
+        L.hphi_engine_device_bytes.argtypes = [_vp]
+        L.hphi_engine_device_bytes.restype = C.c_uint64
+        L.hphi_sz.argtypes = [_vp, _u64p, _u64p, _u64p]
+        L.hphi_diagonalcalc.argtypes = [_vp, _f64p]
+        L.hphi_mltply.argtypes = [_vp, _vp, _vp, _f64p]
+        L.hphi_mltply_dev.argtypes = [_vp, _vp, _vp, _f64p]
+        L.hphi_mltply_block_dev.argtypes = [_vp, C.c_int, _vp, _vp, _f64p]
+        L.hphi_dev_alloc.argtypes = [C.c_size_t, C.POINTER(_vp)]
+        L.hphi_dev_free.argtypes = [_vp]
+        L.hphi_memcpy_h2d.argtypes = [_vp, _vp, C.c_size_t]
+        L.hphi_memcpy_d2h.argtypes = [_vp, _vp, C.c_size_t]
+        L.hphi_dev_zero.argtypes = [_vp, C.c_size_t]
+        L.hphi_fill_random.argtypes = [_vp, C.c_uint64, C.c_int, C.c_uint64]
+        L.hphi_vec_lanczos_beta.argtypes = [_vp, _vp, _vp, C.c_double, _f64p]
+        L.hphi_vec_lanczos_update.argtypes = [_vp, _vp, _vp, C.c_double, C.c_double]
+        L.hphi_vec_norm2.argtypes = [_vp, _vp, C.c_int, _f64p]
+        L.hphi_vec_dot.argtypes = [_vp, _vp, _vp, C.c_int, _f64p]
+        L.hphi_vec_axpby.argtypes = [_vp, _vp, _vp, C.c_int, _f64p]
+        L.hphi_engine_sync.argtypes = [_vp]
+        L.hphi_time_mltply_dev.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _f64p]
+        L.hphi_lanczos.argtypes = [_vp, C.POINTER(LanczosParams), C.POINTER(LanczosResult)]
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise HphiError(lib().hphi_last_error().decode())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None and a.size else None
+
+
+class DeviceArray:
+    """A raw device allocation (complex128 elements) owned by Python."""
+
+    def __init__(self, n_complex: int):
+        self.n = int(n_complex)
+        self.nbytes = 16 * self.n
+        p = _vp()
+        check(lib().hphi_dev_alloc(max(self.nbytes, 16), C.byref(p)))
+        self.ptr = p
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.size == self.n
+        check(lib().hphi_memcpy_h2d(self.ptr, a.ctypes.data_as(_vp), self.nbytes))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, np.complex128)
+        check(lib().hphi_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+    def zero(self) -> "DeviceArray":
+        check(lib().hphi_dev_zero(self.ptr, self.nbytes))
+        return self
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().hphi_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One Hamiltonian on the GPU: tables, diagonal and H·v through the C ABI."""
+
+    def __init__(self, handle: _vp, modpara: Optional[ModPara] = None):
+        self.h = handle
+        self.modpara = modpara
+        d = [C.c_uint64() for _ in range(6)]
+        check(lib().hphi_engine_info(self.h, *[C.byref(x) for x in d]))
+        self.dim, self.size21, self.size22, self.irght, self.ilft, self.ihfbit = [x.value for x in d]
+
+    @classmethod
+    def from_terms(cls, model: int, nsite: int, nup: int, ndown: int, t: Dict[str, np.ndarray]) -> "Engine":
+        keep = {k: np.ascontiguousarray(v) for k, v in t.items()}
+        d = ModelDesc()
+        d.model, d.nsite, d.nup, d.ndown = model, nsite, nup, ndown
+        d.n_exchange, d.exchange_ij, d.exchange_J = len(keep["exchange_J"]), _p(keep["exchange_ij"], _i32p), _p(keep["exchange_J"], _f64p)
+        d.n_pairlift, d.pairlift_ij, d.pairlift_J = len(keep["pairlift_J"]), _p(keep["pairlift_ij"], _i32p), _p(keep["pairlift_J"], _f64p)
+        d.n_transfer, d.transfer_idx, d.transfer_t = len(keep["trans_t"]), _p(keep["trans_idx"], _i32p), _p(keep["trans_t"], _f64p)
+        d.n_chemi, d.chemi_site, d.chemi_spin, d.chemi_mu = (len(keep["chemi_mu"]), _p(keep["chemi_site"], _i32p),
+                                                             _p(keep["chemi_spin"], _i32p), _p(keep["chemi_mu"], _f64p))
+        d.n_intra, d.intra_site, d.intra_U = len(keep["intra_U"]), _p(keep["intra_site"], _i32p), _p(keep["intra_U"], _f64p)
+        d.n_inter, d.inter_ij, d.inter_V = len(keep["inter_V"]), _p(keep["inter_ij"], _i32p), _p(keep["inter_V"], _f64p)
+        d.n_hund, d.hund_ij, d.hund_J = len(keep["hund_J"]), _p(keep["hund_ij"], _i32p), _p(keep["hund_J"], _f64p)
+        h = _vp()
+        check(lib().hphi_engine_create(C.byref(d), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_spec(cls, spec) -> "Engine":
+        from .lattice import processed_terms
+        return cls.from_terms(spec.model, spec.nsite, spec.nup, spec.ndown, processed_terms(spec))
+
+    @classmethod
+    def from_def(cls, namelist: str) -> "Engine":
+        h = _vp()
+        mp = ModPara()
+        check(lib().hphi_engine_create_from_def(namelist.encode(), C.byref(h), C.byref(mp)))
+        return cls(h, mp)
+
+    def close(self):
+        if self.h:
+            lib().hphi_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().hphi_engine_device_bytes(self.h))
+
+    # -- drop-in entry points ------------------------------------------------
+    def sz(self, copy: bool = True) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        if not copy:
+            check(lib().hphi_sz(self.h, None, None, None))
+            return None, None, None
+        l1 = np.zeros(self.dim + 1, np.uint64)
+        l21 = np.zeros(max(self.size21, 1), np.uint64)
+        l22 = np.zeros(max(self.size22, 1), np.uint64)
+        check(lib().hphi_sz(self.h, _p(l1, _u64p), _p(l21, _u64p) if self.size21 else None,
+                            _p(l22, _u64p) if self.size22 else None))
+        return l1, l21, l22
+
+    def diagonalcalc(self, copy: bool = True) -> Optional[np.ndarray]:
+        if not copy:
+            check(lib().hphi_diagonalcalc(self.h, None))
+            return None
+        d = np.zeros(self.dim + 1, np.float64)
+        check(lib().hphi_diagonalcalc(self.h, _p(d, _f64p)))
+        return d
+
+    def mltply(self, v0, v1) -> complex:
+        """Reference-shaped call: v0[1..dim] += H v1[1..dim] on 1-indexed host arrays."""
+        assert v0.dtype == np.complex128 and v1.dtype == np.complex128 and v0.flags.c_contiguous
+        pr = np.zeros(2)
+        check(lib().hphi_mltply(self.h, v0.ctypes.data_as(_vp), v1.ctypes.data_as(_vp), _p(pr, _f64p)))
+        return complex(pr[0], pr[1])
+
+    def mltply_dev(self, v0: DeviceArray, v1: DeviceArray) -> complex:
+        pr = np.zeros(2)
+        check(lib().hphi_mltply_dev(self.h, v0.ptr, v1.ptr, _p(pr, _f64p)))
+        return complex(pr[0], pr[1])
+
+    def mltply_block_dev(self, k: int, V0: DeviceArray, V1: DeviceArray) -> np.ndarray:
+        pr = np.zeros(2 * k)
+        check(lib().hphi_mltply_block_dev(self.h, k, V0.ptr, V1.ptr, _p(pr, _f64p)))
+        return pr[0::2] + 1j * pr[1::2]
+
+    def hv(self, v: np.ndarray) -> Tuple[np.ndarray, complex]:
+        """H v of a 0-based host vector through device memory."""
+        a = DeviceArray(self.dim).upload(v)
+        b = DeviceArray(self.dim).zero()
+        pr = self.mltply_dev(b, a)
+        return b.download(), pr
+
+    def time_mltply(self, k: int, V0: DeviceArray, V1: DeviceArray, warmup: int, reps: int) -> float:
+        ms = C.c_double()
+        check(lib().hphi_time_mltply_dev(self.h, k, V0.ptr, V1.ptr, warmup, reps, C.byref(ms)))
+        return ms.value
+
+    def fill_random(self, v: DeviceArray, k: int, seed: int) -> None:
+        check(lib().hphi_fill_random(v.ptr, self.dim, k, seed))
+
+    def lanczos(self, n_eigen: int = 1, max_iter: int = 2000, eps: float = 1e-14, target: int = 2,
+                seed: int = 0, check_every: int = 1) -> LanczosResult:
+        p = LanczosParams(max_iter, n_eigen, eps, target, seed, check_every)
+        r = LanczosResult()
+        check(lib().hphi_lanczos(self.h, C.byref(p), C.byref(r)))
+        return r
+
+
+def tridiag_lowest(a: np.ndarray, b: np.ndarray, k: int) -> np.ndarray:
+    """Lowest k eigenvalues of the symmetric tridiagonal (host C routine of the library)."""
+    L = lib()
+    L.hphi_tridiag_lowest.argtypes = [_f64p, _f64p, C.c_int, C.c_int, _f64p]
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    ev = np.zeros(k)
+    L.hphi_tridiag_lowest(_p(a, _f64p), _p(b, _f64p) if b.size else None, len(a), k, _p(ev, _f64p))
+    return ev

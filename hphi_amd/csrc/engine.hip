@@ -1,0 +1,1028 @@
+// hphi_amd engine: device basis tables, diagonal and gather-form H·v for
+// CDNA4 (gfx950).  C ABI declared in include/hphi_amd.h.
+//
+// Layout in HBM (per engine):
+//   list1  uint64[dim]      canonical states in ascending order (sz.c list_1[1..dim])
+//   l21    uint32[size21]   rank-in-class + 2, 0 = not in sector (sz.c list_2_1)
+//   l22    uint64[size22]   states before the block + 1, 0 = empty block (sz.c list_2_2)
+//   diag   double[dim]      list_Diagonal[1..dim]
+//   terms  GTerm[...]       off-diagonal terms, grouped low/high/mixed
+// Vectors are 0-based complex128 (double2); block vectors are state-major
+// [dim][k] so that one gather moves k*16 contiguous bytes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hphi_amd.h"
+#include "engine_internal.h"
+
+// ------------------------------------------------------------------ errors --
+static thread_local char g_err[1024] = "";
+
+extern "C" int hphi_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return -1;
+}
+
+extern "C" const char *hphi_last_error(void) { return g_err; }
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return hphi_set_error("%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------ binomials ----
+__constant__ uint64_t c_binom[65][65];
+
+static uint64_t h_binom(int n, int k) {
+  if (k < 0 || n < 0 || k > n) return 0;
+  uint64_t r = 1;
+  for (int i = 1; i <= k; i++) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
+  return r;
+}
+
+__device__ __forceinline__ uint64_t dbinom(int n, int k) {
+  return (k < 0 || n < 0 || k > n) ? 0ull : c_binom[n][k];
+}
+
+static int ensure_binom_table() {
+  static bool done = false;
+  if (done) return 0;
+  static uint64_t t[65][65];
+  for (int n = 0; n <= 64; n++)
+    for (int k = 0; k <= 64; k++) t[n][k] = h_binom(n, k);
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), t, sizeof t));
+  done = true;
+  return 0;
+}
+
+// ------------------------------------------------------- basis tables ----
+// Closed forms restating sz.c (see engine_internal.h for the parameters):
+//   block prefix  jb[ib] = sum_{ib'<ib} cnt(ib'), evaluated per set bit of ib
+//                 with Vandermonde's identity (no scan)
+//   list_2_2[ib]  = jb[ib] + 1 for a non-empty block, else 0   (sz.c:1525,1069)
+//   list_2_1[ia]  = rank of ia among the same class + 2, 0 when no block
+//                 pairs with its class                           (sz.c:1524,1068)
+//   list_1        unranked per block in ascending order          (sz.c:1523,1067)
+// Spin: class = popcount.  Hubbard: class = (up, down) counts, up = even
+// absolute bit (mltplyHubbardCore.c:45-46).
+
+__device__ __forceinline__ void hub_counts(uint64_t x, int shift, int *nu, int *nd) {
+  // counts of set bits at even / odd ABSOLUTE positions, x holding bits from `shift`
+  const uint64_t even = 0x5555555555555555ull, odd = 0xAAAAAAAAAAAAAAAAull;
+  uint64_t e = (shift & 1) ? odd : even, o = (shift & 1) ? even : odd;
+  *nu = __popcll(x & e);
+  *nd = __popcll(x & o);
+}
+
+// number of even (up) absolute positions among relative positions [0, p) of a
+// half starting at absolute bit `shift`
+__device__ __forceinline__ int n_up_below(int p, int shift) {
+  return (shift & 1) ? p / 2 : (p + 1) / 2;
+}
+
+__global__ void k_block_prefix(SzParams P, uint64_t *jb, uint64_t *l22) {
+  uint64_t ib = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib > P.sdim) return;
+  uint64_t acc = 0;
+  if (P.hubbard) {
+    int su = 0, sd = 0;
+    for (int p = P.nhigh; p >= 0; p--) {
+      if (!((ib >> p) & 1)) continue;
+      int eu = n_up_below(p, P.nlow), ed = p - eu;
+      acc += dbinom(eu + P.nlu, P.nup - su) * dbinom(ed + P.nld, P.ndn - sd);
+      if (((p + P.nlow) & 1) == 0) su++; else sd++;
+    }
+  } else {
+    int s = 0;
+    for (int p = P.nhigh; p >= 0; p--) {
+      if (!((ib >> p) & 1)) continue;
+      acc += dbinom(p + P.nlow, P.nup - s);
+      s++;
+    }
+  }
+  jb[ib] = acc;
+  if (ib < P.sdim) {
+    uint64_t cnt;
+    if (P.hubbard) {
+      int nu, nd;
+      hub_counts(ib, P.nlow, &nu, &nd);
+      cnt = dbinom(P.nlu, P.nup - nu) * dbinom(P.nld, P.ndn - nd);
+    } else {
+      cnt = dbinom(P.nlow, P.nup - __popcll(ib));
+    }
+    l22[ib] = cnt ? acc + 1 : 0;
+  } else {
+    l22[ib] = 0;
+    l22[ib + 1] = 0;
+  }
+}
+
+__global__ void k_list21(SzParams P, uint32_t *l21) {
+  uint64_t ia = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= P.size21) return;
+  uint32_t val = 0;
+  if (ia < P.ihfbit) {
+    if (P.hubbard) {
+      int a, b;
+      hub_counts(ia, 0, &a, &b);
+      if (P.nup - a >= 0 && P.nup - a <= P.nhu && P.ndn - b >= 0 && P.ndn - b <= P.nhd) {
+        uint64_t rank = 0;
+        int su = 0, sd = 0;
+        for (int p = P.nlow - 1; p >= 0; p--) {
+          if (!((ia >> p) & 1)) continue;
+          int eu = (p + 1) / 2, ed = p / 2;
+          rank += dbinom(eu, a - su) * dbinom(ed, b - sd);
+          if ((p & 1) == 0) su++; else sd++;
+        }
+        val = (uint32_t)(rank + 2);
+      }
+    } else {
+      int k = __popcll(ia);
+      if (P.nup - k >= 0 && P.nup - k <= P.nhigh) {
+        uint64_t rank = 0;
+        int s = 0;
+        for (int p = P.nlow - 1; p >= 0; p--) {
+          if (!((ia >> p) & 1)) continue;
+          rank += dbinom(p, k - s);
+          s++;
+        }
+        val = (uint32_t)(rank + 2);
+      }
+    }
+  }
+  l21[ia] = val;
+}
+
+// one workgroup per high block; lanes unrank consecutive ranks (coalesced writes)
+__global__ __launch_bounds__(256) void k_list1(SzParams P, const uint64_t *jb, uint64_t *list1) {
+  for (uint64_t ib = blockIdx.x; ib < P.sdim; ib += gridDim.x) {
+    uint64_t base = jb[ib], end = jb[ib + 1];
+    if (end > P.dim) end = P.dim;  // never write past list1, even for inconsistent prefixes
+    if (end <= base) continue;
+    uint64_t cnt = end - base;
+    int a, b;
+    if (P.hubbard) {
+      int nu, nd;
+      hub_counts(ib, P.nlow, &nu, &nd);
+      a = P.nup - nu;
+      b = P.ndn - nd;
+    } else {
+      a = P.nup - __popcll(ib);
+      b = 0;
+    }
+    for (uint64_t r = threadIdx.x; r < cnt; r += blockDim.x) {
+      uint64_t rem = r, ia = 0;
+      int ra = a, rb = b;
+      if (P.hubbard) {
+        for (int p = P.nlow - 1; p >= 0; p--) {
+          uint64_t c = dbinom((p + 1) / 2, ra) * dbinom(p / 2, rb);
+          if (rem >= c) {
+            ia |= 1ull << p;
+            rem -= c;
+            if ((p & 1) == 0) ra--; else rb--;
+          }
+        }
+      } else {
+        for (int p = P.nlow - 1; p >= 0 && ra > 0; p--) {
+          uint64_t c = dbinom(p, ra);
+          if (rem >= c) {
+            ia |= 1ull << p;
+            rem -= c;
+            ra--;
+          }
+        }
+      }
+      list1[base + r] = (ib << P.nlow) | ia;
+    }
+  }
+}
+
+// --------------------------------------------------------------- diagonal --
+// One lane per state, terms applied in the reference's order
+// (diagonalcalc.c:105-180: CoulombIntra, Chemi, CoulombInter, Hund) with the
+// same expressions, so the result is bit-identical.
+__global__ void k_diag(uint64_t dim, const uint64_t *__restrict__ list1, int gc,
+                       const DTerm *__restrict__ terms, int nterms, double *__restrict__ diag) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= dim) return;
+  uint64_t x = gc ? j : list1[j];
+  double D = 0.0;
+  for (int t = 0; t < nterms; t++) {
+    const DTerm T = terms[t];
+    switch (T.kind) {
+      case D_INTRA:
+        if ((x & T.m1) == T.m1) D += T.V;
+        break;
+      case D_CHEMI_H: {
+        uint64_t n = (x & T.m1) / T.m1;
+        D += (double)n * T.V;
+      } break;
+      case D_CHEMI_S: {
+        uint64_t b = ((x & T.m1) / T.m1) ^ T.m2;
+        D += T.V * (double)b;
+      } break;
+      case D_INTER_H: {
+        uint64_t n1 = (x & T.m1) / T.m1 + (x & T.m2) / T.m2;
+        uint64_t n2 = (x & T.m3) / T.m3 + (x & T.m4) / T.m4;
+        D += (double)(n1 * n2) * T.V;
+      } break;
+      case D_INTER_S:
+        D += T.V;
+        break;
+      case D_HUND_H: {
+        uint64_t a = (x & T.m1) / T.m1, b = (x & T.m2) / T.m2;
+        uint64_t c = (x & T.m3) / T.m3, d = (x & T.m4) / T.m4;
+        D += T.V * (double)(a * c + b * d);
+      } break;
+      case D_HUND_S: {
+        uint64_t ibit = x & T.m1;
+        if (ibit == 0 || ibit == T.m1) D += T.V;
+      } break;
+    }
+  }
+  diag[j] = D;
+}
+
+// ------------------------------------------------------------------- H·v --
+// Gather form: lane (state j, vector v) accumulates
+//   acc = diag[j] v1[j] + sum_t c_t * sgn_t(x) * v1[idx(x ^ flip_t)]
+// over the terms whose predicate holds on x = list1[j] (or j for SpinGC).
+// This is the transpose of the reference's per-term scatter
+// v0[idx(s ^ flip)] += c v1[s]: every (term, source) pair of the reference
+// appears once as a (term, target) pair here, with the same coefficient and
+// the same Jordan-Wigner sign (the sign mask never overlaps the flipped bits).
+// prdct = sum_j conj(v1[j]) * acc_j equals the reference's sum of
+// c v1[s] conj(v1[target]) over the same pairs (mltply.c:100, mltplySpinCore.c:304).
+//
+// Loads are made unconditional (inactive lanes re-read their own element and
+// weight it by zero) so the unrolled term loop keeps many gathers in flight.
+
+template <int K>
+__device__ __forceinline__ void wg_reduce_store(double pre, double pim, double2 *partial) {
+  // lanes with equal (lane % K) hold the same vector column
+#pragma unroll
+  for (int off = 32; off >= K; off >>= 1) {
+    pre += __shfl_xor(pre, off, 64);
+    pim += __shfl_xor(pim, off, 64);
+  }
+  __shared__ double2 red[HV_TPB / 64][K];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane < K) red[wave][lane] = make_double2(pre, pim);
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double2 s = red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < HV_TPB / 64; w++) {
+      s.x += red[w][threadIdx.x].x;
+      s.y += red[w][threadIdx.x].y;
+    }
+    partial[(uint64_t)blockIdx.x * K + threadIdx.x] = s;
+  }
+}
+
+__device__ __forceinline__ bool term_active(const GTerm &T, uint64_t x) {
+  uint64_t ym = x & T.tmask;
+  int pc = __popcll(ym);
+  return (T.kind == P_EQ) ? (ym == T.tval) : ((T.kind == P_DIFF) ? (pc == 1) : (pc != 1));
+}
+
+template <int K, bool CANON>
+__global__ __launch_bounds__(HV_TPB) void k_hv(const HvArgs A) {
+  constexpr int SPT = HV_TPB / K;  // states per tile
+  const int vec = threadIdx.x & (K - 1);
+  const int sl = threadIdx.x / K;
+  const double2 *__restrict__ v1 = (const double2 *)A.v1;
+  double2 *__restrict__ v0 = (double2 *)A.v0;
+  double pre = 0.0, pim = 0.0;
+
+  const uint64_t ntiles = (A.dim + SPT - 1) / SPT;
+  uint64_t t_begin, t_end, t_step, tile;
+  if (A.xcd_map) {
+    const uint32_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+    t_begin = (ntiles * xcd) / 8 + slot;
+    t_end = (ntiles * (xcd + 1)) / 8;
+    t_step = nslot;
+  } else {
+    t_begin = blockIdx.x;
+    t_end = ntiles;
+    t_step = gridDim.x;
+  }
+  for (tile = t_begin; tile < t_end; tile += t_step) {
+    const uint64_t j = tile * SPT + sl;
+    if (j >= A.dim) continue;
+    const uint64_t x = CANON ? A.list1[j] : j;
+    const double2 a = v1[j * K + vec];
+    const double d = A.diag[j];
+    double ar = d * a.x, ai = d * a.y;
+    if (CANON) {
+      const uint64_t ia0 = x & A.irght, ib0 = x >> A.nlow;
+      const int64_t b21 = (int64_t)A.l21[ia0], b22 = (int64_t)A.l22[ib0];
+      // low group: only the low half changes, same block
+      for (int t = 0; t < A.n_low; t += HV_UNROLL) {
+        double2 g[HV_UNROLL];
+        double cr[HV_UNROLL], ci[HV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < HV_UNROLL; u++) {
+          const GTerm T = A.terms[t + u];
+          const uint64_t y = x ^ T.flip;
+          const int64_t l = (int64_t)A.l21[y & A.irght];
+          bool act = term_active(T, x) && (l != 0);
+          const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
+          uint64_t idx = (uint64_t)((int64_t)j + l - b21);
+          act = act && idx < A.dim;
+          idx = act ? idx : j;
+          g[u] = v1[idx * K + vec];
+          cr[u] = act ? s * T.cre : 0.0;
+          ci[u] = act ? s * T.cim : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < HV_UNROLL; u++) {
+          ar += cr[u] * g[u].x - ci[u] * g[u].y;
+          ai += cr[u] * g[u].y + ci[u] * g[u].x;
+        }
+      }
+      // high group: only the high half changes, same rank in another block
+      const GTerm *th = A.terms + A.n_low;
+      for (int t = 0; t < A.n_high; t += HV_UNROLL) {
+        double2 g[HV_UNROLL];
+        double cr[HV_UNROLL], ci[HV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < HV_UNROLL; u++) {
+          const GTerm T = th[t + u];
+          const uint64_t y = x ^ T.flip;
+          const int64_t l = (int64_t)A.l22[y >> A.nlow];
+          bool act = term_active(T, x) && (l != 0);
+          const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
+          uint64_t idx = (uint64_t)((int64_t)j + l - b22);
+          act = act && idx < A.dim;
+          idx = act ? idx : j;
+          g[u] = v1[idx * K + vec];
+          cr[u] = act ? s * T.cre : 0.0;
+          ci[u] = act ? s * T.cim : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < HV_UNROLL; u++) {
+          ar += cr[u] * g[u].x - ci[u] * g[u].y;
+          ai += cr[u] * g[u].y + ci[u] * g[u].x;
+        }
+      }
+      // mixed group: both halves change, two lookups
+      const GTerm *tm = th + A.n_high;
+      for (int t = 0; t < A.n_mixed; t += HV_UNROLL) {
+        double2 g[HV_UNROLL];
+        double cr[HV_UNROLL], ci[HV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < HV_UNROLL; u++) {
+          const GTerm T = tm[t + u];
+          const uint64_t y = x ^ T.flip;
+          const int64_t la = (int64_t)A.l21[y & A.irght];
+          const int64_t lb = (int64_t)A.l22[y >> A.nlow];
+          bool act = term_active(T, x) && (la != 0) && (lb != 0);
+          const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
+          uint64_t idx = (uint64_t)(la + lb - 3);
+          act = act && idx < A.dim;
+          idx = act ? idx : j;
+          g[u] = v1[idx * K + vec];
+          cr[u] = act ? s * T.cre : 0.0;
+          ci[u] = act ? s * T.cim : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < HV_UNROLL; u++) {
+          ar += cr[u] * g[u].x - ci[u] * g[u].y;
+          ai += cr[u] * g[u].y + ci[u] * g[u].x;
+        }
+      }
+    } else {
+      for (int t = 0; t < A.n_low; t += HV_UNROLL) {
+        double2 g[HV_UNROLL];
+        double cr[HV_UNROLL], ci[HV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < HV_UNROLL; u++) {
+          const GTerm T = A.terms[t + u];
+          bool act = term_active(T, x);
+          const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
+          uint64_t idx = x ^ T.flip;
+          act = act && idx < A.dim;
+          idx = act ? idx : j;
+          g[u] = v1[idx * K + vec];
+          cr[u] = act ? s * T.cre : 0.0;
+          ci[u] = act ? s * T.cim : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < HV_UNROLL; u++) {
+          ar += cr[u] * g[u].x - ci[u] * g[u].y;
+          ai += cr[u] * g[u].y + ci[u] * g[u].x;
+        }
+      }
+    }
+    double2 o = v0[j * K + vec];
+    o.x += ar;
+    o.y += ai;
+    v0[j * K + vec] = o;
+    // conj(a) * acc
+    pre += a.x * ar + a.y * ai;
+    pim += a.x * ai - a.y * ar;
+  }
+  wg_reduce_store<K>(pre, pim, A.partial);
+}
+
+// deterministic final reduction: out[v] = sum_w partial[w*K + v] in fixed order
+__global__ __launch_bounds__(256) void k_reduce_partials(const double2 *partial, int nblk, int K,
+                                                         double2 *out) {
+  __shared__ double2 s[256];
+  for (int v = 0; v < K; v++) {
+    double re = 0.0, im = 0.0;
+    for (int w = threadIdx.x; w < nblk; w += 256) {
+      re += partial[(uint64_t)w * K + v].x;
+      im += partial[(uint64_t)w * K + v].y;
+    }
+    s[threadIdx.x] = make_double2(re, im);
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+        s[threadIdx.x].x += s[threadIdx.x + st].x;
+        s[threadIdx.x].y += s[threadIdx.x + st].y;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[v] = s[0];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ vectors ----
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double unit_rng(uint64_t seed, uint64_t c) {
+  return (double)(splitmix64((seed * 0x9E3779B97F4A7C15ull) ^ c) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_fill_random(double2 *v, uint64_t dim, int k, uint64_t seed) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim * (uint64_t)k) return;
+  uint64_t j = i / k;
+  int c = (int)(i % k);
+  uint64_t s = seed + (uint64_t)c;
+  v[i] = make_double2(2.0 * unit_rng(s, 2 * j) - 1.0, 2.0 * unit_rng(s, 2 * j + 1) - 1.0);
+}
+
+__global__ void k_zero(double2 *v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = make_double2(0.0, 0.0);
+}
+
+// ====================================================================== host
+static int grid_for(uint64_t n, int tpb) { return (int)((n + tpb - 1) / tpb); }
+
+int hphi_engine_alloc_scratch(hphi_engine *e) {
+  if (e->d_partial) return 0;
+  HIPCHK(hipMalloc(&e->d_partial, sizeof(double2) * (size_t)HV_MAX_GRID * 16));
+  HIPCHK(hipMalloc(&e->d_prdct, sizeof(double2) * 16));
+  HIPCHK(hipMalloc(&e->d_coef, sizeof(double2) * 64));
+  HIPCHK(hipHostMalloc(&e->h_prdct, sizeof(double2) * 16, hipHostMallocDefault));
+  e->bytes += sizeof(double2) * ((size_t)HV_MAX_GRID * 16 + 16);
+  return 0;
+}
+
+static void classify_and_upload_terms(hphi_engine *e) {
+  // groups: low (flip within the low half), high (within the high half), mixed
+  std::vector<GTerm> low, high, mixed;
+  for (const GTerm &t : e->h_terms) {
+    if (e->model == HPHI_MODEL_SPINGC) {
+      low.push_back(t);
+      continue;
+    }
+    bool lo = (t.flip & ~e->irght) == 0;
+    bool hi = (t.flip & e->irght) == 0;
+    if (lo) low.push_back(t);
+    else if (hi) high.push_back(t);
+    else mixed.push_back(t);
+  }
+  auto pad = [](std::vector<GTerm> &v) {
+    GTerm z{};
+    z.kind = P_EQ;
+    z.tmask = 0;
+    z.tval = 1;  // never active
+    while (v.size() % HV_UNROLL) v.push_back(z);
+  };
+  pad(low);
+  pad(high);
+  pad(mixed);
+  e->n_low = (int)low.size();
+  e->n_high = (int)high.size();
+  e->n_mixed = (int)mixed.size();
+  e->h_terms_dev.clear();
+  e->h_terms_dev.insert(e->h_terms_dev.end(), low.begin(), low.end());
+  e->h_terms_dev.insert(e->h_terms_dev.end(), high.begin(), high.end());
+  e->h_terms_dev.insert(e->h_terms_dev.end(), mixed.begin(), mixed.end());
+}
+
+extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
+  if (!d || !out) return hphi_set_error("hphi_engine_create: NULL argument");
+  *out = nullptr;
+  if (ensure_binom_table()) return -1;
+  hphi_engine *e = new hphi_engine();
+  e->model = d->model;
+  e->nsite = d->nsite;
+  e->nup = d->nup;
+  e->ndown = d->ndown;
+  const int N = d->nsite;
+  if (N < 2) {
+    delete e;
+    return hphi_set_error("Nsite must be at least 2");
+  }
+  switch (d->model) {
+    case HPHI_MODEL_SPIN:  // check.c:177, check.c:310-316, bitcalc.c:45-62
+      if (N > 64 || d->nup < 0 || d->nup > N) { delete e; return hphi_set_error("bad Spin sector"); }
+      e->nbits = N;
+      e->dim = h_binom(N, d->nup);
+      e->sdim = 1ull << (N / 2);
+      e->size21 = (N % 2) ? e->sdim * 2 + 2 : e->sdim + 2;
+      e->size22 = e->sdim + 2;
+      break;
+    case HPHI_MODEL_HUBBARD:  // check.c:114-118, check.c:301-307
+      if (N > 32 || d->nup < 0 || d->nup > N || d->ndown < 0 || d->ndown > N) {
+        delete e;
+        return hphi_set_error("bad Hubbard sector");
+      }
+      e->nbits = 2 * N;
+      e->dim = h_binom(N, d->nup) * h_binom(N, d->ndown);
+      e->sdim = 1ull << N;
+      e->size21 = e->sdim + 2;
+      e->size22 = e->sdim + 2;
+      break;
+    case HPHI_MODEL_SPINGC:
+      if (N > 40) { delete e; return hphi_set_error("SpinGC Nsite too large"); }
+      e->nbits = N;
+      e->dim = 1ull << N;
+      e->sdim = 0;
+      e->size21 = e->size22 = 0;
+      break;
+    default:
+      delete e;
+      return hphi_set_error("unsupported CalcModel %d (supported: 0 Hubbard, 1 Spin, 4 SpinGC)", d->model);
+  }
+  e->nlow = (e->nbits + 1) / 2;
+  e->ihfbit = 1ull << e->nlow;
+  e->irght = e->ihfbit - 1;
+  e->ilft = ((e->nbits >= 64) ? ~0ull : ((1ull << e->nbits) - 1)) ^ e->irght;
+  if (e->dim == 0) { delete e; return hphi_set_error("empty sector"); }
+
+  // ---- off-diagonal terms (gather form) ----
+  const int gc = (d->model == HPHI_MODEL_SPINGC);
+  for (int t = 0; t < d->n_exchange; t++) {  // mltplySpinCore.c:273-345
+    int i = d->exchange_ij[2 * t], j = d->exchange_ij[2 * t + 1];
+    if (d->model == HPHI_MODEL_HUBBARD) {
+      delete e;
+      return hphi_set_error("Hubbard Exchange terms are not supported by this engine");
+    }
+    if (i < 0 || j < 0 || i >= N || j >= N || i == j) { delete e; return hphi_set_error("bad Exchange sites"); }
+    GTerm g{};
+    g.flip = (1ull << i) | (1ull << j);
+    g.tmask = g.flip;
+    g.kind = P_DIFF;
+    g.cre = d->exchange_J[t];
+    e->h_terms.push_back(g);
+  }
+  for (int t = 0; t < d->n_pairlift; t++) {  // mltplySpinCore.c:352-382
+    if (!gc) { delete e; return hphi_set_error("PairLift is active only in SpinGC"); }
+    int i = d->pairlift_ij[2 * t], j = d->pairlift_ij[2 * t + 1];
+    if (i < 0 || j < 0 || i >= N || j >= N || i == j) { delete e; return hphi_set_error("bad PairLift sites"); }
+    GTerm g{};
+    g.flip = (1ull << i) | (1ull << j);
+    g.tmask = g.flip;
+    g.kind = P_SAME;
+    g.cre = d->pairlift_J[t];
+    e->h_terms.push_back(g);
+  }
+  for (int t = 0; t < d->n_transfer; t++) {
+    int i1 = d->transfer_idx[4 * t], s1 = d->transfer_idx[4 * t + 1];
+    int i2 = d->transfer_idx[4 * t + 2], s2 = d->transfer_idx[4 * t + 3];
+    double tre = d->transfer_t[2 * t], tim = d->transfer_t[2 * t + 1];
+    if (i1 < 0 || i2 < 0 || i1 >= N || i2 >= N || s1 < 0 || s1 > 1 || s2 < 0 || s2 > 1) {
+      delete e;
+      return hphi_set_error("bad transfer indices");
+    }
+    GTerm g{};
+    if (d->model == HPHI_MODEL_HUBBARD) {
+      // CisAjt (mltplyHubbardCore.c:305-347): c+_a c_b with a = 2 i1 + s1,
+      // b = 2 i2 + s2; target has bit a set and bit b clear; sign from the
+      // bits strictly between a and b (general_hopp_GetInfo, :34-69)
+      int a = 2 * i1 + s1, b = 2 * i2 + s2;
+      if (a == b) continue;  // diagonal: EDChemi, not a transfer
+      if (s1 != s2) {
+        hphi_engine_destroy(e);
+        return hphi_set_error("spin-flip transfer (%d,%d,%d,%d) breaks the canonical Hubbard sector", i1, s1, i2, s2);
+      }
+      int lo = std::min(a, b), hi = std::max(a, b);
+      g.flip = (1ull << a) | (1ull << b);
+      g.tmask = g.flip;
+      g.tval = 1ull << a;
+      g.kind = P_EQ;
+      g.sgn = (hi - lo > 1) ? (((1ull << hi) - 1) & ~((1ull << (lo + 1)) - 1)) : 0ull;
+      g.cre = -tre;
+      g.cim = -tim;
+    } else if (gc) {
+      // child_SpinGC_CisAit (mltplySpinCore.c:203-227): source has bit == s2,
+      // target bit == s1; coefficient -t (mltplySpin.c:547)
+      if (i1 != i2 || s1 == s2) continue;  // the reference ignores these (mltplySpin.c:553-556)
+      g.flip = 1ull << i1;
+      g.tmask = g.flip;
+      g.tval = s1 ? g.flip : 0ull;
+      g.kind = P_EQ;
+      g.cre = -tre;
+      g.cim = -tim;
+    } else {
+      continue;  // canonical spin ignores off-diagonal transfers (mltplySpin.c:192-293)
+    }
+    e->h_terms.push_back(g);
+  }
+  classify_and_upload_terms(e);
+
+  // ---- diagonal terms, in diagonalcalc order ----
+  auto hubbard = d->model == HPHI_MODEL_HUBBARD;
+  for (int t = 0; t < d->n_intra; t++) {
+    if (!hubbard) break;
+    int s = d->intra_site[t];
+    DTerm x{};
+    x.kind = D_INTRA;
+    x.m1 = (1ull << (2 * s)) | (1ull << (2 * s + 1));
+    x.V = d->intra_U[t];
+    e->h_dterms.push_back(x);
+  }
+  for (int t = 0; t < d->n_chemi; t++) {
+    int s = d->chemi_site[t], sp = d->chemi_spin[t];
+    DTerm x{};
+    x.V = -d->chemi_mu[t];
+    if (hubbard) {
+      x.kind = D_CHEMI_H;
+      x.m1 = 1ull << (2 * s + sp);
+    } else {
+      x.kind = D_CHEMI_S;
+      x.m1 = 1ull << s;
+      x.m2 = (uint64_t)(1 - sp);
+    }
+    e->h_dterms.push_back(x);
+  }
+  for (int t = 0; t < d->n_inter; t++) {
+    int s1 = d->inter_ij[2 * t], s2 = d->inter_ij[2 * t + 1];
+    DTerm x{};
+    x.V = d->inter_V[t];
+    if (hubbard) {
+      x.kind = D_INTER_H;
+      x.m1 = 1ull << (2 * s1);
+      x.m2 = 1ull << (2 * s1 + 1);
+      x.m3 = 1ull << (2 * s2);
+      x.m4 = 1ull << (2 * s2 + 1);
+    } else {
+      x.kind = D_INTER_S;
+    }
+    e->h_dterms.push_back(x);
+  }
+  for (int t = 0; t < d->n_hund; t++) {
+    int s1 = d->hund_ij[2 * t], s2 = d->hund_ij[2 * t + 1];
+    DTerm x{};
+    x.V = -d->hund_J[t];
+    if (hubbard) {
+      x.kind = D_HUND_H;
+      x.m1 = 1ull << (2 * s1);
+      x.m2 = 1ull << (2 * s1 + 1);
+      x.m3 = 1ull << (2 * s2);
+      x.m4 = 1ull << (2 * s2 + 1);
+    } else {
+      x.kind = D_HUND_S;
+      x.m1 = (1ull << s1) | (1ull << s2);
+    }
+    e->h_dterms.push_back(x);
+  }
+
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return hphi_set_error("hipStreamCreate failed");
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) e->num_cu = prop.multiProcessorCount;
+  size_t tb = std::max<size_t>(e->h_terms_dev.size(), 1) * sizeof(GTerm);
+  size_t db = std::max<size_t>(e->h_dterms.size(), 1) * sizeof(DTerm);
+  if (hipMalloc(&e->d_terms, tb) != hipSuccess || hipMalloc(&e->d_dterms, db) != hipSuccess) {
+    hphi_engine_destroy(e);
+    return hphi_set_error("hipMalloc(terms) failed");
+  }
+  if (!e->h_terms_dev.empty())
+    (void)hipMemcpy(e->d_terms, e->h_terms_dev.data(), e->h_terms_dev.size() * sizeof(GTerm), hipMemcpyHostToDevice);
+  if (!e->h_dterms.empty())
+    (void)hipMemcpy(e->d_dterms, e->h_dterms.data(), e->h_dterms.size() * sizeof(DTerm), hipMemcpyHostToDevice);
+  e->bytes += tb + db;
+  if (hphi_engine_alloc_scratch(e)) {
+    hphi_engine_destroy(e);
+    return -1;
+  }
+  *out = e;
+  return 0;
+}
+
+extern "C" void hphi_engine_destroy(hphi_engine *e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  (void)hipFree(e->d_list1);
+  (void)hipFree(e->d_l21);
+  (void)hipFree(e->d_l22);
+  (void)hipFree(e->d_jb);
+  (void)hipFree(e->d_diag);
+  (void)hipFree(e->d_terms);
+  (void)hipFree(e->d_dterms);
+  (void)hipFree(e->d_partial);
+  (void)hipFree(e->d_prdct);
+  (void)hipFree(e->d_coef);
+  (void)hipFree(e->d_stage0);
+  (void)hipFree(e->d_stage1);
+  if (e->h_prdct) (void)hipHostFree(e->h_prdct);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+extern "C" int hphi_engine_info(const hphi_engine *e, uint64_t *dim, uint64_t *s21, uint64_t *s22,
+                                uint64_t *irght, uint64_t *ilft, uint64_t *ihfbit) {
+  if (!e) return hphi_set_error("NULL engine");
+  if (dim) *dim = e->dim;
+  if (s21) *s21 = e->size21;
+  if (s22) *s22 = e->size22;
+  if (irght) *irght = e->irght;
+  if (ilft) *ilft = e->ilft;
+  if (ihfbit) *ihfbit = e->ihfbit;
+  return 0;
+}
+
+extern "C" uint64_t hphi_engine_device_bytes(const hphi_engine *e) { return e ? e->bytes : 0; }
+
+static SzParams sz_params(const hphi_engine *e) {
+  SzParams P{};
+  P.hubbard = e->model == HPHI_MODEL_HUBBARD;
+  P.nlow = e->nlow;
+  P.nhigh = e->nbits - e->nlow;
+  P.nup = e->nup;
+  P.ndn = e->ndown;
+  P.sdim = e->sdim;
+  P.ihfbit = e->ihfbit;
+  P.size21 = e->size21;
+  P.dim = e->dim;
+  // Hubbard: up positions are even absolute bits; the low half is bits [0, nlow)
+  P.nlu = (e->nlow + 1) / 2;
+  P.nld = e->nlow / 2;
+  P.nhu = e->nsite - P.nlu;
+  P.nhd = e->nsite - P.nld;
+  return P;
+}
+
+int hphi_build_tables(hphi_engine *e) {
+  if (e->tables_ready) return 0;
+  if (e->model == HPHI_MODEL_SPINGC) {
+    e->tables_ready = true;
+    return 0;
+  }
+  SzParams P = sz_params(e);
+  HIPCHK(hipMalloc(&e->d_list1, sizeof(uint64_t) * e->dim));
+  HIPCHK(hipMalloc(&e->d_l21, sizeof(uint32_t) * e->size21));
+  HIPCHK(hipMalloc(&e->d_l22, sizeof(uint64_t) * e->size22));
+  HIPCHK(hipMalloc(&e->d_jb, sizeof(uint64_t) * (e->sdim + 1)));
+  e->bytes += sizeof(uint64_t) * (e->dim + e->size22 + e->sdim + 1) + sizeof(uint32_t) * e->size21;
+  hipLaunchKernelGGL(k_block_prefix, dim3(grid_for(e->sdim + 1, 256)), dim3(256), 0, e->stream, P,
+                     e->d_jb, e->d_l22);
+  hipLaunchKernelGGL(k_list21, dim3(grid_for(e->size21, 256)), dim3(256), 0, e->stream, P, e->d_l21);
+  int g = (int)std::min<uint64_t>(e->sdim, 1u << 20);
+  hipLaunchKernelGGL(k_list1, dim3(g), dim3(256), 0, e->stream, P, e->d_jb, e->d_list1);
+  HIPCHK(hipGetLastError());
+  uint64_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, e->d_jb + e->sdim, sizeof total, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (total != e->dim)  // sz.c:810-820
+    return hphi_set_error("basis count %llu differs from idim_max %llu", (unsigned long long)total,
+                          (unsigned long long)e->dim);
+  e->tables_ready = true;
+  return 0;
+}
+
+extern "C" int hphi_sz(hphi_engine *e, uint64_t *list_1, uint64_t *list_2_1, uint64_t *list_2_2) {
+  if (!e) return hphi_set_error("NULL engine");
+  if (hphi_build_tables(e)) return -1;
+  if (e->model == HPHI_MODEL_SPINGC) return 0;
+  if (list_1) {
+    list_1[0] = 0;
+    HIPCHK(hipMemcpy(list_1 + 1, e->d_list1, sizeof(uint64_t) * e->dim, hipMemcpyDeviceToHost));
+  }
+  if (list_2_1) {
+    std::vector<uint32_t> t(e->size21);
+    HIPCHK(hipMemcpy(t.data(), e->d_l21, sizeof(uint32_t) * e->size21, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < e->size21; i++) list_2_1[i] = t[i];
+  }
+  if (list_2_2)
+    HIPCHK(hipMemcpy(list_2_2, e->d_l22, sizeof(uint64_t) * e->size22, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int hphi_build_diag(hphi_engine *e) {
+  if (e->diag_ready) return 0;
+  if (hphi_build_tables(e)) return -1;
+  HIPCHK(hipMalloc(&e->d_diag, sizeof(double) * e->dim));
+  e->bytes += sizeof(double) * e->dim;
+  hipLaunchKernelGGL(k_diag, dim3(grid_for(e->dim, 256)), dim3(256), 0, e->stream, e->dim, e->d_list1,
+                     e->model == HPHI_MODEL_SPINGC ? 1 : 0, e->d_dterms, (int)e->h_dterms.size(), e->d_diag);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->diag_ready = true;
+  return 0;
+}
+
+extern "C" int hphi_diagonalcalc(hphi_engine *e, double *list_Diagonal) {
+  if (!e) return hphi_set_error("NULL engine");
+  if (hphi_build_diag(e)) return -1;
+  if (list_Diagonal) {
+    list_Diagonal[0] = 0.0;
+    HIPCHK(hipMemcpy(list_Diagonal + 1, e->d_diag, sizeof(double) * e->dim, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+HvArgs hphi_hv_args(hphi_engine *e, int k, double *v0, const double *v1) {
+  HvArgs A{};
+  A.dim = e->dim;
+  A.list1 = e->d_list1;
+  A.l21 = e->d_l21;
+  A.l22 = e->d_l22;
+  A.diag = e->d_diag;
+  A.terms = e->d_terms;
+  A.n_low = e->n_low;
+  A.n_high = e->n_high;
+  A.n_mixed = e->n_mixed;
+  A.irght = e->irght;
+  A.nlow = e->nlow;
+  A.v0 = v0;
+  A.v1 = v1;
+  A.partial = (double2 *)e->d_partial;
+  A.xcd_map = e->xcd_map;
+  (void)k;
+  return A;
+}
+
+int hphi_hv_grid(const hphi_engine *e, int k) {
+  uint64_t spt = HV_TPB / k;
+  uint64_t ntiles = (e->dim + spt - 1) / spt;
+  uint64_t g = std::min<uint64_t>(ntiles, (uint64_t)e->grid_per_cu * std::max(e->num_cu, 1));
+  g = std::max<uint64_t>(g, 8);
+  g = (g + 7) / 8 * 8;
+  if (g > HV_MAX_GRID) g = HV_MAX_GRID;
+  return (int)g;
+}
+
+// launches H·v and the final reduction on e->stream; result in e->d_prdct[0..k)
+int hphi_hv_launch(hphi_engine *e, int k, double *v0, const double *v1) {
+  HvArgs A = hphi_hv_args(e, k, v0, v1);
+  int g = hphi_hv_grid(e, k);
+  bool canon = e->model != HPHI_MODEL_SPINGC;
+#define LAUNCH(KK)                                                                                   \
+  if (canon) hipLaunchKernelGGL((k_hv<KK, true>), dim3(g), dim3(HV_TPB), 0, e->stream, A);          \
+  else hipLaunchKernelGGL((k_hv<KK, false>), dim3(g), dim3(HV_TPB), 0, e->stream, A);
+  switch (k) {
+    case 1: LAUNCH(1); break;
+    case 2: LAUNCH(2); break;
+    case 4: LAUNCH(4); break;
+    case 8: LAUNCH(8); break;
+    case 16: LAUNCH(16); break;
+    default: return hphi_set_error("block size k=%d not in {1,2,4,8,16}", k);
+  }
+#undef LAUNCH
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, e->stream, (const double2 *)e->d_partial,
+                     g, k, (double2 *)e->d_prdct);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int ensure_ready(hphi_engine *e) {
+  if (hphi_build_tables(e)) return -1;
+  if (hphi_build_diag(e)) return -1;
+  return 0;
+}
+
+int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct) {
+  HIPCHK(hipMemcpyAsync(e->h_prdct, e->d_prdct, sizeof(double2) * k, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (prdct) memcpy(prdct, e->h_prdct, sizeof(double2) * k);
+  return 0;
+}
+
+extern "C" int hphi_mltply_dev(hphi_engine *e, double *v0, const double *v1, double *prdct) {
+  if (!e || !v0 || !v1) return hphi_set_error("hphi_mltply_dev: NULL argument");
+  if (ensure_ready(e)) return -1;
+  if (hphi_hv_launch(e, 1, v0, v1)) return -1;
+  return hphi_fetch_prdct(e, 1, prdct);
+}
+
+extern "C" int hphi_mltply_block_dev(hphi_engine *e, int k, double *V0, const double *V1, double *prdct) {
+  if (!e || !V0 || !V1) return hphi_set_error("hphi_mltply_block_dev: NULL argument");
+  if (ensure_ready(e)) return -1;
+  if (hphi_hv_launch(e, k, V0, V1)) return -1;
+  return hphi_fetch_prdct(e, k, prdct);
+}
+
+static bool is_device_ptr(const void *p) {
+  hipPointerAttribute_t at;
+  hipError_t err = hipPointerGetAttributes(&at, p);
+  if (err != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice;
+}
+
+extern "C" int hphi_mltply(hphi_engine *e, double *v0, const double *v1, double *prdct) {
+  if (!e || !v0 || !v1) return hphi_set_error("hphi_mltply: NULL argument");
+  if (ensure_ready(e)) return -1;
+  bool d0 = is_device_ptr(v0), d1 = is_device_ptr(v1);
+  if (d0 && d1) return hphi_mltply_dev(e, v0 + 2, v1 + 2, prdct);
+  // compatibility path: stage the 1-indexed host arrays through the device
+  size_t bytes = sizeof(double2) * e->dim;
+  if (!e->d_stage0) {
+    HIPCHK(hipMalloc(&e->d_stage0, bytes));
+    HIPCHK(hipMalloc(&e->d_stage1, bytes));
+    e->bytes += 2 * bytes;
+  }
+  HIPCHK(hipMemcpyAsync(e->d_stage1, v1 + 2, bytes, d1 ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(e->d_stage0, v0 + 2, bytes, d0 ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, e->stream));
+  if (hphi_hv_launch(e, 1, e->d_stage0, e->d_stage1)) return -1;
+  HIPCHK(hipMemcpyAsync(v0 + 2, e->d_stage0, bytes, d0 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e->stream));
+  return hphi_fetch_prdct(e, 1, prdct);
+}
+
+// ----------------------------------------------------------- helpers ----
+extern "C" int hphi_dev_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return hphi_set_error("NULL ptr");
+  HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+  return 0;
+}
+extern "C" int hphi_dev_free(void *ptr) {
+  HIPCHK(hipFree(ptr));
+  return 0;
+}
+extern "C" int hphi_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+extern "C" int hphi_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+extern "C" int hphi_dev_zero(void *ptr, size_t bytes) {
+  uint64_t n = bytes / sizeof(double2);
+  if (n) hipLaunchKernelGGL(k_zero, dim3(grid_for(n, 256)), dim3(256), 0, 0, (double2 *)ptr, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+extern "C" int hphi_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+extern "C" int hphi_fill_random(double *v, uint64_t dim, int k, uint64_t seed) {
+  if (!v || k < 1) return hphi_set_error("hphi_fill_random: bad argument");
+  uint64_t n = dim * (uint64_t)k;
+  hipLaunchKernelGGL(k_fill_random, dim3(grid_for(n, 256)), dim3(256), 0, 0, (double2 *)v, dim, k, seed);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+
+extern "C" int hphi_time_mltply_dev(hphi_engine *e, int k, double *v0, const double *v1, int warmup,
+                                    int reps, double *avg_ms) {
+  if (!e || !v0 || !v1 || reps < 1) return hphi_set_error("hphi_time_mltply_dev: bad argument");
+  if (ensure_ready(e)) return -1;
+  for (int i = 0; i < warmup; i++)
+    if (hphi_hv_launch(e, k, v0, v1)) return -1;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, e->stream));
+  for (int i = 0; i < reps; i++)
+    if (hphi_hv_launch(e, k, v0, v1)) return -1;
+  HIPCHK(hipEventRecord(b, e->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *avg_ms = ms / reps;
+  return 0;
+}

@@ -1,0 +1,86 @@
+// Internal types of the hphi_amd engine (device term tables, kernel argument
+// blocks, engine state).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "hphi_amd.h"
+
+#define HV_TPB 256        // threads per workgroup of the H·v kernels
+#define HV_UNROLL 8       // terms per unrolled gather batch (term groups are padded to it)
+#define HV_MAX_GRID 65536 // upper bound of H·v workgroups (sizes the partial-sum buffer)
+
+// predicate kinds of a gather term, evaluated on the TARGET state x
+enum { P_EQ = 0,    // (x & tmask) == tval            (hopping, transverse field)
+       P_DIFF = 1,  // popcount(x & tmask) == 1       (exchange: the two spins differ)
+       P_SAME = 2   // popcount(x & tmask) != 1       (pair lift: the two spins agree)
+};
+
+// one off-diagonal term in gather form: source = x ^ flip, weight
+// (cre + i cim) * (-1)^popcount(x & sgn) when the predicate holds
+struct GTerm {
+  uint64_t flip, tmask, tval, sgn;
+  double cre, cim;
+  int kind, pad;
+};
+
+// diagonal term kinds (diagonalcalc.c set functions, per model)
+enum { D_INTRA = 0, D_CHEMI_H, D_CHEMI_S, D_INTER_H, D_INTER_S, D_HUND_H, D_HUND_S };
+
+struct DTerm {
+  int kind, pad;
+  uint64_t m1, m2, m3, m4;
+  double V;
+};
+
+struct SzParams {
+  int hubbard, nlow, nhigh, nup, ndn;
+  int nlu, nld, nhu, nhd;  // Hubbard: up/down positions in the low / high half
+  uint64_t sdim, ihfbit, size21, dim;
+};
+
+struct HvArgs {
+  uint64_t dim;
+  const uint64_t *list1;
+  const uint32_t *l21;
+  const uint64_t *l22;
+  const double *diag;
+  const GTerm *terms;
+  int n_low, n_high, n_mixed;
+  int nlow;
+  uint64_t irght;
+  double *v0;
+  const double *v1;
+  double2 *partial;
+  int xcd_map;
+};
+
+struct hphi_engine {
+  int model = 0, nsite = 0, nup = 0, ndown = 0, nbits = 0, nlow = 0;
+  uint64_t dim = 0, sdim = 0, size21 = 0, size22 = 0, irght = 0, ilft = 0, ihfbit = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 256;
+  int grid_per_cu = 8;  // H·v workgroups per CU (tunable)
+  int xcd_map = 1;      // contiguous tile ranges per XCD (tunable)
+  bool tables_ready = false, diag_ready = false;
+  uint64_t *d_list1 = nullptr, *d_l22 = nullptr, *d_jb = nullptr;
+  uint32_t *d_l21 = nullptr;
+  double *d_diag = nullptr;
+  GTerm *d_terms = nullptr;
+  DTerm *d_dterms = nullptr;
+  std::vector<GTerm> h_terms, h_terms_dev;
+  std::vector<DTerm> h_dterms;
+  int n_low = 0, n_high = 0, n_mixed = 0;
+  void *d_partial = nullptr, *d_prdct = nullptr, *h_prdct = nullptr, *d_coef = nullptr;
+  double *d_stage0 = nullptr, *d_stage1 = nullptr;
+  uint64_t bytes = 0;
+};
+
+extern "C" int hphi_set_error(const char *fmt, ...);
+int hphi_build_tables(hphi_engine *e);
+int hphi_build_diag(hphi_engine *e);
+int hphi_hv_launch(hphi_engine *e, int k, double *v0, const double *v1);
+int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct);
+int hphi_hv_grid(const hphi_engine *e, int k);

@@ -1,0 +1,167 @@
+/*
+ * hphi_amd — MI355X-native matrix-free H·v engine for HPhi, C ABI.
+ *
+ * Plain C types only (pointers, sizes, doubles); complex128 data is passed as
+ * interleaved (re, im) doubles, layout-compatible with C99 `double complex`.
+ * Every entry point returns 0 on success and -1 on failure; the message is
+ * available from hphi_last_error().  No entry point aborts the process.
+ *
+ * Reference interfaces replaced (paths relative to the HPhi source tree):
+ *   hphi_sz            <- int sz(struct BindStruct*, long unsigned int *list_1,
+ *                          long unsigned int *list_2_1, long unsigned int *list_2_2)
+ *                          src/include/sz.h:20, src/sz.c:62
+ *   hphi_diagonalcalc  <- int diagonalcalc(struct BindStruct*)
+ *                          src/include/diagonalcalc.h:22, src/diagonalcalc.c:84
+ *   hphi_mltply        <- int mltply(struct BindStruct*, double complex *tmp_v0,
+ *                          double complex *tmp_v1)   src/include/mltply.h:22, src/mltply.c:56
+ *   hphi_engine_create_from_def
+ *                      <- ReadDefFileNInt + ReadDefFileIdxPara (src/readdef.c:463,1006)
+ *                          followed by check() (src/check.c:51) for the sizes
+ *   hphi_lanczos       <- CalcByLanczos -> Lanczos_EigenValue (src/CalcByLanczos.c:55,
+ *                          src/Lanczos_EigenValue.c:50), eigenvalues only
+ *   hphi_tpq           <- CalcByTPQ (src/CalcByTPQ.c:52) step loop with
+ *                          Multiply (src/Multiply.c:42) and expec_energy_flct
+ *                          (src/expec_energy_flct.c:34), k samples at once
+ *   hphi_lobpcg        <- CalcByLOBPCG (src/CalcByLOBPCG.c:328), eigenvalues only
+ */
+#ifndef HPHI_AMD_H
+#define HPHI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* calcmod.def CalcModel values (src/include/DefCommon.h:22-37) */
+#define HPHI_MODEL_HUBBARD 0
+#define HPHI_MODEL_SPIN 1
+#define HPHI_MODEL_SPINGC 4
+
+typedef struct hphi_engine hphi_engine;
+
+/* Term tables in the order the reference holds them after ReadDefFileIdxPara
+ * (struct DefineList, src/include/struct.h:43-310).  Sites are 0-based.
+ *   exchange  ExchangeCoupling[n][2], ParaExchangeCoupling[n]
+ *   pairlift  PairLiftCoupling[n][2], ParaPairLiftCoupling[n]       (SpinGC)
+ *   transfer  EDGeneralTransfer[n][4] (i, s1, j, s2), EDParaGeneralTransfer[n]
+ *             as (re, im); term is -t c+_{i s1} c_{j s2}
+ *   chemi     EDChemi[n], EDSpinChemi[n], EDParaChemi[n]
+ *   intra     CoulombIntra[n], ParaCoulombIntra[n]
+ *   inter     CoulombInter[n][2], ParaCoulombInter[n]   (Ising part included)
+ *   hund      HundCoupling[n][2], ParaHundCoupling[n]   (Ising part included)
+ * nup/ndown: Spin — nup = number of 1-bits (X->Def.Ne); Hubbard — electrons
+ * per spin (X->Def.Nup, X->Def.Ndown); SpinGC — ignored. */
+typedef struct {
+  int model, nsite, nup, ndown;
+  int n_exchange;  const int *exchange_ij;  const double *exchange_J;
+  int n_pairlift;  const int *pairlift_ij;  const double *pairlift_J;
+  int n_transfer;  const int *transfer_idx; const double *transfer_t;
+  int n_chemi;     const int *chemi_site;   const int *chemi_spin; const double *chemi_mu;
+  int n_intra;     const int *intra_site;   const double *intra_U;
+  int n_inter;     const int *inter_ij;     const double *inter_V;
+  int n_hund;      const int *hund_ij;      const double *hund_J;
+} hphi_model_desc;
+
+/* Modelling parameters read from modpara.def (src/readdef.c:555-650). */
+typedef struct {
+  int calc_type, lanczos_max, initial_iv, exct, lanczos_eps, lanczos_target, num_ave,
+      expec_interval;
+  double large_value;
+} hphi_modpara;
+
+const char *hphi_last_error(void);
+
+/* ---- engine ------------------------------------------------------------ */
+int hphi_engine_create(const hphi_model_desc *desc, hphi_engine **out);
+/* Reads an expert-mode namelist.def and the files it names (subset: CalcMod,
+ * ModPara, LocSpin, Trans, CoulombIntra, CoulombInter, Hund, Exchange, Ising,
+ * PairLift); modpara may be NULL. */
+int hphi_engine_create_from_def(const char *namelist_path, hphi_engine **out, hphi_modpara *modpara);
+void hphi_engine_destroy(hphi_engine *e);
+/* Host-only parse of an expert-mode input into a descriptor (no device work);
+ * release it with hphi_model_desc_free. */
+int hphi_read_def(const char *namelist_path, hphi_model_desc **desc, hphi_modpara *modpara);
+void hphi_model_desc_free(hphi_model_desc *desc);
+/* dim = idim_max; size21/size22 = lengths of list_2_1/list_2_2 (xsetmem.c:284-318);
+ * irght/ilft/ihfbit = split masks (bitcalc.c:45-62). */
+int hphi_engine_info(const hphi_engine *e, uint64_t *dim, uint64_t *size21, uint64_t *size22,
+                     uint64_t *irght, uint64_t *ilft, uint64_t *ihfbit);
+/* HBM bytes the engine holds (tables, diagonal, scratch). */
+uint64_t hphi_engine_device_bytes(const hphi_engine *e);
+
+/* ---- basis, diagonal, H·v (drop-in for sz / diagonalcalc / mltply) ----- */
+/* Builds list_1 / list_2_1 / list_2_2 on the device (bit-exact with sz()).
+ * Host arrays, when non-NULL, receive the reference-shaped copies:
+ * list_1[0..dim] (1-indexed, [0] = 0), list_2_1[0..size21-1], list_2_2[0..size22-1]. */
+int hphi_sz(hphi_engine *e, uint64_t *list_1, uint64_t *list_2_1, uint64_t *list_2_2);
+/* Precomputes list_Diagonal on the device (bit-exact with diagonalcalc());
+ * a non-NULL host array receives list_Diagonal[0..dim] ([0] = 0). */
+int hphi_diagonalcalc(hphi_engine *e, double *list_Diagonal);
+/* v0[1..dim] += H v1[1..dim], prdct[2] = <v1|H|v1>.  v0/v1 are the reference's
+ * 1-indexed arrays of dim+1 complex128 and may be host or device memory
+ * (host memory is staged through the device: PCIe-bound).  Builds tables and
+ * diagonal on first use. */
+int hphi_mltply(hphi_engine *e, double *v0, const double *v1, double *prdct);
+/* Device-resident form: v0/v1 are 0-indexed device arrays of dim complex128. */
+int hphi_mltply_dev(hphi_engine *e, double *v0, const double *v1, double *prdct);
+/* Block form: k vectors stored state-major, V[(j*k + v)*2 + {0,1}], device
+ * memory; prdct[2k].  k in {1, 2, 4, 8, 16}. */
+int hphi_mltply_block_dev(hphi_engine *e, int k, double *V0, const double *V1, double *prdct);
+
+/* ---- device memory and vector helpers for host drivers ----------------- */
+int hphi_dev_alloc(size_t bytes, void **ptr);
+int hphi_dev_free(void *ptr);
+int hphi_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int hphi_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int hphi_dev_zero(void *ptr, size_t bytes);
+int hphi_synchronize(void);
+/* Synthetic vector: v[j*k+v] = (2u(2c)-1) + i(2u(2c+1)-1), c = j (k = 1) with
+ * u from SplitMix64 of seed (seed+v for column v of a block).  Device memory. */
+int hphi_fill_random(double *v, uint64_t dim, int k, uint64_t seed);
+
+/* Streaming kernels on the engine's stream for the host drivers; vectors are
+ * device arrays of dim (x k) complex128.  Results are copied to host memory.
+ *   lanczos_beta    beta2 = sum |v0 - alpha v1|^2      (Lanczos_EigenValue.c:170-177)
+ *   lanczos_update  v1 <- (v0 - alpha v1)/beta, v0 <- -beta v1_old  (:154-160)
+ *   norm2 / dot     per-column squared norms / <x_c|y_c> of [dim][k] blocks
+ *   axpby           y_c <- a_c x_c + b_c y_c, ab = {a_0, b_0, a_1, b_1, ...} complex */
+int hphi_vec_lanczos_beta(hphi_engine *e, const double *v0, const double *v1, double alpha, double *beta2);
+int hphi_vec_lanczos_update(hphi_engine *e, double *v0, double *v1, double alpha, double beta);
+int hphi_vec_norm2(hphi_engine *e, const double *v, int k, double *out);
+int hphi_vec_dot(hphi_engine *e, const double *x, const double *y, int k, double *out);
+int hphi_vec_axpby(hphi_engine *e, const double *x, double *y, int k, const double *ab);
+int hphi_engine_sync(hphi_engine *e);
+
+/* Times `reps` device H·v calls on the engine's stream with HIP events
+ * (after `warmup` untimed calls); avg_ms = mean per call. */
+int hphi_time_mltply_dev(hphi_engine *e, int k, double *v0, const double *v1, int warmup,
+                         int reps, double *avg_ms);
+
+/* ---- solvers (host C drivers, device-resident vectors) ----------------- */
+typedef struct {
+  int max_iter;      /* Lanczos_max */
+  int n_eigen;       /* number of lowest distinct eigenvalues wanted (exct) */
+  double eps;        /* relative convergence of the target eigenvalue (10^-LanczosEps) */
+  int target;        /* LanczosTarget (0-based index of the converged value tested) */
+  uint64_t seed;     /* initial vector seed */
+  int check_every;   /* tridiagonal solve interval (1 = every step, as the reference) */
+} hphi_lanczos_params;
+
+typedef struct {
+  double eigenvalues[16];
+  int n_found, iterations, converged;
+  double seconds, hv_seconds;
+} hphi_lanczos_result;
+
+int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r);
+/* k lowest eigenvalues (ascending) of the symmetric tridiagonal matrix with
+ * diagonal a[0..n) and off-diagonal b[0..n-1), by Sturm bisection (host only;
+ * replaces DSEVvalue -> dsyev_, src/matrixlapack.c:93-150). */
+int hphi_tridiag_lowest(const double *a, const double *b, int n, int k, double *ev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

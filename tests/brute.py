@@ -1,0 +1,105 @@
+"""Dense Hamiltonians from the textbook operator definitions (test helper).
+
+Independent of the reference's code structure: states are enumerated by
+brute force, and every operator is applied from its definition
+(doc/en/source/filespecification/expertmode_en/*_file_en.rst):
+
+* Exchange J:  J (S+_i S-_j + S-_i S+_j): couples states whose bits i, j differ
+* Ising J:     J Sz_i Sz_j with Sz = +-1/2
+* Trans t:     -t c+_{i s1} c_{j s2}; fermion order = bit index 2i+s, so
+               c+_a c_b |x> = (-1)^(#occupied strictly between a, b) |x'>
+* spin Trans:  on one site, c+_{i s1} c_{i s2} flips the bit s2 -> s1
+* CoulombIntra U: U n_up n_down
+Used only for small sizes.
+"""
+
+from __future__ import annotations
+
+from itertools import combinations
+from typing import List
+
+import numpy as np
+
+from hphi_amd import lattice as L
+
+
+def basis(spec) -> List[int]:
+    n = spec.nbits
+    if spec.model == L.SPIN_GC:
+        return list(range(1 << n))
+    out = []
+    for x in range(1 << n):
+        if spec.model == L.SPIN:
+            if bin(x).count("1") == spec.nup:
+                out.append(x)
+        else:
+            up = sum((x >> (2 * i)) & 1 for i in range(spec.nsite))
+            dn = sum((x >> (2 * i + 1)) & 1 for i in range(spec.nsite))
+            if up == spec.nup and dn == spec.ndown:
+                out.append(x)
+    return out
+
+
+def dense(spec) -> np.ndarray:
+    states = basis(spec)
+    index = {x: k for k, x in enumerate(states)}
+    D = len(states)
+    H = np.zeros((D, D), np.complex128)
+    for k, x in enumerate(states):
+        if spec.model in (L.SPIN, L.SPIN_GC):
+            for i, j, J in spec.exchange:
+                if ((x >> i) & 1) != ((x >> j) & 1):
+                    H[index[x ^ (1 << i) ^ (1 << j)], k] += J
+            for i, j, J in spec.ising:
+                si = 0.5 if (x >> i) & 1 else -0.5
+                sj = 0.5 if (x >> j) & 1 else -0.5
+                H[k, k] += J * si * sj
+            for i, s1, j, s2, re, im in spec.trans:
+                t = complex(re, im)
+                if i == j and s1 != s2:
+                    if ((x >> i) & 1) == s2:
+                        H[index[x ^ (1 << i)], k] += -t
+                elif i == j and s1 == s2:
+                    # longitudinal field: -t n_{i s1}, n_{i s} = [bit_i == s]
+                    if ((x >> i) & 1) == s1:
+                        H[k, k] += -t
+        else:
+            for i, s1, j, s2, re, im in spec.trans:
+                t = complex(re, im)
+                a, b = 2 * i + s1, 2 * j + s2
+                if a == b:
+                    if (x >> a) & 1:
+                        H[k, k] += -t
+                    continue
+                if not ((x >> b) & 1) or ((x >> a) & 1):
+                    continue
+                lo, hi = min(a, b), max(a, b)
+                between = bin(x & (((1 << hi) - 1) & ~((1 << (lo + 1)) - 1))).count("1")
+                y = x ^ (1 << a) ^ (1 << b)
+                if y in index:
+                    H[index[y], k] += -t * (-1) ** between
+            for i, U in spec.coulomb_intra:
+                if (x >> (2 * i)) & 1 and (x >> (2 * i + 1)) & 1:
+                    H[k, k] += U
+    return H
+
+
+def kron_xxz_gc(nsite, bonds, Jz, Jxy, hx) -> np.ndarray:
+    """SpinGC XXZ + transverse field from Pauli-matrix Kronecker products."""
+    sx = np.array([[0, 0.5], [0.5, 0]])
+    sy = np.array([[0, -0.5j], [0.5j, 0]])
+    sz = np.array([[0.5, 0], [0, -0.5]])
+
+    def op(o, i):
+        m = np.array([[1.0]])
+        for s in range(nsite):
+            m = np.kron(m, o if s == i else np.eye(2))
+        return m
+
+    H = np.zeros((1 << nsite, 1 << nsite), np.complex128)
+    for i, j in bonds:
+        H += Jz * op(sz, i) @ op(sz, j)
+        H += Jxy * (op(sx, i) @ op(sx, j) + op(sy, i) @ op(sy, j))
+    for i in range(nsite):
+        H += -hx * op(sx, i)
+    return H

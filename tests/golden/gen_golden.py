@@ -1,0 +1,122 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/ref_harness (this project's harness linked against the
+reference's own hot-path sources) on each case of cases.py and stores, per
+case, the reference's list_1 / list_2_1 / list_2_2 / list_Diagonal and one
+H·v of the SplitMix64 vector with SEED, plus prdct:
+
+* golden.json       dims, SHA-256 of every table, prdct, H·v checksums
+* <case>.npz        the full arrays for cases with dim <= FULL_MAX
+
+Full-size workloads (pass --large) store checksums only: S1 = sum v0,
+S2 = sum |v0|^2, S3 = sum ((j mod 97)+1) v0[j] and 64 sampled entries
+(the harness's `sums` mode).
+
+usage: python tests/golden/gen_golden.py [--large]
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from hphi_amd import lattice as L  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+from cases import SEED, golden_cases  # noqa: E402
+
+FULL_MAX = 5000
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def run_case(name, spec, work):
+    d = os.path.join(work, name)
+    L.write_expert(spec, d)
+    O.run_reference(d, "tables", d + "/out")
+    O.run_reference(d, "hv", d + "/out", seed=SEED)
+    meta = dict(line.split() for line in open(d + "/out/meta.txt"))
+    dim = int(meta["dim"])
+    rec = {"dim": dim, "model": spec.model, "nsite": spec.nsite, "seed": SEED}
+    arrays = {}
+    canon = spec.model in (L.SPIN, L.HUBBARD)
+    if canon:
+        for t in ("list_1", "list_2_1", "list_2_2"):
+            a = np.fromfile(f"{d}/out/{t}.bin", np.uint64)
+            rec[f"sha_{t}"] = sha(a)
+            rec[f"len_{t}"] = int(a.size)
+            arrays[t] = a
+    diag = np.fromfile(f"{d}/out/list_Diagonal.bin", np.float64)
+    rec["sha_list_Diagonal"] = sha(diag)
+    arrays["list_Diagonal"] = diag
+    v0 = np.fromfile(f"{d}/out/v0.bin", np.complex128)
+    pr = open(d + "/out/prdct.txt").readline().split()
+    rec["prdct"] = [float(pr[0]), float(pr[1])]
+    rec["hv_S1"] = [float(v0.sum().real), float(v0.sum().imag)]
+    rec["hv_S2"] = float(np.sum(np.abs(v0) ** 2))
+    rec["hv_max_abs"] = float(np.max(np.abs(v0)))
+    arrays["hv"] = v0
+    if dim <= FULL_MAX:
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **arrays)
+        rec["arrays"] = f"{name}.npz"
+    else:
+        idx = (np.arange(64, dtype=np.uint64) * np.uint64(2654435761)) % np.uint64(dim)
+        rec["hv_samples"] = [[int(j), float(v0[j].real), float(v0[j].imag)] for j in idx]
+    return rec
+
+
+def run_large(n, work):
+    spec = L.baseline_config(n)
+    d = os.path.join(work, f"cfg{n}")
+    L.write_expert(spec, d)
+    seed = {2: 1, 3: 2, 4: 0}[n]
+    O.run_reference(d, "sums", d + "/out", seed=seed, timeout=6 * 3600)
+    lines = open(d + "/out/sums.txt").read().split("\n")
+    pr = open(d + "/out/prdct.txt").read().split()
+    rec = {"dim": spec.dim, "seed": seed, "prdct": [float(pr[0]), float(pr[1])], "samples": []}
+    for ln in lines:
+        p = ln.split()
+        if not p:
+            continue
+        if p[0] == "S1":
+            rec["S1"] = [float(p[1]), float(p[2])]
+        elif p[0] == "S2":
+            rec["S2"] = float(p[1])
+        elif p[0] == "S3":
+            rec["S3"] = [float(p[1]), float(p[2])]
+        elif p[0] == "E":
+            rec["samples"].append([int(p[1]), float(p[2]), float(p[3])])  # 1-based index
+    return rec
+
+
+def main():
+    out_path = os.path.join(HERE, "golden.json")
+    gold = json.load(open(out_path)) if os.path.exists(out_path) else {}
+    with tempfile.TemporaryDirectory() as work:
+        if "--large" in sys.argv:
+            for n in [int(a) for a in sys.argv[sys.argv.index("--large") + 1:] if a.isdigit()] or [4, 2, 3]:
+                gold.setdefault("large", {})[f"cfg{n}"] = run_large(n, work)
+                json.dump(gold, open(out_path, "w"), indent=1)
+                print("large", n, "done", flush=True)
+        else:
+            cases = {}
+            for name, spec in golden_cases().items():
+                cases[name] = run_case(name, spec, work)
+                print(name, cases[name]["dim"], flush=True)
+            gold["cases"] = cases
+    json.dump(gold, open(out_path, "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+"""C ABI on the CPU: the library loads, exports every declared symbol, and its
+host-only parts (expert-mode reader, tridiagonal eigensolver) are correct.
+No device work here."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from hphi_amd import lattice as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from hphi_amd import capi as A
+    if not os.path.exists(A.LIB_PATH):
+        from hphi_amd import build
+        build.build()
+    return A
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "hphi_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(hphi_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(capi):
+    lib = capi.lib()
+    names = _declared()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(capi.EXPORTED) <= set(names)
+
+
+def _read(capi, path):
+    lib = capi.lib()
+    lib.hphi_read_def.argtypes = [C.c_char_p, C.POINTER(C.POINTER(capi.ModelDesc)), C.POINTER(capi.ModPara)]
+    lib.hphi_model_desc_free.argtypes = [C.POINTER(capi.ModelDesc)]
+    pd = C.POINTER(capi.ModelDesc)()
+    mp = capi.ModPara()
+    rc = lib.hphi_read_def(path.encode(), C.byref(pd), C.byref(mp))
+    return rc, pd, mp
+
+
+def _arr(ptr, n, w=1):
+    return np.array([ptr[i] for i in range(n * w)]).reshape(n, w) if n else np.zeros((0, w))
+
+
+@pytest.mark.parametrize("spec", [
+    L.baseline_config(1), L.hubbard_chain(5), L.hubbard_square(4, 2, ncond=8),
+    L.triangular_xxz_gc(4, 2), L.j1j2_square(4, 3, 1.0, 0.31, two_sz=2),
+], ids=lambda s: s.name)
+def test_def_reader_matches_python_mirror(capi, tmp_path, spec):
+    """The C reader reproduces readdef's term processing (merge, Hermitian
+    pairing, Ising split), cross-checked against lattice.processed_terms."""
+    nl = L.write_expert(spec, str(tmp_path))
+    rc, pd, mp = _read(capi, nl)
+    assert rc == 0, capi.lib().hphi_last_error()
+    d = pd.contents
+    t = L.processed_terms(spec)
+    assert (d.model, d.nsite, d.nup, d.ndown) == (spec.model, spec.nsite, spec.nup, spec.ndown)
+    assert np.array_equal(_arr(d.exchange_ij, d.n_exchange, 2), t["exchange_ij"].reshape(-1, 2))
+    assert np.array_equal(_arr(d.exchange_J, d.n_exchange)[:, 0], t["exchange_J"])
+    assert np.array_equal(_arr(d.transfer_idx, d.n_transfer, 4), t["trans_idx"].reshape(-1, 4))
+    assert np.array_equal(_arr(d.transfer_t, d.n_transfer, 2), t["trans_t"].reshape(-1, 2))
+    assert np.array_equal(_arr(d.chemi_mu, d.n_chemi)[:, 0] if d.n_chemi else np.zeros(0), t["chemi_mu"])
+    assert np.array_equal(_arr(d.intra_U, d.n_intra)[:, 0] if d.n_intra else np.zeros(0), t["intra_U"])
+    assert np.array_equal(_arr(d.inter_ij, d.n_inter, 2), t["inter_ij"].reshape(-1, 2))
+    assert np.array_equal(_arr(d.inter_V, d.n_inter)[:, 0] if d.n_inter else np.zeros(0), t["inter_V"])
+    assert np.array_equal(_arr(d.hund_ij, d.n_hund, 2), t["hund_ij"].reshape(-1, 2))
+    assert np.array_equal(_arr(d.hund_J, d.n_hund)[:, 0] if d.n_hund else np.zeros(0), t["hund_J"])
+    assert mp.exct == int(spec.params["exct"]) and mp.lanczos_max == int(spec.params["Lanczos_max"])
+    capi.lib().hphi_model_desc_free(pd)
+
+
+def test_def_reader_merges_and_pairs_transfers(capi, tmp_path):
+    spec = L.hubbard_chain(4)
+    spec.trans = spec.trans + [(0, 0, 1, 0, 0.25, 0.0), (1, 0, 0, 0, 0.25, 0.0), (2, 1, 2, 1, 0.3, 0.0)]
+    nl = L.write_expert(spec, str(tmp_path))
+    rc, pd, _ = _read(capi, nl)
+    assert rc == 0
+    d = pd.contents
+    t = L.processed_terms(spec)
+    assert d.n_transfer == len(t["trans_t"]) == 16
+    assert np.array_equal(_arr(d.transfer_t, d.n_transfer, 2), t["trans_t"])
+    assert d.n_chemi == 1 and d.chemi_site[0] == 2 and d.chemi_spin[0] == 1 and d.chemi_mu[0] == 0.3
+    capi.lib().hphi_model_desc_free(pd)
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda s: s.trans.append((0, 0, 1, 0, 1.0, 0.0)), "Hermitian"),
+    (lambda s: setattr(s, "two_sz", None), "2Sz"),
+])
+def test_def_reader_errors(capi, tmp_path, mutate, msg):
+    spec = L.hubbard_chain(4) if msg == "Hermitian" else L.heisenberg_chain(6)
+    mutate(spec)
+    if msg == "2Sz":
+        nl = L.write_expert(L.heisenberg_chain(6), str(tmp_path))
+        txt = open(os.path.join(str(tmp_path), "modpara.def")).read().replace("2Sz 0\n", "")
+        open(os.path.join(str(tmp_path), "modpara.def"), "w").write(txt)
+    else:
+        nl = L.write_expert(spec, str(tmp_path))
+    rc, _, _ = _read(capi, nl)
+    assert rc == -1
+    assert msg in capi.lib().hphi_last_error().decode()
+
+
+def test_def_reader_rejects_unsupported_keyword(capi, tmp_path):
+    nl = L.write_expert(L.heisenberg_chain(6), str(tmp_path))
+    with open(nl, "a") as f:
+        f.write("InterAll interall.def\n")
+    rc, _, _ = _read(capi, nl)
+    assert rc == -1 and "InterAll" in capi.lib().hphi_last_error().decode()
+
+
+def test_tridiagonal_eigensolver(capi):
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 3, 10, 200):
+        a = rng.normal(size=n)
+        b = np.abs(rng.normal(size=n - 1)) + 1e-3
+        T = np.diag(a) + np.diag(b, 1) + np.diag(b, -1)
+        k = min(n, 4)
+        ev = capi.tridiag_lowest(a, b, k)
+        assert np.allclose(ev, np.linalg.eigvalsh(T)[:k], atol=1e-13, rtol=0)
+    # degenerate pair
+    a = np.array([1.0, 1.0, 3.0])
+    b = np.array([0.0, 0.0])
+    assert np.allclose(capi.tridiag_lowest(a, b, 3), [1.0, 1.0, 3.0])

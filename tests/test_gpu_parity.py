@@ -1,0 +1,176 @@
+"""GPU parity: the HIP path through the C ABI against the oracle and the
+reference's golden fixtures.
+
+Bars: basis tables and list_Diagonal bit-exact; H·v within 1e-12 relative
+infinity-norm of the oracle (the gather form sums in another order than the
+reference's scatter, so it is not bit-identical); prdct within 1e-12
+relative; eigenvalues within 1e-10 of an independent solver."""
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from hphi_amd import lattice as L
+from tests.golden.cases import SEED, golden_cases
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+HV_TOL = 1e-12
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def A():
+    from hphi_amd import capi
+    capi.lib()
+    return capi
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return json.load(open(os.path.join(GOLD, "golden.json")))
+
+
+def rel_inf(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def prdct_ok(pr, opr, v, hv):
+    """prdct = <v|Hv> is a dot product: its rounding error scales with
+    |v| |Hv| (Cauchy-Schwarz), not with the possibly cancelled result."""
+    return abs(pr - opr) <= HV_TOL * np.linalg.norm(v) * np.linalg.norm(hv)
+
+
+@pytest.mark.parametrize("name", sorted(golden_cases()))
+def test_tables_diag_hv_vs_reference_and_oracle(A, oracle_lib, gold, name):
+    print(f"[gpu] case {name}", flush=True)
+    spec = golden_cases()[name]
+    rec = gold["cases"][name]
+    eng = A.Engine.from_spec(spec)
+    assert eng.dim == rec["dim"]
+    l1, l21, l22 = eng.sz()
+    if spec.model in (L.SPIN, L.HUBBARD):
+        assert sha(l1) == rec["sha_list_1"]
+        assert sha(l21) == rec["sha_list_2_1"]
+        assert sha(l22) == rec["sha_list_2_2"]
+    diag = eng.diagonalcalc()
+    assert sha(diag) == rec["sha_list_Diagonal"]
+    v = oracle_lib.fill_vector(SEED, eng.dim)
+    hv, pr = eng.hv(v)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    ohv, opr = orc.hv(v)
+    assert rel_inf(hv, ohv) <= HV_TOL
+    assert prdct_ok(pr, opr, v, ohv)
+    assert prdct_ok(pr, complex(*rec["prdct"]), v, ohv)
+    if "arrays" in rec:
+        ref = np.load(os.path.join(GOLD, rec["arrays"]))["hv"]
+        assert rel_inf(hv, ref) <= HV_TOL
+    eng.close()
+
+
+def test_device_generator_matches_host(A, oracle_lib):
+    eng = A.Engine.from_spec(L.heisenberg_chain(10))
+    d = A.DeviceArray(eng.dim)
+    eng.fill_random(d, 1, 3)
+    assert np.array_equal(d.download(), oracle_lib.fill_vector(3, eng.dim))
+    k = 4
+    b = A.DeviceArray(eng.dim * k)
+    eng.fill_random(b, k, 11)
+    B = b.download().reshape(eng.dim, k)
+    for c in range(k):
+        assert np.array_equal(B[:, c], oracle_lib.fill_vector(11 + c, eng.dim))
+
+
+@pytest.mark.parametrize("name", ["chain13_sz1", "hubbard_4x2", "tri_4x3_gc"])
+def test_mltply_reference_shaped_host_arrays(A, oracle_lib, name):
+    """hphi_mltply on the reference's 1-indexed host arrays: accumulates into v0,
+    leaves element 0 alone, returns prdct."""
+    spec = golden_cases()[name]
+    eng = A.Engine.from_spec(spec)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    dim = eng.dim
+    v1 = np.zeros(dim + 1, np.complex128)
+    v1[1:] = oracle_lib.fill_vector(5, dim)
+    v0 = np.zeros(dim + 1, np.complex128)
+    v0[1:] = oracle_lib.fill_vector(6, dim)
+    v0[0] = 123.0 + 4.0j
+    w0, w1, v0_in = v0.copy(), v1.copy(), v0.copy()
+    pr = eng.mltply(v0, v1)
+    opr = orc.mltply(w0, w1)
+    assert v0[0] == 123.0 + 4.0j
+    assert rel_inf(v0[1:], w0[1:]) <= HV_TOL
+    assert prdct_ok(pr, opr, v1[1:], w0[1:] - v0_in[1:])
+    eng.close()
+
+
+@pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("name", ["j1j2_4x4", "hubbard_4x2", "tri_4x3_gc"])
+def test_block_hv_equals_columns(A, oracle_lib, name, k):
+    print(f"[gpu] block {name} k={k}", flush=True)
+    spec = golden_cases()[name]
+    eng = A.Engine.from_spec(spec)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    dim = eng.dim
+    V1 = A.DeviceArray(dim * k)
+    eng.fill_random(V1, k, 40)
+    V0 = A.DeviceArray(dim * k).zero()
+    pr = eng.mltply_block_dev(k, V0, V1)
+    out = V0.download().reshape(dim, k)
+    for c in range(k):
+        vc = oracle_lib.fill_vector(40 + c, dim)
+        ohv, opr = orc.hv(vc)
+        assert rel_inf(out[:, c], ohv) <= HV_TOL
+        assert prdct_ok(pr[c], opr, vc, ohv)
+    eng.close()
+
+
+def test_from_def_equals_from_terms(A, tmp_path, oracle_lib):
+    spec = golden_cases()["hubbard_chain6_complex"]
+    nl = L.write_expert(spec, str(tmp_path))
+    e1 = A.Engine.from_def(nl)
+    e2 = A.Engine.from_spec(spec)
+    v = oracle_lib.fill_vector(9, e1.dim)
+    h1, p1 = e1.hv(v)
+    h2, p2 = e2.hv(v)
+    assert np.array_equal(h1, h2) and p1 == p2
+    assert np.array_equal(e1.diagonalcalc(), e2.diagonalcalc())
+
+
+def test_hv_is_deterministic(A, oracle_lib):
+    spec = golden_cases()["j1j2_4x4"]
+    eng = A.Engine.from_spec(spec)
+    v = oracle_lib.fill_vector(1, eng.dim)
+    a, pa = eng.hv(v)
+    b, pb = eng.hv(v)
+    assert np.array_equal(a, b) and pa == pb
+
+
+LANCZOS_KNOWN = [
+    # doc/tutorial/en/source/zero_temperature/spin_chain.rst:30,39 (6 decimals)
+    ("chain16", L.heisenberg_chain(16), [-7.142296], 5e-7),
+    # test/lobcg_hubbard_square.sh reference.dat
+    ("hubbard_4x2", L.hubbard_square(4, 2, ncond=8), [-10.2529529552637033], 1e-10),
+]
+
+
+@pytest.mark.parametrize("name,spec,ev,tol", LANCZOS_KNOWN, ids=[k[0] for k in LANCZOS_KNOWN])
+def test_lanczos_known_answers(A, oracle_lib, name, spec, ev, tol):
+    import scipy.sparse.linalg as sla
+    print(f"[gpu] lanczos {name}", flush=True)
+    eng = A.Engine.from_spec(spec)
+    r = eng.lanczos(n_eigen=1, max_iter=2000, eps=1e-14, target=2, seed=0)
+    assert r.converged
+    assert abs(r.eigenvalues[0] - ev[0]) <= tol
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    Aop = sla.LinearOperator((orc.dim, orc.dim), dtype=np.complex128,
+                             matvec=lambda x: orc.hv(np.asarray(x, np.complex128).ravel())[0])
+    w = np.sort(sla.eigsh(Aop, k=3, which="SA", tol=1e-14)[0])
+    assert abs(r.eigenvalues[0] - w[0]) <= 1e-10
+    eng.close()
