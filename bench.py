@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain H·v (mltply) on one MI355X.
+
+Workload: spin-1/2 J1-J2 Heisenberg model on the 8x4 periodic square lattice,
+J1 = 1, J2 = 0.5, canonical Sz = 0 sector (dim C(32,16) = 601,080,390,
+9.6 GB per complex128 vector), synthetic input vector (SplitMix64, seed 2).
+One step = one H·v over the whole sector (v0 += H v1), tables and diagonal
+already resident in HBM, vectors resident in HBM.
+
+value = algorithmic bytes of one H·v / time per H·v, where
+  B_alg = 16 NNZ_off + dim (16 [own v1] + 32 [v0 read+write] + 8 [list_Diagonal] + 8 [list_1])
+and NNZ_off = 128 bonds x 2 C(30,15) = 39,710,085,120 off-diagonal elements.
+
+cpu_baseline: the CPU oracle (a C + OpenMP restatement of the reference's
+per-term scatter sweep) timed on a bounded, contiguous slice of source
+states of the same workload (all tables and full-length vectors), its
+algorithmic bytes counted the same way.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H·v GB/s (% HBM peak); solves/s + p50/p99 latency vs plain, |ΔE|≤1e-10"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "J1-J2 Heisenberg 8x4 PBC, J1=1, J2=0.5, Sz=0 (dim 601,080,390), plain H·v"
+
+
+def b_alg(dim: int, nnz: int, basis_bytes: int = 8, v0_bytes: int = 32) -> int:
+    return 16 * nnz + dim * (16 + v0_bytes + 8 + basis_bytes)
+
+
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    copy-kernel calibrated), recorded under profiles/ by the profiling runs."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def gpu_leg(args):
+    from hphi_amd import capi, lattice as L
+
+    spec = L.baseline_config(3)
+    t0 = time.time()
+    eng = capi.Engine.from_spec(spec)
+    eng.sz(copy=False)
+    eng.diagonalcalc(copy=False)
+    t_setup = time.time() - t0
+    dim = eng.dim
+    nb = len(spec.exchange)
+    nnz = nb * 2 * math.comb(spec.nsite - 2, spec.nup - 1)
+    balg = b_alg(dim, nnz)
+    v1 = capi.DeviceArray(dim)
+    v0 = capi.DeviceArray(dim)
+    eng.fill_random(v1, 1, 2)
+    v0.zero()
+    # warmup, then K timed H·v bracketed by device synchronisation
+    if args.warmup:
+        eng.time_mltply(1, v0, v1, 0, args.warmup)
+    capi.lib().hphi_synchronize()
+    t = time.time()
+    ms = eng.time_mltply(1, v0, v1, 0, args.steps)  # HIP events on the engine's stream
+    capi.lib().hphi_synchronize()
+    wall = time.time() - t
+    # Lanczos step (H·v + update + beta + tridiagonal), device-resident driver
+    lz = eng.lanczos(n_eigen=1, max_iter=args.lanczos_steps, eps=1e-300, target=2, seed=2)
+    res = {
+        "dim": dim, "nnz": nnz, "balg": balg, "ms": ms, "wall_s": wall, "setup_s": t_setup,
+        "lanczos_step_ms": 1e3 * lz.seconds / max(lz.iterations, 1),
+        "device_gb": (eng.device_bytes + 2 * 16 * dim) / 1e9,
+    }
+    v0.free()
+    v1.free()
+    eng.close()
+    return res
+
+
+def cpu_leg(args):
+    import numpy as np
+
+    from hphi_amd import lattice as L
+    from oracle import oracle as O
+
+    spec = L.baseline_config(3)
+    orc = O.Oracle(spec, L.processed_terms(spec))
+    orc.sz()
+    dim = orc.dim
+    S = min(int(args.cpu_sample), dim)
+    jlo = 1 + dim // 3
+    jhi = jlo + S - 1
+    orc.diag = np.zeros(dim + 1)
+    O.lib().or_diagonalcalc(C.byref(orc.m), orc.list_1[jlo - 1:].ctypes.data_as(O._u64p), S,
+                            orc.diag[jlo - 1:].ctypes.data_as(O._f64p))
+    v1 = np.empty(dim + 1, np.complex128)
+    v0 = np.empty(dim + 1, np.complex128)
+    O.lib().or_fill_vector(2, dim, v1.ctypes.data_as(O._f64p))
+    O.lib().or_fill_vector(3, dim, v0.ctypes.data_as(O._f64p))  # touches every page of v0
+    nnz = int(O.lib().or_count_nnz(C.byref(orc.m), orc.list_1.ctypes.data_as(O._u64p), jlo, jhi))
+    times = []
+    for r in range(args.cpu_reps):
+        t = time.time()
+        orc.mltply(v0, v1, jlo, jhi)
+        times.append(time.time() - t)
+    t = statistics.median(times)
+    return {"value": b_alg(S, nnz) / t / 1e9, "unit": "GB/s", "cores": O.lib().or_num_threads(),
+            "kind": "port",
+            "sample": f"{S} contiguous source states (j={jlo}..{jhi}) of the same 8x4 J1-J2 sector, "
+                      f"all 128 Exchange sweeps + diagonal, full-length tables and vectors; median of "
+                      f"{args.cpu_reps} runs, {t:.2f} s each; CPU oracle (C+OpenMP restatement of the "
+                      f"reference's per-term scatter)",
+            "seconds": t}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--lanczos-steps", type=int, default=5)
+    ap.add_argument("--cpu-sample", type=float, default=2e7)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    g = gpu_leg(args)
+    s_per = g["ms"] / 1e3
+    value = g["balg"] / s_per / 1e9
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(g["ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "dim": g["dim"], "nnz_offdiag": g["nnz"],
+                   "bytes_alg_per_hv": g["balg"], "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+                   "lanczos_step_ms": round(g["lanczos_step_ms"], 3),
+                   "device_gb": round(g["device_gb"], 2), "setup_s": round(g["setup_s"], 2)},
+        "roofline": {"bound": "hbm", "achieved": round(value, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(value / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("hv_j1j2_8x4")},
+    }
+    if not args.no_cpu:
+        c = cpu_leg(args)
+        line["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        line["cpu_baseline"]["value"] = round(c["value"], 3)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

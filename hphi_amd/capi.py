@@ -28,6 +28,7 @@ EXPORTED = [
     "hphi_memcpy_d2h", "hphi_dev_zero", "hphi_synchronize", "hphi_fill_random",
     "hphi_vec_lanczos_beta", "hphi_vec_lanczos_update", "hphi_vec_norm2", "hphi_vec_dot",
     "hphi_vec_axpby", "hphi_engine_sync", "hphi_time_mltply_dev", "hphi_lanczos",
+    "hphi_engine_tune", "hphi_engine_get_tune", "hphi_autotune",
 ]
 
 
@@ -99,6 +100,9 @@ def lib():
         L.hphi_engine_sync.argtypes = [_vp]
         L.hphi_time_mltply_dev.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _f64p]
         L.hphi_lanczos.argtypes = [_vp, C.POINTER(LanczosParams), C.POINTER(LanczosResult)]
+        L.hphi_engine_tune.argtypes = [_vp, C.c_char_p, C.c_int]
+        L.hphi_engine_get_tune.argtypes = [_vp, C.c_char_p, C.POINTER(C.c_int)]
+        L.hphi_autotune.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_char_p, C.c_size_t]
         _lib = L
     return _lib
 
@@ -251,6 +255,20 @@ class Engine:
         ms = C.c_double()
         check(lib().hphi_time_mltply_dev(self.h, k, V0.ptr, V1.ptr, warmup, reps, C.byref(ms)))
         return ms.value
+
+    def tune(self, **knobs) -> None:
+        for k, v in knobs.items():
+            check(lib().hphi_engine_tune(self.h, k.encode(), int(v)))
+
+    def get_tune(self, knob: str) -> int:
+        v = C.c_int()
+        check(lib().hphi_engine_get_tune(self.h, knob.encode(), C.byref(v)))
+        return v.value
+
+    def autotune(self, k: int, V0: DeviceArray, V1: DeviceArray, reps: int = 2) -> str:
+        buf = C.create_string_buffer(8192)
+        check(lib().hphi_autotune(self.h, k, V0.ptr, V1.ptr, reps, buf, 8192))
+        return buf.value.decode()
 
     def fill_random(self, v: DeviceArray, k: int, seed: int) -> None:
         check(lib().hphi_fill_random(v.ptr, self.dim, k, seed))

@@ -333,7 +333,7 @@ __global__ __launch_bounds__(HV_TPB) void k_hv(const HvArgs A) {
         double cr[HV_UNROLL], ci[HV_UNROLL];
 #pragma unroll
         for (int u = 0; u < HV_UNROLL; u++) {
-          const GTerm T = A.terms[t + u];
+          const GTerm T = load_term((CTerm)A.terms + t + u);
           const uint64_t y = x ^ T.flip;
           const int64_t l = (int64_t)A.l21[y & A.irght];
           bool act = term_active(T, x) && (l != 0);
@@ -352,13 +352,13 @@ __global__ __launch_bounds__(HV_TPB) void k_hv(const HvArgs A) {
         }
       }
       // high group: only the high half changes, same rank in another block
-      const GTerm *th = A.terms + A.n_low;
+      const CTerm th = (CTerm)A.terms + A.n_low;
       for (int t = 0; t < A.n_high; t += HV_UNROLL) {
         double2 g[HV_UNROLL];
         double cr[HV_UNROLL], ci[HV_UNROLL];
 #pragma unroll
         for (int u = 0; u < HV_UNROLL; u++) {
-          const GTerm T = th[t + u];
+          const GTerm T = load_term(th + t + u);
           const uint64_t y = x ^ T.flip;
           const int64_t l = (int64_t)A.l22[y >> A.nlow];
           bool act = term_active(T, x) && (l != 0);
@@ -377,13 +377,13 @@ __global__ __launch_bounds__(HV_TPB) void k_hv(const HvArgs A) {
         }
       }
       // mixed group: both halves change, two lookups
-      const GTerm *tm = th + A.n_high;
+      const CTerm tm = th + A.n_high;
       for (int t = 0; t < A.n_mixed; t += HV_UNROLL) {
         double2 g[HV_UNROLL];
         double cr[HV_UNROLL], ci[HV_UNROLL];
 #pragma unroll
         for (int u = 0; u < HV_UNROLL; u++) {
-          const GTerm T = tm[t + u];
+          const GTerm T = load_term(tm + t + u);
           const uint64_t y = x ^ T.flip;
           const int64_t la = (int64_t)A.l21[y & A.irght];
           const int64_t lb = (int64_t)A.l22[y >> A.nlow];
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(HV_TPB) void k_hv(const HvArgs A) {
         double cr[HV_UNROLL], ci[HV_UNROLL];
 #pragma unroll
         for (int u = 0; u < HV_UNROLL; u++) {
-          const GTerm T = A.terms[t + u];
+          const GTerm T = load_term((CTerm)A.terms + t + u);
           bool act = term_active(T, x);
           const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
           uint64_t idx = x ^ T.flip;
@@ -727,6 +727,15 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
   if (!e->h_dterms.empty())
     (void)hipMemcpy(e->d_dterms, e->h_dterms.data(), e->h_dterms.size() * sizeof(DTerm), hipMemcpyHostToDevice);
   e->bytes += tb + db;
+  hphi_canon_config(e);
+  if (e->c2_lds21) {  // list_2_1 in LDS: one 1024-thread workgroup per CU
+    e->c2_tpb = 1024;
+    e->c2_wgs_per_cu = 1;
+    e->xcd_map = 0;     // measured best for this shape (8x4 J1-J2, 4x4 Hubbard)
+  } else {
+    e->c2_tpb = 512;
+    e->c2_wgs_per_cu = 8;
+  }
   if (hphi_engine_alloc_scratch(e)) {
     hphi_engine_destroy(e);
     return -1;
@@ -768,6 +777,52 @@ extern "C" int hphi_engine_info(const hphi_engine *e, uint64_t *dim, uint64_t *s
 }
 
 extern "C" uint64_t hphi_engine_device_bytes(const hphi_engine *e) { return e ? e->bytes : 0; }
+
+extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
+  if (!e || !knob) return hphi_set_error("hphi_engine_tune: NULL argument");
+  std::string k(knob);
+  if (k == "tpb") {
+    if (value != 256 && value != 512 && value != 1024) return hphi_set_error("tpb must be 256, 512 or 1024");
+    e->c2_tpb = value;
+  } else if (k == "wgs_per_cu") {
+    if (value < 1 || value > 64) return hphi_set_error("wgs_per_cu out of range");
+    e->c2_wgs_per_cu = value;
+  } else if (k == "lds21") {
+    if (value) {
+      int saved = e->c2_lds21;
+      hphi_canon_config(e);
+      if (!e->c2_lds21) {
+        e->c2_lds21 = saved;
+        return hphi_set_error("list_2_1 does not fit LDS for this model");
+      }
+    } else {
+      e->c2_lds21 = 0;
+    }
+  } else if (k == "xcd_map") {
+    e->xcd_map = value ? 1 : 0;
+  } else if (k == "use_c2") {
+    e->use_c2 = value ? 1 : 0;
+  } else if (k == "grid_per_cu") {
+    if (value < 1 || value > 64) return hphi_set_error("grid_per_cu out of range");
+    e->grid_per_cu = value;
+  } else {
+    return hphi_set_error("unknown tuning knob %s", knob);
+  }
+  return 0;
+}
+
+extern "C" int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int *value) {
+  if (!e || !knob || !value) return hphi_set_error("hphi_engine_get_tune: NULL argument");
+  std::string k(knob);
+  if (k == "tpb") *value = e->c2_tpb;
+  else if (k == "wgs_per_cu") *value = e->c2_wgs_per_cu;
+  else if (k == "lds21") *value = e->c2_lds21;
+  else if (k == "xcd_map") *value = e->xcd_map;
+  else if (k == "use_c2") *value = e->use_c2 && e->c2_ok;
+  else if (k == "grid_per_cu") *value = e->grid_per_cu;
+  else return hphi_set_error("unknown tuning knob %s", knob);
+  return 0;
+}
 
 static SzParams sz_params(const hphi_engine *e) {
   SzParams P{};
@@ -874,6 +929,7 @@ HvArgs hphi_hv_args(hphi_engine *e, int k, double *v0, const double *v1) {
   A.v1 = v1;
   A.partial = (double2 *)e->d_partial;
   A.xcd_map = e->xcd_map;
+  A.size21 = e->size21;
   (void)k;
   return A;
 }
@@ -889,8 +945,25 @@ int hphi_hv_grid(const hphi_engine *e, int k) {
 }
 
 // launches H·v and the final reduction on e->stream; result in e->d_prdct[0..k)
+static int canon_grid(const hphi_engine *e) {
+  uint64_t ntiles = (e->dim + e->c2_tpb - 1) / e->c2_tpb;
+  uint64_t g = std::min<uint64_t>(ntiles, (uint64_t)e->c2_wgs_per_cu * std::max(e->num_cu, 1));
+  g = std::max<uint64_t>(g, 8);
+  g = (g + 7) / 8 * 8;
+  if (g > HV_MAX_GRID) g = HV_MAX_GRID;
+  return (int)g;
+}
+
 int hphi_hv_launch(hphi_engine *e, int k, double *v0, const double *v1) {
   HvArgs A = hphi_hv_args(e, k, v0, v1);
+  if (k == 1 && e->c2_ok && e->use_c2) {
+    int g = canon_grid(e);
+    if (hphi_hv_launch_canon(e, A, g)) return -1;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, e->stream, (const double2 *)e->d_partial,
+                       g, 1, (double2 *)e->d_prdct);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   int g = hphi_hv_grid(e, k);
   bool canon = e->model != HPHI_MODEL_SPINGC;
 #define LAUNCH(KK)                                                                                   \

@@ -26,6 +26,24 @@ struct GTerm {
   int kind, pad;
 };
 
+// Term tables are read through the constant address space: the loads are
+// known invariant, so uniform accesses become scalar (SMEM) loads instead of
+// per-lane vector loads that would queue behind the gathers.
+typedef const GTerm __attribute__((address_space(4))) *CTerm;
+
+__device__ __forceinline__ GTerm load_term(CTerm p) {
+  GTerm t;
+  t.flip = p->flip;
+  t.tmask = p->tmask;
+  t.tval = p->tval;
+  t.sgn = p->sgn;
+  t.cre = p->cre;
+  t.cim = p->cim;
+  t.kind = p->kind;
+  t.pad = 0;
+  return t;
+}
+
 // diagonal term kinds (diagonalcalc.c set functions, per model)
 enum { D_INTRA = 0, D_CHEMI_H, D_CHEMI_S, D_INTER_H, D_INTER_S, D_HUND_H, D_HUND_S };
 
@@ -55,6 +73,7 @@ struct HvArgs {
   const double *v1;
   double2 *partial;
   int xcd_map;
+  uint64_t size21;
 };
 
 struct hphi_engine {
@@ -76,6 +95,9 @@ struct hphi_engine {
   void *d_partial = nullptr, *d_prdct = nullptr, *h_prdct = nullptr, *d_coef = nullptr;
   double *d_stage0 = nullptr, *d_stage1 = nullptr;
   uint64_t bytes = 0;
+  // canonical single-vector kernel (hv_canon.hip) configuration
+  int c2_ok = 0, c2_realc = 1, c2_lds21 = 0, c2_max_lds = 0;
+  int c2_tpb = 1024, c2_wgs_per_cu = 1, use_c2 = 1;
 };
 
 extern "C" int hphi_set_error(const char *fmt, ...);
@@ -84,3 +106,5 @@ int hphi_build_diag(hphi_engine *e);
 int hphi_hv_launch(hphi_engine *e, int k, double *v0, const double *v1);
 int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct);
 int hphi_hv_grid(const hphi_engine *e, int k);
+int hphi_canon_config(hphi_engine *e);
+int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g);

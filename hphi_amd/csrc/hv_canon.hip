@@ -1,0 +1,282 @@
+// Gather-form H·v for canonical models (Spin, Hubbard) with states of at
+// most 32 bits, one lane per basis state — the hot kernel of the plain path.
+//
+// Same arithmetic as k_hv in engine.hip (transpose of the reference's
+// per-term scatter, mltplySpinCore.c:273-307, mltplyHubbardCore.c:305-347),
+// organised for the CDNA4 memory pipeline:
+//  * list_2_1 lives in LDS as uint16 when it fits (rank-in-class + 2 <= 65535):
+//    the per-lane rank lookups of low-half and mixed terms never touch L1/L2;
+//  * a wave whose 64 states share one high block (the common case: blocks
+//    hold up to C(16,8) = 12870 states) takes a wave-uniform path for the
+//    high-half terms: predicate, list_2_2 lookups and sign are scalar, the
+//    gather is one contiguous 1 KB read, and inactive terms issue nothing;
+//  * gathers are masked (only active lanes present an address to L1);
+//  * state words are 32-bit; real couplings skip the imaginary products.
+#include <hip/hip_runtime.h>
+
+#include "engine_internal.h"
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return hphi_set_error("%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+#define C2_UNROLL 8
+
+__device__ __forceinline__ bool pred32(int kind, uint32_t tmask, uint32_t tval, uint32_t x) {
+  const uint32_t ym = x & tmask;
+  const int pc = __popc(ym);
+  return (kind == P_EQ) ? (ym == tval) : ((kind == P_DIFF) ? (pc == 1) : (pc != 1));
+}
+
+template <bool REALC>
+__device__ __forceinline__ void cmac(double &ar, double &ai, double cr, double ci, double2 g) {
+  if (REALC) {
+    ar += cr * g.x;
+    ai += cr * g.y;
+  } else {
+    ar += cr * g.x - ci * g.y;
+    ai += cr * g.y + ci * g.x;
+  }
+}
+
+template <int TPB, bool REALC, bool LDS21>
+__global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
+  extern __shared__ uint16_t s21[];
+  const uint32_t tid = threadIdx.x;
+  if (LDS21) {
+    for (uint32_t i = tid; i < (uint32_t)A.size21; i += TPB) s21[i] = (uint16_t)A.l21[i];
+    __syncthreads();
+  }
+  const double2 *__restrict__ v1 = (const double2 *)A.v1;
+  double2 *__restrict__ v0 = (double2 *)A.v0;
+  const uint32_t irght = (uint32_t)A.irght;
+  const int nlow = A.nlow;
+  const uint64_t dim = A.dim;
+  const CTerm tl = (CTerm)A.terms;
+  const CTerm th = tl + A.n_low;
+  const CTerm tm = th + A.n_high;
+  double pre = 0.0, pim = 0.0;
+
+  const uint64_t ntiles = (dim + TPB - 1) / TPB;
+  uint64_t t_begin, t_end, t_step;
+  if (A.xcd_map) {
+    const uint32_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+    t_begin = (ntiles * xcd) / 8 + slot;
+    t_end = (ntiles * (xcd + 1)) / 8;
+    t_step = nslot;
+  } else {
+    t_begin = blockIdx.x;
+    t_end = ntiles;
+    t_step = gridDim.x;
+  }
+  for (uint64_t tile = t_begin; tile < t_end; tile += t_step) {
+    const uint64_t j = tile * TPB + tid;
+    const bool valid = j < dim;
+    const uint64_t jj = valid ? j : dim - 1;
+    const uint32_t x = (uint32_t)A.list1[jj];
+    const uint32_t ia = x & irght, ib = x >> nlow;
+    const double2 a = v1[jj];
+    const double d = A.diag[jj];
+    double ar = d * a.x, ai = d * a.y;
+    const int64_t b21 = LDS21 ? (int64_t)s21[ia] : (int64_t)A.l21[ia];
+
+    // ---- low-half terms: target in the same block, per-lane rank lookup
+    for (int t = 0; t < A.n_low; t += C2_UNROLL) {
+      double2 g[C2_UNROLL];
+      int64_t l[C2_UNROLL];
+      bool act[C2_UNROLL];
+      // phase 1: all rank lookups of the batch (LDS, or L2 issued back to back)
+#pragma unroll
+      for (int u = 0; u < C2_UNROLL; u++) {
+        const GTerm T = load_term(tl + t + u);
+        const uint32_t y = x ^ (uint32_t)T.flip;
+        act[u] = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
+        l[u] = LDS21 ? (int64_t)s21[y & irght] : (int64_t)A.l21[y & irght];
+      }
+      // phase 2: all gathers of the batch, masked to the active lanes
+#pragma unroll
+      for (int u = 0; u < C2_UNROLL; u++) {
+        const uint64_t idx = (uint64_t)((int64_t)j + l[u] - b21);
+        const bool a2 = act[u] & (l[u] != 0) & (idx < dim);
+        g[u] = make_double2(0.0, 0.0);
+        if (a2) g[u] = v1[idx];
+      }
+#pragma unroll
+      for (int u = 0; u < C2_UNROLL; u++) {
+        const GTerm T = load_term(tl + t + u);
+        double2 gg = g[u];
+        if ((uint32_t)T.sgn != 0u && (__popc(x & (uint32_t)T.sgn) & 1)) {
+          gg.x = -gg.x;
+          gg.y = -gg.y;
+        }
+        cmac<REALC>(ar, ai, T.cre, T.cim, gg);
+      }
+    }
+
+    // ---- high-half and mixed terms
+    const uint32_t ib0 = __builtin_amdgcn_readfirstlane(ib);
+    const bool uni = (__ballot(ib != ib0) == 0ull);
+    if (uni) {
+      // wave-uniform block: predicate, lookups and sign of high terms are scalar
+      const uint32_t xh = ib0 << nlow;
+      const int64_t l22_0 = (int64_t)A.l22[ib0];
+      for (int t = 0; t < A.n_high; t += C2_UNROLL) {
+        double2 g[C2_UNROLL];
+        bool au[C2_UNROLL];
+#pragma unroll
+        for (int u = 0; u < C2_UNROLL; u++) {
+          const GTerm T = load_term(th + t + u);
+          au[u] = pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, xh);
+          g[u] = make_double2(0.0, 0.0);
+          if (au[u]) {
+            const int64_t l = (int64_t)A.l22[(xh ^ (uint32_t)T.flip) >> nlow];
+            au[u] = (l != 0);
+            if (au[u] && valid) g[u] = v1[(uint64_t)((int64_t)j + l - l22_0)];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < C2_UNROLL; u++) {
+          const GTerm T = load_term(th + t + u);
+          if (au[u]) {
+            double s = ((uint32_t)T.sgn != 0u && (__popc(xh & (uint32_t)T.sgn) & 1)) ? -1.0 : 1.0;
+            cmac<REALC>(ar, ai, s * T.cre, s * T.cim, g[u]);
+          }
+        }
+      }
+      for (int t = 0; t < A.n_mixed; t += C2_UNROLL) {
+        double2 g[C2_UNROLL];
+        int64_t la[C2_UNROLL], lb[C2_UNROLL];
+        bool act[C2_UNROLL];
+#pragma unroll
+        for (int u = 0; u < C2_UNROLL; u++) {
+          const GTerm T = load_term(tm + t + u);
+          const uint32_t y = x ^ (uint32_t)T.flip;
+          act[u] = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
+          lb[u] = (int64_t)A.l22[(xh ^ (uint32_t)T.flip) >> nlow];  // uniform
+          la[u] = LDS21 ? (int64_t)s21[y & irght] : (int64_t)A.l21[y & irght];
+        }
+#pragma unroll
+        for (int u = 0; u < C2_UNROLL; u++) {
+          const uint64_t idx = (uint64_t)(la[u] + lb[u] - 3);
+          const bool a2 = act[u] & (la[u] != 0) & (lb[u] != 0) & (idx < dim);
+          g[u] = make_double2(0.0, 0.0);
+          if (a2) g[u] = v1[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < C2_UNROLL; u++) {
+          const GTerm T = load_term(tm + t + u);
+          double2 gg = g[u];
+          if ((uint32_t)T.sgn != 0u && (__popc(x & (uint32_t)T.sgn) & 1)) {
+            gg.x = -gg.x;
+            gg.y = -gg.y;
+          }
+          cmac<REALC>(ar, ai, T.cre, T.cim, gg);
+        }
+      }
+    } else {
+      // lanes span several blocks: per-lane list_2_2 lookups
+      const int64_t b22 = (int64_t)A.l22[ib];
+      for (int t = 0; t < A.n_high + A.n_mixed; t += C2_UNROLL) {
+        double2 g[C2_UNROLL];
+#pragma unroll
+        for (int u = 0; u < C2_UNROLL; u++) {
+          const GTerm T = load_term(th + t + u);
+          const uint32_t y = x ^ (uint32_t)T.flip;
+          bool act = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
+          int64_t la = b21, lb = 0;
+          if (act) {
+            lb = (int64_t)A.l22[y >> nlow];
+            if ((y & irght) != ia) la = LDS21 ? (int64_t)s21[y & irght] : (int64_t)A.l21[y & irght];
+          }
+          const uint64_t idx = (uint64_t)(la + lb - 3);
+          act = act & (la != 0) & (lb != 0) & (idx < dim);
+          g[u] = make_double2(0.0, 0.0);
+          if (act) g[u] = v1[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < C2_UNROLL; u++) {
+          const GTerm T = load_term(th + t + u);
+          double2 gg = g[u];
+          if ((uint32_t)T.sgn != 0u && (__popc(x & (uint32_t)T.sgn) & 1)) {
+            gg.x = -gg.x;
+            gg.y = -gg.y;
+          }
+          cmac<REALC>(ar, ai, T.cre, T.cim, gg);
+        }
+      }
+      (void)b22;
+    }
+
+    if (valid) {
+      double2 o = v0[j];
+      o.x += ar;
+      o.y += ai;
+      v0[j] = o;
+      pre += a.x * ar + a.y * ai;
+      pim += a.x * ai - a.y * ar;
+    }
+  }
+
+  // workgroup reduction of prdct (fixed order)
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    pre += __shfl_xor(pre, off, 64);
+    pim += __shfl_xor(pim, off, 64);
+  }
+  __shared__ double2 red[TPB / 64];
+  if ((tid & 63) == 0) red[tid >> 6] = make_double2(pre, pim);
+  __syncthreads();
+  if (tid == 0) {
+    double2 s = red[0];
+    for (int w = 1; w < TPB / 64; w++) {
+      s.x += red[w].x;
+      s.y += red[w].y;
+    }
+    A.partial[blockIdx.x] = s;
+  }
+}
+
+// Which terms are real and whether the list_2_1 table fits LDS as uint16.
+int hphi_canon_config(hphi_engine *e) {
+  e->c2_realc = 1;
+  for (const GTerm &t : e->h_terms_dev)
+    if (t.cim != 0.0) e->c2_realc = 0;
+  int maxlds = 0;
+  if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess) maxlds = 65536;
+  e->c2_max_lds = maxlds;
+  size_t need = 2 * (size_t)e->size21;
+  // rank + 2 must fit uint16: the largest class has C(nlow, nlow/2) members (spin)
+  bool small_rank = e->nlow <= 16;
+  e->c2_lds21 = small_rank && need + 1024 <= (size_t)maxlds;
+  e->c2_ok = e->nbits <= 32 && e->model != HPHI_MODEL_SPINGC;
+  return 0;
+}
+
+template <int TPB, bool REALC, bool LDS21>
+static int launch_one(hphi_engine *e, const HvArgs &A, int g) {
+  size_t lds = LDS21 ? 2 * (size_t)e->size21 : 0;
+  if (LDS21 && lds > 65536)
+    (void)hipFuncSetAttribute((const void *)k_hv_canon<TPB, REALC, LDS21>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_hv_canon<TPB, REALC, LDS21>), dim3(g), dim3(TPB), lds, e->stream, A);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g) {
+  const int tpb = e->c2_tpb;
+#define SEL(TPB)                                                                      \
+  if (tpb == TPB) {                                                                   \
+    if (e->c2_realc) return e->c2_lds21 ? launch_one<TPB, true, true>(e, A, g)        \
+                                        : launch_one<TPB, true, false>(e, A, g);      \
+    return e->c2_lds21 ? launch_one<TPB, false, true>(e, A, g)                        \
+                       : launch_one<TPB, false, false>(e, A, g);                      \
+  }
+  SEL(256)
+  SEL(512)
+  SEL(1024)
+#undef SEL
+  return hphi_set_error("unsupported workgroup size %d", tpb);
+}

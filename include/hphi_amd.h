@@ -88,6 +88,17 @@ void hphi_model_desc_free(hphi_model_desc *desc);
  * irght/ilft/ihfbit = split masks (bitcalc.c:45-62). */
 int hphi_engine_info(const hphi_engine *e, uint64_t *dim, uint64_t *size21, uint64_t *size22,
                      uint64_t *irght, uint64_t *ilft, uint64_t *ihfbit);
+/* Launch-shape knobs of the H·v kernels (set by the autotuner or by hand):
+ *   "tpb" 256|512|1024, "wgs_per_cu" 1..64, "lds21" 0|1 (list_2_1 staged in
+ *   LDS), "xcd_map" 0|1 (contiguous state ranges per XCD), "use_c2" 0|1
+ *   (32-bit canonical kernel), "grid_per_cu" 1..64 (generic kernel). */
+int hphi_engine_tune(hphi_engine *e, const char *knob, int value);
+int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int *value);
+/* Times every candidate launch shape on the caller's device vectors (k = block
+ * size; v0 accumulates, re-initialise it afterwards) and keeps the fastest;
+ * one line per candidate goes to `report` when non-NULL. */
+int hphi_autotune(hphi_engine *e, int k, double *v0, const double *v1, int reps, char *report,
+                  size_t report_len);
 /* HBM bytes the engine holds (tables, diagonal, scratch). */
 uint64_t hphi_engine_device_bytes(const hphi_engine *e);
 

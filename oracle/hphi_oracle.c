@@ -437,6 +437,25 @@ int or_mltply(const or_model *m, const uint64_t *list_1, const uint64_t *l21, co
   return 0;
 }
 
+/* Number of off-diagonal (term, source) pairs or_mltply processes for sources
+ * jlo..jhi: the NNZ that the algorithmic-byte count of a sweep uses. */
+uint64_t or_count_nnz(const or_model *m, const uint64_t *list_1, uint64_t jlo, uint64_t jhi) {
+  uint64_t n = 0;
+  if (m->model == OR_SPIN) {
+    for (int t = 0; t < m->n_ex; t++) {
+      uint64_t is = (1ULL << m->ex_ij[2 * t]) + (1ULL << m->ex_ij[2 * t + 1]), c = 0;
+#pragma omp parallel for reduction(+ : c)
+      for (uint64_t j = jlo; j <= jhi; j++) {
+        uint64_t b = list_1[j] & is;
+        c += (b != 0 && b != is);
+      }
+      n += c;
+    }
+    return n;
+  }
+  return 0; /* other models: not needed by the benchmark */
+}
+
 /* ------------------------------------------------------ synthetic input -- */
 
 static inline uint64_t splitmix64(uint64_t x) {

@@ -57,6 +57,9 @@ int or_mltply(const or_model *m, const uint64_t *list_1, const uint64_t *list_2_
               const uint64_t *list_2_2, const double *list_Diagonal, uint64_t dim,
               double *v0, const double *v1, double *prdct, uint64_t jlo, uint64_t jhi);
 
+/* Off-diagonal (term, source) pairs for sources jlo..jhi (Spin model). */
+uint64_t or_count_nnz(const or_model *m, const uint64_t *list_1, uint64_t jlo, uint64_t jhi);
+
 /* Synthetic input vector: v[1+j] = (2u(2j)-1) + i(2u(2j+1)-1), u from SplitMix64. */
 void or_fill_vector(uint64_t seed, uint64_t dim, double *v);
 

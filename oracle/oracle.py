@@ -58,6 +58,8 @@ def lib():
                                 _f64p, _f64p, _f64p, C.c_uint64, C.c_uint64]
         L.or_fill_vector.argtypes = [C.c_uint64, C.c_uint64, _f64p]
         L.or_num_threads.restype = C.c_int
+        L.or_count_nnz.argtypes = [C.POINTER(OrModel), _u64p, C.c_uint64, C.c_uint64]
+        L.or_count_nnz.restype = C.c_uint64
         _lib = L
     return _lib
 

@@ -174,3 +174,28 @@ def test_lanczos_known_answers(A, oracle_lib, name, spec, ev, tol):
     w = np.sort(sla.eigsh(Aop, k=3, which="SA", tol=1e-14)[0])
     assert abs(r.eigenvalues[0] - w[0]) <= 1e-10
     eng.close()
+
+
+TUNINGS = [dict(use_c2=0), dict(use_c2=1, lds21=0, tpb=256, wgs_per_cu=4),
+           dict(use_c2=1, lds21=1, tpb=1024, wgs_per_cu=1, xcd_map=1),
+           dict(use_c2=1, lds21=1, tpb=512, wgs_per_cu=1, xcd_map=0)]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
+@pytest.mark.parametrize("name", ["chain16", "j1j2_4x3_sz2", "hubbard_4x2", "hubbard_3x2_n5",
+                                  "hubbard_chain6_complex", "chain10_field"])
+def test_every_tuning_matches_oracle(A, oracle_lib, name, tuning):
+    print(f"[gpu] tuning {name} {tuning}", flush=True)
+    spec = golden_cases()[name]
+    eng = A.Engine.from_spec(spec)
+    try:
+        eng.tune(**tuning)
+    except A.HphiError:
+        pytest.skip("tuning not applicable to this model")
+    v = oracle_lib.fill_vector(SEED, eng.dim)
+    hv, pr = eng.hv(v)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    ohv, opr = orc.hv(v)
+    assert rel_inf(hv, ohv) <= HV_TOL
+    assert prdct_ok(pr, opr, v, ohv)
+    eng.close()
