@@ -28,7 +28,8 @@ EXPORTED = [
     "hphi_memcpy_d2h", "hphi_dev_zero", "hphi_synchronize", "hphi_fill_random",
     "hphi_vec_lanczos_beta", "hphi_vec_lanczos_update", "hphi_vec_norm2", "hphi_vec_dot",
     "hphi_vec_axpby", "hphi_engine_sync", "hphi_time_mltply_dev", "hphi_lanczos",
-    "hphi_engine_tune", "hphi_engine_get_tune", "hphi_autotune",
+    "hphi_engine_tune", "hphi_engine_get_tune", "hphi_autotune", "hphi_tpq", "hphi_diag_maxabs",
+    "hphi_read_def", "hphi_model_desc_free", "hphi_tridiag_lowest",
 ]
 
 
@@ -57,6 +58,15 @@ class LanczosParams(C.Structure):
 class LanczosResult(C.Structure):
     _fields_ = [("eigenvalues", C.c_double * 16), ("n_found", C.c_int), ("iterations", C.c_int),
                 ("converged", C.c_int), ("seconds", C.c_double), ("hv_seconds", C.c_double)]
+
+
+class TpqParams(C.Structure):
+    _fields_ = [("k", C.c_int), ("steps", C.c_int), ("init_kind", C.c_int), ("layout", C.c_int),
+                ("use_graph", C.c_int), ("large_value", C.c_double), ("seed", C.c_uint64)]
+
+
+class TpqResult(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("large_value", C.c_double), ("steps", C.c_int)]
 
 
 _lib = None
@@ -103,6 +113,8 @@ def lib():
         L.hphi_engine_tune.argtypes = [_vp, C.c_char_p, C.c_int]
         L.hphi_engine_get_tune.argtypes = [_vp, C.c_char_p, C.POINTER(C.c_int)]
         L.hphi_autotune.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_char_p, C.c_size_t]
+        L.hphi_tpq.argtypes = [_vp, C.POINTER(TpqParams), _f64p, C.POINTER(TpqResult)]
+        L.hphi_diag_maxabs.argtypes = [_vp, _f64p]
         _lib = L
     return _lib
 
@@ -269,6 +281,20 @@ class Engine:
         buf = C.create_string_buffer(8192)
         check(lib().hphi_autotune(self.h, k, V0.ptr, V1.ptr, reps, buf, 8192))
         return buf.value.decode()
+
+    def tpq(self, k: int, steps: int, seed: int = 0, init_kind: int = -1, layout: int = 1,
+            use_graph: bool = True, large_value: float = 0.0):
+        """Microcanonical TPQ for k samples; returns (out[steps, k, 8], TpqResult)."""
+        p = TpqParams(k, steps, init_kind, layout, 1 if use_graph else 0, large_value, seed)
+        out = np.zeros((steps, k, 8))
+        r = TpqResult()
+        check(lib().hphi_tpq(self.h, C.byref(p), _p(out, _f64p), C.byref(r)))
+        return out, r
+
+    def diag_maxabs(self) -> float:
+        m = C.c_double()
+        check(lib().hphi_diag_maxabs(self.h, C.byref(m)))
+        return m.value
 
     def fill_random(self, v: DeviceArray, k: int, seed: int) -> None:
         check(lib().hphi_fill_random(v.ptr, self.dim, k, seed))

@@ -80,8 +80,8 @@ __device__ __forceinline__ void hub_counts(uint64_t x, int shift, int *nu, int *
   // counts of set bits at even / odd ABSOLUTE positions, x holding bits from `shift`
   const uint64_t even = 0x5555555555555555ull, odd = 0xAAAAAAAAAAAAAAAAull;
   uint64_t e = (shift & 1) ? odd : even, o = (shift & 1) ? even : odd;
-  *nu = __popcll(x & e);
-  *nd = __popcll(x & o);
+  *nu = (int)__popcll(x & e);
+  *nd = (int)__popcll(x & o);
 }
 
 // number of even (up) absolute positions among relative positions [0, p) of a
@@ -118,7 +118,7 @@ __global__ void k_block_prefix(SzParams P, uint64_t *jb, uint64_t *l22) {
       hub_counts(ib, P.nlow, &nu, &nd);
       cnt = dbinom(P.nlu, P.nup - nu) * dbinom(P.nld, P.ndn - nd);
     } else {
-      cnt = dbinom(P.nlow, P.nup - __popcll(ib));
+      cnt = dbinom(P.nlow, P.nup - (int)__popcll(ib));
     }
     l22[ib] = cnt ? acc + 1 : 0;
   } else {
@@ -147,7 +147,7 @@ __global__ void k_list21(SzParams P, uint32_t *l21) {
         val = (uint32_t)(rank + 2);
       }
     } else {
-      int k = __popcll(ia);
+      int k = (int)__popcll(ia);
       if (P.nup - k >= 0 && P.nup - k <= P.nhigh) {
         uint64_t rank = 0;
         int s = 0;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_list1(SzParams P, const uint64_t *jb, u
       a = P.nup - nu;
       b = P.ndn - nd;
     } else {
-      a = P.nup - __popcll(ib);
+      a = P.nup - (int)__popcll(ib);
       b = 0;
     }
     for (uint64_t r = threadIdx.x; r < cnt; r += blockDim.x) {
@@ -425,9 +425,12 @@ __global__ __launch_bounds__(HV_TPB) void k_hv(const HvArgs A) {
         }
       }
     }
-    double2 o = v0[j * K + vec];
-    o.x += ar;
-    o.y += ai;
+    double2 o = make_double2(ar, ai);
+    if (!A.overwrite) {
+      const double2 p = v0[j * K + vec];
+      o.x += p.x;
+      o.y += p.y;
+    }
     v0[j * K + vec] = o;
     // conj(a) * acc
     pre += a.x * ar + a.y * ai;
@@ -955,7 +958,12 @@ static int canon_grid(const hphi_engine *e) {
 }
 
 int hphi_hv_launch(hphi_engine *e, int k, double *v0, const double *v1) {
+  return hphi_hv_launch_ex(e, k, v0, v1, 0);
+}
+
+int hphi_hv_launch_ex(hphi_engine *e, int k, double *v0, const double *v1, int overwrite) {
   HvArgs A = hphi_hv_args(e, k, v0, v1);
+  A.overwrite = overwrite;
   if (k == 1 && e->c2_ok && e->use_c2) {
     int g = canon_grid(e);
     if (hphi_hv_launch_canon(e, A, g)) return -1;

@@ -74,6 +74,7 @@ struct HvArgs {
   double2 *partial;
   int xcd_map;
   uint64_t size21;
+  int overwrite;  // 1: v0 = H v1 (v0 not read); 0: v0 += H v1 (reference semantics)
 };
 
 struct hphi_engine {
@@ -104,6 +105,7 @@ extern "C" int hphi_set_error(const char *fmt, ...);
 int hphi_build_tables(hphi_engine *e);
 int hphi_build_diag(hphi_engine *e);
 int hphi_hv_launch(hphi_engine *e, int k, double *v0, const double *v1);
+int hphi_hv_launch_ex(hphi_engine *e, int k, double *v0, const double *v1, int overwrite);
 int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct);
 int hphi_hv_grid(const hphi_engine *e, int k);
 int hphi_canon_config(hphi_engine *e);

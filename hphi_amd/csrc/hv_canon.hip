@@ -210,9 +210,12 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
     }
 
     if (valid) {
-      double2 o = v0[j];
-      o.x += ar;
-      o.y += ai;
+      double2 o = make_double2(ar, ai);
+      if (!A.overwrite) {
+        const double2 p = v0[j];
+        o.x += p.x;
+        o.y += p.y;
+      }
       v0[j] = o;
       pre += a.x * ar + a.y * ai;
       pim += a.x * ai - a.y * ar;

@@ -1,0 +1,331 @@
+// Microcanonical TPQ on the device for k samples at once.
+//
+// Restates the step of the reference (src/CalcByTPQ.c:216-272):
+//   Multiply         v0 = l v1 - v0/Ns, v0 /= |v0|          (src/Multiply.c:42-66)
+//   expec_energy_flct v1 <- v0, v0 = H v1, E = <v1|v0>, E2 = <v0|v0>
+//                    (+ Sz, Sz^2 for SpinGC)               (src/expec_energy_flct.c:126-157,429-500)
+//   inv_temp = 2 step/Ns / (l - E/Ns)                      (src/CalcByTPQ.c:235)
+// with the normalisation carried as a scalar: the vector kept in HBM is
+// w_s = (l w_{s-1} - H w_{s-1}/Ns)/|w_{s-1}|, whose norm is the reference's
+// global_norm, and E, E2 are divided by |w_s|^2.  One step is an overwriting
+// H·v (prdct fused) plus one streaming kernel (k_tpq_fused) plus a one-block
+// finalisation that writes the step's output row and the next scale on the
+// device, so a step needs no host synchronisation and is captured once in a
+// hipGraph.  Layout: column c of sample vectors at element j*sj + c*sc
+// (state-major block: sj = k, sc = 1; vector-major: sj = 1, sc = dim).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "engine_internal.h"
+#include "hphi_amd.h"
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return hphi_set_error("%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+#define TQ_TPB 256
+#define TQ_GRID 1024
+#define TQ_NRED 4  // per column: |Hw|^2, obs, obs^2, |w_next|^2
+
+__device__ __forceinline__ uint64_t tq_splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double tq_unit(uint64_t seed, uint64_t c) {
+  return (double)(tq_splitmix64((seed * 0x9E3779B97F4A7C15ull) ^ c) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// column c from seed0 + c; kind 0: (2u-1) + i(2u-1) (MakeIniVec.c:69, InitialVecType 0),
+// kind -1: complex Gaussian by Box-Muller (MakeIniVec.c:81-91, InitialVecType -1)
+__global__ void k_tpq_init(double2 *v, uint64_t dim, int k, uint64_t sj, uint64_t sc, uint64_t seed0, int kind) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim * (uint64_t)k) return;
+  uint64_t j = i / k;
+  int c = (int)(i % k);
+  uint64_t s = seed0 + (uint64_t)c;
+  double u1 = tq_unit(s, 2 * j), u2 = tq_unit(s, 2 * j + 1);
+  double2 z;
+  if (kind == -1) {
+    double r = sqrt(-2.0 * log(1.0 - u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    z = make_double2(r * cs, r * sn);
+  } else {
+    z = make_double2(2.0 * u1 - 1.0, 2.0 * u2 - 1.0);
+  }
+  v[j * sj + c * sc] = z;
+}
+
+// squared norms of the columns (initial |w_0|^2)
+__global__ __launch_bounds__(TQ_TPB) void k_tpq_colnorm(const double2 *v, uint64_t dim, int k, uint64_t sj,
+                                                        uint64_t sc, double *partial) {
+  const int c = threadIdx.x % k, jl = threadIdx.x / k, spb = TQ_TPB / k;
+  double s = 0.0;
+  for (uint64_t j = (uint64_t)blockIdx.x * spb + jl; j < dim; j += (uint64_t)gridDim.x * spb) {
+    double2 a = v[j * sj + c * sc];
+    s += a.x * a.x + a.y * a.y;
+  }
+  for (int off = 32; off >= k; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double red[TQ_TPB / 64][16];
+  if ((threadIdx.x & 63) < k) red[threadIdx.x >> 6][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double t = 0.0;
+    for (int w = 0; w < TQ_TPB / 64; w++) t += red[w][threadIdx.x];
+    partial[(uint64_t)blockIdx.x * k + threadIdx.x] = t;
+  }
+}
+
+__device__ __forceinline__ double tq_obs(int kind, uint64_t x, int nsite) {
+  if (kind == 1) return (double)(2 * (int)__popcll(x) - nsite);  // SpinGC: 2 Sz(x)
+  if (kind == 2) return (double)(int)__popcll(x & (x >> 1) & 0x5555555555555555ull);  // Hubbard: doublons
+  return 0.0;
+}
+
+// step s: var += |h|^2, obs moments of w, w_next = (l w - h/Ns) * inv_norm, |w_next|^2
+__global__ __launch_bounds__(TQ_TPB) void k_tpq_fused(double2 *v1, const double2 *v0, uint64_t dim, int k,
+                                                      uint64_t sj, uint64_t sc, const double *state, double lv,
+                                                      double invNs, int obs_kind, int nsite,
+                                                      const uint64_t *list1, double *partial) {
+  const int c = threadIdx.x % k, jl = threadIdx.x / k, spb = TQ_TPB / k;
+  const double sc_c = state[16 + c];  // 1/|w_s|
+  double r[TQ_NRED] = {0.0, 0.0, 0.0, 0.0};
+  for (uint64_t j = (uint64_t)blockIdx.x * spb + jl; j < dim; j += (uint64_t)gridDim.x * spb) {
+    const uint64_t idx = j * sj + c * sc;
+    const double2 w = v1[idx], h = v0[idx];
+    r[0] += h.x * h.x + h.y * h.y;
+    if (obs_kind) {
+      const double ww = w.x * w.x + w.y * w.y;
+      const double o = tq_obs(obs_kind, list1 ? list1[j] : j, nsite);
+      r[1] += ww * o;
+      r[2] += ww * o * o;
+    }
+    const double2 wn = make_double2((lv * w.x - h.x * invNs) * sc_c, (lv * w.y - h.y * invNs) * sc_c);
+    v1[idx] = wn;
+    r[3] += wn.x * wn.x + wn.y * wn.y;
+  }
+  __shared__ double red[TQ_TPB / 64][16][TQ_NRED];
+#pragma unroll
+  for (int q = 0; q < TQ_NRED; q++) {
+    double s = r[q];
+    for (int off = 32; off >= k; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) < k) red[threadIdx.x >> 6][threadIdx.x & 63][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < k * TQ_NRED) {
+    const int cc = threadIdx.x / TQ_NRED, q = threadIdx.x % TQ_NRED;
+    double t = 0.0;
+    for (int w = 0; w < TQ_TPB / 64; w++) t += red[w][cc][q];
+    partial[((uint64_t)blockIdx.x * k + cc) * TQ_NRED + q] = t;
+  }
+}
+
+// state layout (doubles): [0..16) |w_s|^2, [16..32) 1/|w_s|, [32] step counter
+__global__ void k_tpq_start(const double *partial, int nblk, int k, double *state) {
+  const int c = threadIdx.x;
+  if (c < k) {
+    double n = 0.0;
+    for (int b = 0; b < nblk; b++) n += partial[(uint64_t)b * k + c];
+    state[c] = n;
+    state[16 + c] = 1.0 / sqrt(n);
+  }
+  if (c == 0) state[32] = 0.0;
+}
+
+// one block: output row of step s, next scale
+__global__ void k_tpq_finalize(const double *partial, int nblk, int k, const double2 *prdct, double *state,
+                               double lv, double Ns, double num_const, double *out, int max_steps) {
+  const int c = threadIdx.x;
+  const int s = (int)state[32];
+  __syncthreads();
+  if (c < k) {
+    double r[TQ_NRED] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < nblk; b++)
+      for (int q = 0; q < TQ_NRED; q++) r[q] += partial[((uint64_t)b * k + c) * TQ_NRED + q];
+    const double n = state[c];
+    const double E = prdct[c].x / n;
+    const double var = r[0] / n;
+    if (s < max_steps) {
+      double *o = out + ((uint64_t)s * k + c) * 8;
+      o[0] = (s == 0) ? 0.0 : (2.0 * s / Ns) / (lv - E / Ns);
+      o[1] = E;
+      o[2] = var;
+      o[3] = sqrt(n);
+      o[4] = r[1] / n;  // SpinGC: 2<Sz>; Hubbard: <doublon>
+      o[5] = r[2] / n;  // SpinGC: 4<Sz^2>; Hubbard: <doublon^2>
+      o[6] = num_const;
+      o[7] = prdct[c].y / n;
+    }
+    state[c] = r[3];
+    state[16 + c] = 1.0 / sqrt(r[3]);
+  }
+  __syncthreads();
+  if (c == 0) state[32] = (double)(s + 1);
+}
+
+// max_j |list_Diagonal[j]| (for l = 3 max|D| / Ns)
+__global__ __launch_bounds__(256) void k_maxabs(const double *d, uint64_t n, double *partial) {
+  double m = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    m = fmax(m, fabs(d[i]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__global__ void k_copy_prdct(const double2 *src, double2 *dst, int c) { dst[c] = src[0]; }
+
+// ------------------------------------------------------------------ host --
+static int tq_grid(uint64_t dim, int k) {
+  uint64_t spb = TQ_TPB / k;
+  uint64_t g = (dim + spb - 1) / spb;
+  if (g > TQ_GRID) g = TQ_GRID;
+  return (int)(g < 1 ? 1 : g);
+}
+
+extern "C" int hphi_diag_maxabs(hphi_engine *e, double *out) {
+  if (!e || !out) return hphi_set_error("hphi_diag_maxabs: NULL argument");
+  if (hphi_build_diag(e)) return -1;
+  int g = (int)std::min<uint64_t>((e->dim + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_maxabs, dim3(g), dim3(256), 0, e->stream, e->d_diag, e->dim, (double *)e->d_partial);
+  std::vector<double> h(g);
+  HIPCHK(hipMemcpyAsync(h.data(), e->d_partial, sizeof(double) * g, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  double m = 0.0;
+  for (double x : h) m = std::max(m, x);
+  *out = m;
+  return 0;
+}
+
+struct TpqPlan {
+  double *V0, *V1, *state, *part, *out;
+  int k, layout, g, grid_hv;
+  uint64_t sj, sc;
+  double lv, Ns, num_const;
+  int obs_kind, steps;
+};
+
+// enqueue one step on e->stream (capturable: kernels only)
+static int tpq_enqueue_step(hphi_engine *e, const TpqPlan &P) {
+  const uint64_t dim = e->dim;
+  if (P.layout == 1) {
+    if (hphi_hv_launch_ex(e, P.k, P.V0, P.V1, 1)) return -1;
+  } else {
+    // vector-major: one single-vector H·v per column; their prdct go to columns of a
+    // per-column buffer through the shared reduction kernel
+    for (int c = 0; c < P.k; c++) {
+      if (hphi_hv_launch_ex(e, 1, P.V0 + 2 * (uint64_t)c * dim, P.V1 + 2 * (uint64_t)c * dim, 1)) return -1;
+      hipLaunchKernelGGL(k_copy_prdct, dim3(1), dim3(1), 0, e->stream, (const double2 *)e->d_prdct,
+                         (double2 *)e->d_coef, c);
+    }
+  }
+  hipLaunchKernelGGL(k_tpq_fused, dim3(P.g), dim3(TQ_TPB), 0, e->stream, (double2 *)P.V1, (const double2 *)P.V0,
+                     dim, P.k, P.sj, P.sc, (const double *)P.state, P.lv, 1.0 / P.Ns, P.obs_kind, e->nsite,
+                     (e->model == HPHI_MODEL_SPINGC) ? (const uint64_t *)nullptr : (const uint64_t *)e->d_list1,
+                     P.part);
+  hipLaunchKernelGGL(k_tpq_finalize, dim3(1), dim3(64), 0, e->stream, (const double *)P.part, P.g, P.k,
+                     (const double2 *)(P.layout == 1 ? e->d_prdct : e->d_coef), P.state, P.lv, P.Ns, P.num_const,
+                     P.out, P.steps);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_host, hphi_tpq_result *r) {
+  if (!e || !p || !out_host) return hphi_set_error("hphi_tpq: NULL argument");
+  const int k = p->k;
+  if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return hphi_set_error("hphi_tpq: k must be 1, 2, 4, 8 or 16");
+  if (p->steps < 1) return hphi_set_error("hphi_tpq: steps must be positive");
+  if (hphi_build_tables(e) || hphi_build_diag(e)) return -1;
+  const uint64_t dim = e->dim;
+  TpqPlan P{};
+  P.k = k;
+  P.layout = (p->layout == 1) ? 1 : 0;
+  P.sj = P.layout == 1 ? (uint64_t)k : 1;
+  P.sc = P.layout == 1 ? 1 : dim;
+  P.Ns = (double)e->nsite;
+  P.steps = p->steps;
+  if (p->large_value > 0.0) {
+    P.lv = p->large_value;
+  } else {
+    double m = 0.0;
+    if (hphi_diag_maxabs(e, &m)) return -1;
+    P.lv = 3.0 * m / P.Ns;  // l = 3 max_j |list_Diagonal[j]| / Ns
+  }
+  P.obs_kind = (e->model == HPHI_MODEL_SPINGC) ? 1 : (e->model == HPHI_MODEL_HUBBARD ? 2 : 0);
+  P.num_const = (e->model == HPHI_MODEL_HUBBARD) ? (double)(e->nup + e->ndown) : (double)e->nsite;
+  P.g = tq_grid(dim, k);
+  size_t vbytes = sizeof(double2) * dim * k;
+  HIPCHK(hipMalloc(&P.V0, vbytes));
+  HIPCHK(hipMalloc(&P.V1, vbytes));
+  HIPCHK(hipMalloc(&P.state, sizeof(double) * 64));
+  HIPCHK(hipMalloc(&P.part, sizeof(double) * (size_t)TQ_GRID * k * TQ_NRED));
+  HIPCHK(hipMalloc(&P.out, sizeof(double) * (size_t)p->steps * k * 8));
+  int rc = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0, ev1;
+  (void)hipEventCreate(&ev0);
+  (void)hipEventCreate(&ev1);
+  do {
+    uint64_t n = dim * (uint64_t)k;
+    hipLaunchKernelGGL(k_tpq_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, (double2 *)P.V1, dim, k,
+                       P.sj, P.sc, p->seed, p->init_kind);
+    hipLaunchKernelGGL(k_tpq_colnorm, dim3(P.g), dim3(TQ_TPB), 0, e->stream, (const double2 *)P.V1, dim, k, P.sj,
+                       P.sc, P.part);
+    hipLaunchKernelGGL(k_tpq_start, dim3(1), dim3(64), 0, e->stream, (const double *)P.part, P.g, k, P.state);
+    if (hipGetLastError() != hipSuccess) { rc = hphi_set_error("hphi_tpq: init launch failed"); break; }
+    if (hipEventRecord(ev0, e->stream) != hipSuccess) { rc = hphi_set_error("event record failed"); break; }
+    int s0 = 0;
+    if ((rc = tpq_enqueue_step(e, P))) break;  // step 0 eagerly (also loads the code objects)
+    s0 = 1;
+    if (p->use_graph && p->steps > 1) {
+      if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        rc = hphi_set_error("hipStreamBeginCapture failed");
+        break;
+      }
+      int erc = tpq_enqueue_step(e, P);
+      hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+      if (erc || ce != hipSuccess) { rc = hphi_set_error("graph capture of the TPQ step failed"); break; }
+      if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        rc = hphi_set_error("hipGraphInstantiate failed");
+        break;
+      }
+      for (int s = s0; s < p->steps; s++)
+        if (hipGraphLaunch(exec, e->stream) != hipSuccess) { rc = hphi_set_error("hipGraphLaunch failed"); break; }
+    } else {
+      for (int s = s0; s < p->steps && !rc; s++) rc = tpq_enqueue_step(e, P);
+    }
+    if (rc) break;
+    if (hipEventRecord(ev1, e->stream) != hipSuccess) { rc = hphi_set_error("event record failed"); break; }
+    if (hipStreamSynchronize(e->stream) != hipSuccess) { rc = hphi_set_error("hphi_tpq: stream sync failed"); break; }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    if (hipMemcpy(out_host, P.out, sizeof(double) * (size_t)p->steps * k * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = hphi_set_error("hphi_tpq: copy of results failed");
+      break;
+    }
+    if (r) {
+      r->seconds = ms * 1e-3;
+      r->large_value = P.lv;
+      r->steps = p->steps;
+    }
+  } while (0);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  (void)hipFree(P.V0);
+  (void)hipFree(P.V1);
+  (void)hipFree(P.state);
+  (void)hipFree(P.part);
+  (void)hipFree(P.out);
+  return rc;
+}

@@ -167,6 +167,29 @@ typedef struct {
 } hphi_lanczos_result;
 
 int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r);
+/* Microcanonical TPQ (CalcByTPQ, src/CalcByTPQ.c:52-294) for k samples at once.
+ * Sample c starts from a random vector of seed + c (init_kind 0: uniform
+ * complex, -1: complex Gaussian); l = large_value, or 3 max|list_Diagonal|/Nsite
+ * when large_value <= 0.  layout 1 keeps the k vectors state-major and uses
+ * the block H·v; layout 0 keeps them vector-major with one H·v per sample.
+ * use_graph captures one step in a hipGraph and replays it.
+ * out[(s*k + c)*8 + q], s = 0..steps-1 (the reference's step_i), q:
+ *   0 inv_temp, 1 energy, 2 <H^2>, 3 norm (global_norm; step 0: global_1st_norm),
+ *   4 SpinGC 2<Sz> / Hubbard <doublon>, 5 SpinGC 4<Sz^2> / Hubbard <doublon^2>,
+ *   6 particle number, 7 Im <H> (rounding check). */
+typedef struct {
+  int k, steps, init_kind, layout, use_graph;
+  double large_value;
+  uint64_t seed;
+} hphi_tpq_params;
+typedef struct {
+  double seconds, large_value;
+  int steps;
+} hphi_tpq_result;
+int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out, hphi_tpq_result *r);
+/* max_j |list_Diagonal[j]| */
+int hphi_diag_maxabs(hphi_engine *e, double *out);
+
 /* k lowest eigenvalues (ascending) of the symmetric tridiagonal matrix with
  * diagonal a[0..n) and off-diagonal b[0..n-1), by Sturm bisection (host only;
  * replaces DSEVvalue -> dsyev_, src/matrixlapack.c:93-150). */

@@ -199,3 +199,43 @@ def test_every_tuning_matches_oracle(A, oracle_lib, name, tuning):
     assert rel_inf(hv, ohv) <= HV_TOL
     assert prdct_ok(pr, opr, v, ohv)
     eng.close()
+
+
+@pytest.mark.parametrize("layout,graph", [(1, True), (0, True), (1, False)])
+@pytest.mark.parametrize("name,k", [("tri_4x3_gc", 4), ("hubbard_4x2", 2), ("j1j2_4x4", 2)])
+def test_tpq_matches_restated_reference_step(A, oracle_lib, name, k, layout, graph):
+    """Per-step TPQ output (inv_temp, E, <H^2>, norm, Sz moments) equals the
+    reference's step restated in numpy on the oracle's H·v, same initial vectors."""
+    from tests.tpq_restated import tpq_restated
+    print(f"[gpu] tpq {name} k={k} layout={layout} graph={graph}", flush=True)
+    spec = golden_cases()[name]
+    eng = A.Engine.from_spec(spec)
+    steps = 25
+    out, res = eng.tpq(k, steps, seed=3, init_kind=0, layout=layout, use_graph=graph)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    lv = 3.0 * np.max(np.abs(orc.diag[1:])) / spec.nsite
+    assert abs(res.large_value - lv) <= 1e-15 * lv
+    sz = None  # per-state observable: SpinGC 2 Sz(x), Hubbard doublon count
+    if spec.model == L.SPIN_GC:
+        sz = np.array([2 * bin(x).count("1") - spec.nsite for x in range(orc.dim)], float)
+    elif spec.model == L.HUBBARD:
+        sz = np.array([bin(int(x) & (int(x) >> 1) & 0x5555555555555555).count("1")
+                       for x in orc.list_1[1:]], float)
+    hv = lambda v: orc.hv(v)[0]
+    for c in range(k):
+        ref = tpq_restated(hv, oracle_lib.fill_vector(3 + c, orc.dim), steps, lv, spec.nsite, sz)
+        got = out[:, c, :6]
+        assert np.allclose(got[:, :4], ref[:, :4], rtol=1e-10, atol=1e-12), (got[:3], ref[:3])
+        if sz is not None:
+            assert np.allclose(got[:, 4:6], ref[:, 4:6], rtol=1e-10, atol=1e-12)
+        assert np.all(np.abs(out[:, c, 7]) <= 1e-10 * np.abs(out[:, c, 1]) + 1e-12)  # <H> real
+    eng.close()
+
+
+def test_tpq_gaussian_start_is_normalised(A):
+    spec = golden_cases()["tri_4x3_gc"]
+    eng = A.Engine.from_spec(spec)
+    out, _ = eng.tpq(4, 3, seed=0, init_kind=-1, layout=1)
+    # |w_0|^2 of a complex Gaussian vector ~ 2 dim
+    assert np.all(np.abs(out[0, :, 3] ** 2 / (2 * eng.dim) - 1.0) < 0.1)
+    eng.close()

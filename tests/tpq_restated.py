@@ -1,0 +1,27 @@
+"""The reference's mTPQ step restated in numpy on top of the CPU oracle's H·v
+(test helper): MakeIniVec normalisation (src/MakeIniVec.c:95-116), FirstMultiply
+(src/FirstMultiply.c:41-85), Multiply (src/Multiply.c:42-66), expec_energy_flct
+(src/expec_energy_flct.c:126-157,429-500) and inv_temp (src/CalcByTPQ.c:235)."""
+
+import numpy as np
+
+
+def tpq_restated(hv, v_init, steps, lv, Ns, sz_of=None):
+    """Rows (inv_temp, energy, <H^2>, norm, 2<Sz>, 4<Sz^2>) of one sample."""
+    rows = []
+    first_norm = np.linalg.norm(v_init)
+    v1 = v_init / first_norm
+    h = hv(v1)
+    obs = lambda v: ((np.sum(np.abs(v) ** 2 * sz_of), np.sum(np.abs(v) ** 2 * sz_of ** 2))
+                     if sz_of is not None else (0.0, 0.0))
+    o = obs(v1)
+    rows.append((0.0, np.vdot(v1, h).real, np.vdot(h, h).real, first_norm, *o))
+    for s in range(1, steps):
+        v0 = lv * v1 - h / Ns
+        gnorm = np.linalg.norm(v0)
+        v1 = v0 / gnorm
+        h = hv(v1)
+        E = np.vdot(v1, h).real
+        o = obs(v1)
+        rows.append((2.0 * s / Ns / (lv - E / Ns), E, np.vdot(h, h).real, gnorm, *o))
+    return np.array(rows)
