@@ -30,6 +30,9 @@ EXPORTED = [
     "hphi_vec_axpby", "hphi_engine_sync", "hphi_time_mltply_dev", "hphi_lanczos",
     "hphi_engine_tune", "hphi_engine_get_tune", "hphi_autotune", "hphi_tpq", "hphi_diag_maxabs",
     "hphi_read_def", "hphi_model_desc_free", "hphi_tridiag_lowest",
+    "hphi_lanczos_good_eigenvalues", "hphi_lanczos_converged",
+    "hphi_server_create", "hphi_server_destroy", "hphi_server_submit", "hphi_server_tick",
+    "hphi_server_results", "hphi_server_now",
 ]
 
 
@@ -52,7 +55,7 @@ class ModPara(C.Structure):
 
 class LanczosParams(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("n_eigen", C.c_int), ("eps", C.c_double),
-                ("target", C.c_int), ("seed", C.c_uint64), ("check_every", C.c_int)]
+                ("target", C.c_int), ("seed", C.c_uint64), ("check_every", C.c_int), ("mode", C.c_int)]
 
 
 class LanczosResult(C.Structure):
@@ -67,6 +70,17 @@ class TpqParams(C.Structure):
 
 class TpqResult(C.Structure):
     _fields_ = [("seconds", C.c_double), ("large_value", C.c_double), ("steps", C.c_int)]
+
+
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_jobs", C.c_int), ("n_eigen", C.c_int),
+                ("eps", C.c_double), ("max_iter", C.c_int), ("check_every", C.c_int)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("id", C.c_int), ("n_found", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
+                ("dim", C.c_uint64), ("eigenvalues", C.c_double * 16), ("t_arrival", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
 
 
 _lib = None
@@ -115,6 +129,14 @@ def lib():
         L.hphi_autotune.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_char_p, C.c_size_t]
         L.hphi_tpq.argtypes = [_vp, C.POINTER(TpqParams), _f64p, C.POINTER(TpqResult)]
         L.hphi_diag_maxabs.argtypes = [_vp, _f64p]
+        L.hphi_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(_vp)]
+        L.hphi_server_destroy.argtypes = [_vp]
+        L.hphi_server_destroy.restype = None
+        L.hphi_server_submit.argtypes = [_vp, C.POINTER(ModelDesc), C.c_uint64, C.c_double, C.POINTER(C.c_int)]
+        L.hphi_server_tick.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.hphi_server_results.argtypes = [_vp, C.POINTER(JobResult), C.c_int, C.POINTER(C.c_int)]
+        L.hphi_server_now.argtypes = [_vp]
+        L.hphi_server_now.restype = C.c_double
         _lib = L
     return _lib
 
@@ -191,6 +213,22 @@ class Engine:
         h = _vp()
         check(lib().hphi_engine_create(C.byref(d), C.byref(h)))
         return cls(h)
+
+    @staticmethod
+    def desc_from_terms(model, nsite, nup, ndown, t):
+        """(ModelDesc, keep-alive arrays) for the C ABI."""
+        keep = {k: np.ascontiguousarray(v) for k, v in t.items()}
+        d = ModelDesc()
+        d.model, d.nsite, d.nup, d.ndown = model, nsite, nup, ndown
+        d.n_exchange, d.exchange_ij, d.exchange_J = len(keep["exchange_J"]), _p(keep["exchange_ij"], _i32p), _p(keep["exchange_J"], _f64p)
+        d.n_pairlift, d.pairlift_ij, d.pairlift_J = len(keep["pairlift_J"]), _p(keep["pairlift_ij"], _i32p), _p(keep["pairlift_J"], _f64p)
+        d.n_transfer, d.transfer_idx, d.transfer_t = len(keep["trans_t"]), _p(keep["trans_idx"], _i32p), _p(keep["trans_t"], _f64p)
+        d.n_chemi, d.chemi_site, d.chemi_spin, d.chemi_mu = (len(keep["chemi_mu"]), _p(keep["chemi_site"], _i32p),
+                                                             _p(keep["chemi_spin"], _i32p), _p(keep["chemi_mu"], _f64p))
+        d.n_intra, d.intra_site, d.intra_U = len(keep["intra_U"]), _p(keep["intra_site"], _i32p), _p(keep["intra_U"], _f64p)
+        d.n_inter, d.inter_ij, d.inter_V = len(keep["inter_V"]), _p(keep["inter_ij"], _i32p), _p(keep["inter_V"], _f64p)
+        d.n_hund, d.hund_ij, d.hund_J = len(keep["hund_J"]), _p(keep["hund_ij"], _i32p), _p(keep["hund_J"], _f64p)
+        return d, keep
 
     @classmethod
     def from_spec(cls, spec) -> "Engine":
@@ -300,8 +338,8 @@ class Engine:
         check(lib().hphi_fill_random(v.ptr, self.dim, k, seed))
 
     def lanczos(self, n_eigen: int = 1, max_iter: int = 2000, eps: float = 1e-14, target: int = 2,
-                seed: int = 0, check_every: int = 1) -> LanczosResult:
-        p = LanczosParams(max_iter, n_eigen, eps, target, seed, check_every)
+                seed: int = 0, check_every: int = 1, mode: int = 0) -> LanczosResult:
+        p = LanczosParams(max_iter, n_eigen, eps, target, seed, check_every, mode)
         r = LanczosResult()
         check(lib().hphi_lanczos(self.h, C.byref(p), C.byref(r)))
         return r
@@ -316,3 +354,46 @@ def tridiag_lowest(a: np.ndarray, b: np.ndarray, k: int) -> np.ndarray:
     ev = np.zeros(k)
     L.hphi_tridiag_lowest(_p(a, _f64p), _p(b, _f64p) if b.size else None, len(a), k, _p(ev, _f64p))
     return ev
+
+
+class Server:
+    """Continuous-batching Lanczos service (hphi_server_* in the C ABI)."""
+
+    def __init__(self, hbm_budget: int = 16 << 30, max_jobs: int = 64, n_eigen: int = 4,
+                 eps: float = 1e-12, max_iter: int = 2000, check_every: int = 5):
+        p = ServerParams(hbm_budget, max_jobs, n_eigen, eps, max_iter, check_every)
+        self.h = _vp()
+        check(lib().hphi_server_create(C.byref(p), C.byref(self.h)))
+
+    def now(self) -> float:
+        return lib().hphi_server_now(self.h)
+
+    def submit(self, spec, seed: int, t_arrival: float) -> int:
+        from .lattice import processed_terms
+        d, keep = Engine.desc_from_terms(spec.model, spec.nsite, spec.nup, spec.ndown, processed_terms(spec))
+        i = C.c_int()
+        check(lib().hphi_server_submit(self.h, C.byref(d), seed, t_arrival, C.byref(i)))
+        del keep
+        return i.value
+
+    def tick(self):
+        a, w = C.c_int(), C.c_int()
+        check(lib().hphi_server_tick(self.h, C.byref(a), C.byref(w)))
+        return a.value, w.value
+
+    def results(self, max_n: int = 4096):
+        buf = (JobResult * max_n)()
+        n = C.c_int()
+        check(lib().hphi_server_results(self.h, buf, max_n, C.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def close(self):
+        if self.h:
+            lib().hphi_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

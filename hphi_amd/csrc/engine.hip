@@ -54,7 +54,7 @@ __device__ __forceinline__ uint64_t dbinom(int n, int k) {
   return (k < 0 || n < 0 || k > n) ? 0ull : c_binom[n][k];
 }
 
-static int ensure_binom_table() {
+int hphi_ensure_binom_table() {
   static bool done = false;
   if (done) return 0;
   static uint64_t t[65][65];
@@ -267,46 +267,14 @@ __global__ void k_diag(uint64_t dim, const uint64_t *__restrict__ list1, int gc,
 // Loads are made unconditional (inactive lanes re-read their own element and
 // weight it by zero) so the unrolled term loop keeps many gathers in flight.
 
-template <int K>
-__device__ __forceinline__ void wg_reduce_store(double pre, double pim, double2 *partial) {
-  // lanes with equal (lane % K) hold the same vector column
-#pragma unroll
-  for (int off = 32; off >= K; off >>= 1) {
-    pre += __shfl_xor(pre, off, 64);
-    pim += __shfl_xor(pim, off, 64);
-  }
-  __shared__ double2 red[HV_TPB / 64][K];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane < K) red[wave][lane] = make_double2(pre, pim);
-  __syncthreads();
-  if (threadIdx.x < K) {
-    double2 s = red[0][threadIdx.x];
-#pragma unroll
-    for (int w = 1; w < HV_TPB / 64; w++) {
-      s.x += red[w][threadIdx.x].x;
-      s.y += red[w][threadIdx.x].y;
-    }
-    partial[(uint64_t)blockIdx.x * K + threadIdx.x] = s;
-  }
-}
-
-__device__ __forceinline__ bool term_active(const GTerm &T, uint64_t x) {
-  uint64_t ym = x & T.tmask;
-  int pc = __popcll(ym);
-  return (T.kind == P_EQ) ? (ym == T.tval) : ((T.kind == P_DIFF) ? (pc == 1) : (pc != 1));
-}
+#include "hv_device.h"
 
 template <int K, bool CANON>
 __global__ __launch_bounds__(HV_TPB) void k_hv(const HvArgs A) {
-  constexpr int SPT = HV_TPB / K;  // states per tile
-  const int vec = threadIdx.x & (K - 1);
-  const int sl = threadIdx.x / K;
-  const double2 *__restrict__ v1 = (const double2 *)A.v1;
-  double2 *__restrict__ v0 = (double2 *)A.v0;
+  constexpr int SPT = HV_TPB / K;
   double pre = 0.0, pim = 0.0;
-
   const uint64_t ntiles = (A.dim + SPT - 1) / SPT;
-  uint64_t t_begin, t_end, t_step, tile;
+  uint64_t t_begin, t_end, t_step;
   if (A.xcd_map) {
     const uint32_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
     t_begin = (ntiles * xcd) / 8 + slot;
@@ -317,125 +285,7 @@ __global__ __launch_bounds__(HV_TPB) void k_hv(const HvArgs A) {
     t_end = ntiles;
     t_step = gridDim.x;
   }
-  for (tile = t_begin; tile < t_end; tile += t_step) {
-    const uint64_t j = tile * SPT + sl;
-    if (j >= A.dim) continue;
-    const uint64_t x = CANON ? A.list1[j] : j;
-    const double2 a = v1[j * K + vec];
-    const double d = A.diag[j];
-    double ar = d * a.x, ai = d * a.y;
-    if (CANON) {
-      const uint64_t ia0 = x & A.irght, ib0 = x >> A.nlow;
-      const int64_t b21 = (int64_t)A.l21[ia0], b22 = (int64_t)A.l22[ib0];
-      // low group: only the low half changes, same block
-      for (int t = 0; t < A.n_low; t += HV_UNROLL) {
-        double2 g[HV_UNROLL];
-        double cr[HV_UNROLL], ci[HV_UNROLL];
-#pragma unroll
-        for (int u = 0; u < HV_UNROLL; u++) {
-          const GTerm T = load_term((CTerm)A.terms + t + u);
-          const uint64_t y = x ^ T.flip;
-          const int64_t l = (int64_t)A.l21[y & A.irght];
-          bool act = term_active(T, x) && (l != 0);
-          const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
-          uint64_t idx = (uint64_t)((int64_t)j + l - b21);
-          act = act && idx < A.dim;
-          idx = act ? idx : j;
-          g[u] = v1[idx * K + vec];
-          cr[u] = act ? s * T.cre : 0.0;
-          ci[u] = act ? s * T.cim : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < HV_UNROLL; u++) {
-          ar += cr[u] * g[u].x - ci[u] * g[u].y;
-          ai += cr[u] * g[u].y + ci[u] * g[u].x;
-        }
-      }
-      // high group: only the high half changes, same rank in another block
-      const CTerm th = (CTerm)A.terms + A.n_low;
-      for (int t = 0; t < A.n_high; t += HV_UNROLL) {
-        double2 g[HV_UNROLL];
-        double cr[HV_UNROLL], ci[HV_UNROLL];
-#pragma unroll
-        for (int u = 0; u < HV_UNROLL; u++) {
-          const GTerm T = load_term(th + t + u);
-          const uint64_t y = x ^ T.flip;
-          const int64_t l = (int64_t)A.l22[y >> A.nlow];
-          bool act = term_active(T, x) && (l != 0);
-          const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
-          uint64_t idx = (uint64_t)((int64_t)j + l - b22);
-          act = act && idx < A.dim;
-          idx = act ? idx : j;
-          g[u] = v1[idx * K + vec];
-          cr[u] = act ? s * T.cre : 0.0;
-          ci[u] = act ? s * T.cim : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < HV_UNROLL; u++) {
-          ar += cr[u] * g[u].x - ci[u] * g[u].y;
-          ai += cr[u] * g[u].y + ci[u] * g[u].x;
-        }
-      }
-      // mixed group: both halves change, two lookups
-      const CTerm tm = th + A.n_high;
-      for (int t = 0; t < A.n_mixed; t += HV_UNROLL) {
-        double2 g[HV_UNROLL];
-        double cr[HV_UNROLL], ci[HV_UNROLL];
-#pragma unroll
-        for (int u = 0; u < HV_UNROLL; u++) {
-          const GTerm T = load_term(tm + t + u);
-          const uint64_t y = x ^ T.flip;
-          const int64_t la = (int64_t)A.l21[y & A.irght];
-          const int64_t lb = (int64_t)A.l22[y >> A.nlow];
-          bool act = term_active(T, x) && (la != 0) && (lb != 0);
-          const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
-          uint64_t idx = (uint64_t)(la + lb - 3);
-          act = act && idx < A.dim;
-          idx = act ? idx : j;
-          g[u] = v1[idx * K + vec];
-          cr[u] = act ? s * T.cre : 0.0;
-          ci[u] = act ? s * T.cim : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < HV_UNROLL; u++) {
-          ar += cr[u] * g[u].x - ci[u] * g[u].y;
-          ai += cr[u] * g[u].y + ci[u] * g[u].x;
-        }
-      }
-    } else {
-      for (int t = 0; t < A.n_low; t += HV_UNROLL) {
-        double2 g[HV_UNROLL];
-        double cr[HV_UNROLL], ci[HV_UNROLL];
-#pragma unroll
-        for (int u = 0; u < HV_UNROLL; u++) {
-          const GTerm T = load_term((CTerm)A.terms + t + u);
-          bool act = term_active(T, x);
-          const double s = (__popcll(x & T.sgn) & 1) ? -1.0 : 1.0;
-          uint64_t idx = x ^ T.flip;
-          act = act && idx < A.dim;
-          idx = act ? idx : j;
-          g[u] = v1[idx * K + vec];
-          cr[u] = act ? s * T.cre : 0.0;
-          ci[u] = act ? s * T.cim : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < HV_UNROLL; u++) {
-          ar += cr[u] * g[u].x - ci[u] * g[u].y;
-          ai += cr[u] * g[u].y + ci[u] * g[u].x;
-        }
-      }
-    }
-    double2 o = make_double2(ar, ai);
-    if (!A.overwrite) {
-      const double2 p = v0[j * K + vec];
-      o.x += p.x;
-      o.y += p.y;
-    }
-    v0[j * K + vec] = o;
-    // conj(a) * acc
-    pre += a.x * ar + a.y * ai;
-    pim += a.x * ai - a.y * ar;
-  }
+  for (uint64_t tile = t_begin; tile < t_end; tile += t_step) hv_tile<K, CANON>(A, tile, pre, pim);
   wg_reduce_store<K>(pre, pim, A.partial);
 }
 
@@ -534,23 +384,20 @@ static void classify_and_upload_terms(hphi_engine *e) {
   e->h_terms_dev.insert(e->h_terms_dev.end(), mixed.begin(), mixed.end());
 }
 
-extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
-  if (!d || !out) return hphi_set_error("hphi_engine_create: NULL argument");
-  *out = nullptr;
-  if (ensure_binom_table()) return -1;
-  hphi_engine *e = new hphi_engine();
+// Host-side preparation: sector sizes, split masks, gather terms (grouped
+// low/high/mixed) and diagonal terms.  No device work.
+int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
   e->model = d->model;
   e->nsite = d->nsite;
   e->nup = d->nup;
   e->ndown = d->ndown;
   const int N = d->nsite;
   if (N < 2) {
-    delete e;
     return hphi_set_error("Nsite must be at least 2");
   }
   switch (d->model) {
     case HPHI_MODEL_SPIN:  // check.c:177, check.c:310-316, bitcalc.c:45-62
-      if (N > 64 || d->nup < 0 || d->nup > N) { delete e; return hphi_set_error("bad Spin sector"); }
+      if (N > 64 || d->nup < 0 || d->nup > N) { return hphi_set_error("bad Spin sector"); }
       e->nbits = N;
       e->dim = h_binom(N, d->nup);
       e->sdim = 1ull << (N / 2);
@@ -559,7 +406,6 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
       break;
     case HPHI_MODEL_HUBBARD:  // check.c:114-118, check.c:301-307
       if (N > 32 || d->nup < 0 || d->nup > N || d->ndown < 0 || d->ndown > N) {
-        delete e;
         return hphi_set_error("bad Hubbard sector");
       }
       e->nbits = 2 * N;
@@ -569,31 +415,29 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
       e->size22 = e->sdim + 2;
       break;
     case HPHI_MODEL_SPINGC:
-      if (N > 40) { delete e; return hphi_set_error("SpinGC Nsite too large"); }
+      if (N > 40) { return hphi_set_error("SpinGC Nsite too large"); }
       e->nbits = N;
       e->dim = 1ull << N;
       e->sdim = 0;
       e->size21 = e->size22 = 0;
       break;
     default:
-      delete e;
       return hphi_set_error("unsupported CalcModel %d (supported: 0 Hubbard, 1 Spin, 4 SpinGC)", d->model);
   }
   e->nlow = (e->nbits + 1) / 2;
   e->ihfbit = 1ull << e->nlow;
   e->irght = e->ihfbit - 1;
   e->ilft = ((e->nbits >= 64) ? ~0ull : ((1ull << e->nbits) - 1)) ^ e->irght;
-  if (e->dim == 0) { delete e; return hphi_set_error("empty sector"); }
+  if (e->dim == 0) { return hphi_set_error("empty sector"); }
 
   // ---- off-diagonal terms (gather form) ----
   const int gc = (d->model == HPHI_MODEL_SPINGC);
   for (int t = 0; t < d->n_exchange; t++) {  // mltplySpinCore.c:273-345
     int i = d->exchange_ij[2 * t], j = d->exchange_ij[2 * t + 1];
     if (d->model == HPHI_MODEL_HUBBARD) {
-      delete e;
       return hphi_set_error("Hubbard Exchange terms are not supported by this engine");
     }
-    if (i < 0 || j < 0 || i >= N || j >= N || i == j) { delete e; return hphi_set_error("bad Exchange sites"); }
+    if (i < 0 || j < 0 || i >= N || j >= N || i == j) { return hphi_set_error("bad Exchange sites"); }
     GTerm g{};
     g.flip = (1ull << i) | (1ull << j);
     g.tmask = g.flip;
@@ -602,9 +446,9 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
     e->h_terms.push_back(g);
   }
   for (int t = 0; t < d->n_pairlift; t++) {  // mltplySpinCore.c:352-382
-    if (!gc) { delete e; return hphi_set_error("PairLift is active only in SpinGC"); }
+    if (!gc) { return hphi_set_error("PairLift is active only in SpinGC"); }
     int i = d->pairlift_ij[2 * t], j = d->pairlift_ij[2 * t + 1];
-    if (i < 0 || j < 0 || i >= N || j >= N || i == j) { delete e; return hphi_set_error("bad PairLift sites"); }
+    if (i < 0 || j < 0 || i >= N || j >= N || i == j) { return hphi_set_error("bad PairLift sites"); }
     GTerm g{};
     g.flip = (1ull << i) | (1ull << j);
     g.tmask = g.flip;
@@ -617,7 +461,6 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
     int i2 = d->transfer_idx[4 * t + 2], s2 = d->transfer_idx[4 * t + 3];
     double tre = d->transfer_t[2 * t], tim = d->transfer_t[2 * t + 1];
     if (i1 < 0 || i2 < 0 || i1 >= N || i2 >= N || s1 < 0 || s1 > 1 || s2 < 0 || s2 > 1) {
-      delete e;
       return hphi_set_error("bad transfer indices");
     }
     GTerm g{};
@@ -628,7 +471,6 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
       int a = 2 * i1 + s1, b = 2 * i2 + s2;
       if (a == b) continue;  // diagonal: EDChemi, not a transfer
       if (s1 != s2) {
-        hphi_engine_destroy(e);
         return hphi_set_error("spin-flip transfer (%d,%d,%d,%d) breaks the canonical Hubbard sector", i1, s1, i2, s2);
       }
       int lo = std::min(a, b), hi = std::max(a, b);
@@ -713,6 +555,18 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
     e->h_dterms.push_back(x);
   }
 
+  return 0;
+}
+
+extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
+  if (!d || !out) return hphi_set_error("hphi_engine_create: NULL argument");
+  *out = nullptr;
+  if (hphi_ensure_binom_table()) return -1;
+  hphi_engine *e = new hphi_engine();
+  if (hphi_engine_prepare(d, e)) {
+    delete e;
+    return -1;
+  }
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
     delete e;
     return hphi_set_error("hipStreamCreate failed");
@@ -827,7 +681,7 @@ extern "C" int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int 
   return 0;
 }
 
-static SzParams sz_params(const hphi_engine *e) {
+SzParams hphi_sz_params(const hphi_engine *e) {
   SzParams P{};
   P.hubbard = e->model == HPHI_MODEL_HUBBARD;
   P.nlow = e->nlow;
@@ -846,24 +700,37 @@ static SzParams sz_params(const hphi_engine *e) {
   return P;
 }
 
+// enqueue the basis-table kernels (block prefix + list_2_2, list_2_1, list_1) on `st`
+int hphi_launch_tables(const SzParams &P, uint64_t sdim, uint64_t size21, uint64_t *jb, uint32_t *l21,
+                       uint64_t *l22, uint64_t *list1, hipStream_t st) {
+  hipLaunchKernelGGL(k_block_prefix, dim3(grid_for(sdim + 1, 256)), dim3(256), 0, st, P, jb, l22);
+  hipLaunchKernelGGL(k_list21, dim3(grid_for(size21, 256)), dim3(256), 0, st, P, l21);
+  int g = (int)std::min<uint64_t>(sdim, 1u << 20);
+  hipLaunchKernelGGL(k_list1, dim3(g), dim3(256), 0, st, P, (const uint64_t *)jb, list1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int hphi_launch_diag(uint64_t dim, const uint64_t *list1, int gc, const DTerm *dterms, int nd, double *diag,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_diag, dim3(grid_for(dim, 256)), dim3(256), 0, st, dim, list1, gc, dterms, nd, diag);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int hphi_build_tables(hphi_engine *e) {
   if (e->tables_ready) return 0;
   if (e->model == HPHI_MODEL_SPINGC) {
     e->tables_ready = true;
     return 0;
   }
-  SzParams P = sz_params(e);
+  SzParams P = hphi_sz_params(e);
   HIPCHK(hipMalloc(&e->d_list1, sizeof(uint64_t) * e->dim));
   HIPCHK(hipMalloc(&e->d_l21, sizeof(uint32_t) * e->size21));
   HIPCHK(hipMalloc(&e->d_l22, sizeof(uint64_t) * e->size22));
   HIPCHK(hipMalloc(&e->d_jb, sizeof(uint64_t) * (e->sdim + 1)));
   e->bytes += sizeof(uint64_t) * (e->dim + e->size22 + e->sdim + 1) + sizeof(uint32_t) * e->size21;
-  hipLaunchKernelGGL(k_block_prefix, dim3(grid_for(e->sdim + 1, 256)), dim3(256), 0, e->stream, P,
-                     e->d_jb, e->d_l22);
-  hipLaunchKernelGGL(k_list21, dim3(grid_for(e->size21, 256)), dim3(256), 0, e->stream, P, e->d_l21);
-  int g = (int)std::min<uint64_t>(e->sdim, 1u << 20);
-  hipLaunchKernelGGL(k_list1, dim3(g), dim3(256), 0, e->stream, P, e->d_jb, e->d_list1);
-  HIPCHK(hipGetLastError());
+  if (hphi_launch_tables(P, e->sdim, e->size21, e->d_jb, e->d_l21, e->d_l22, e->d_list1, e->stream)) return -1;
   uint64_t total = 0;
   HIPCHK(hipMemcpyAsync(&total, e->d_jb + e->sdim, sizeof total, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -897,9 +764,9 @@ int hphi_build_diag(hphi_engine *e) {
   if (hphi_build_tables(e)) return -1;
   HIPCHK(hipMalloc(&e->d_diag, sizeof(double) * e->dim));
   e->bytes += sizeof(double) * e->dim;
-  hipLaunchKernelGGL(k_diag, dim3(grid_for(e->dim, 256)), dim3(256), 0, e->stream, e->dim, e->d_list1,
-                     e->model == HPHI_MODEL_SPINGC ? 1 : 0, e->d_dterms, (int)e->h_dterms.size(), e->d_diag);
-  HIPCHK(hipGetLastError());
+  if (hphi_launch_diag(e->dim, e->d_list1, e->model == HPHI_MODEL_SPINGC ? 1 : 0, e->d_dterms,
+                       (int)e->h_dterms.size(), e->d_diag, e->stream))
+    return -1;
   HIPCHK(hipStreamSynchronize(e->stream));
   e->diag_ready = true;
   return 0;

@@ -109,4 +109,9 @@ int hphi_hv_launch_ex(hphi_engine *e, int k, double *v0, const double *v1, int o
 int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct);
 int hphi_hv_grid(const hphi_engine *e, int k);
 int hphi_canon_config(hphi_engine *e);
+int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e);
+int hphi_ensure_binom_table();
+SzParams hphi_sz_params(const hphi_engine *e);
+int hphi_launch_tables(const SzParams &P, uint64_t sdim, uint64_t size21, uint64_t *jb, uint32_t *l21, uint64_t *l22, uint64_t *list1, hipStream_t st);
+int hphi_launch_diag(uint64_t dim, const uint64_t *list1, int gc, const DTerm *dterms, int nd, double *diag, hipStream_t st);
 int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g);

@@ -93,8 +93,8 @@ int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_resu
   double *alpha = (double *)calloc(maxit + 2, sizeof(double));
   double *beta = (double *)calloc(maxit + 2, sizeof(double));
   double *ev = (double *)calloc(maxit + 2, sizeof(double));
-  int rc = -1, stp = 1;
-  double pr[2], nrm, b2, ebefor = 0.0;
+  int rc = -1, stp = 1, nprev = 0;
+  double pr[2], nrm, b2, ebefor = 0.0, prevv[16];
 
   /* initial vector (SetInitialVector): random, normalised */
   if (hphi_fill_random(v1, dim, 1, p->seed) || hphi_vec_norm2(e, v1, 1, &nrm)) goto done;
@@ -128,6 +128,21 @@ int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_resu
     if (hphi_vec_lanczos_beta(e, v0, v1, alpha[stp], &b2)) goto done;
     beta[stp] = sqrt(b2);
     if (stp % every != 0 && stp != maxit && beta[stp] >= 1e-14) continue;
+    if (p->mode == 1) {
+      double cur[16];
+      int nc = hphi_lanczos_good_eigenvalues(alpha + 1, beta + 1, stp, nwant, cur);
+      r->iterations = stp;
+      r->n_found = nc;
+      for (int i = 0; i < nc; i++) r->eigenvalues[i] = cur[i];
+      if (hphi_lanczos_converged(prevv, nprev, cur, nc, nwant, eps) || fabs(beta[stp]) < 1e-14 ||
+          (uint64_t)stp >= dim) {
+        r->converged = (nc >= nwant || (uint64_t)stp >= dim || fabs(beta[stp]) < 1e-14);
+        break;
+      }
+      memcpy(prevv, cur, sizeof cur);
+      nprev = nc;
+      continue;
+    }
     int want = (target + 1 > nwant) ? target + 1 : nwant;
     if (want > stp) want = stp;
     hphi_tridiag_lowest(alpha + 1, beta + 1, stp, want, ev);

@@ -1,0 +1,546 @@
+// Continuous-batching Lanczos service: many small diagonalisation jobs share
+// every kernel launch.
+//
+// Each job is a canonical Hamiltonian (Spin or Hubbard) whose lowest n
+// eigenvalues are wanted.  Jobs arrive at any time; a tick
+//   1. admits waiting jobs in arrival order while their HBM footprint
+//      (tables + diagonal + two Lanczos vectors) fits the arena and a slot is
+//      free, and builds their tables, diagonal and start vector on the device;
+//   2. advances every running job by one step of the reference's plain
+//      Lanczos recurrence (src/Lanczos_EigenValue.c:153-177) with five packed
+//      launches for all jobs together (update, H·v, alpha, beta, beta-final);
+//   3. copies the new (alpha, beta) of all jobs to the host in one transfer,
+//      checks convergence of the lowest n distinct Ritz values
+//      (Cullum-Willoughby filtered) and retires converged jobs, freeing their
+//      arena memory for the next admissions.
+// Tiles map to jobs through a prefix table; a job's reductions run over its
+// own tiles in a fixed order, so its result is bit-identical whether it runs
+// alone or among other jobs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <time.h>
+#include <vector>
+
+#include "engine_internal.h"
+#include "hphi_amd.h"
+#include "hv_device.h"
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return hphi_set_error("%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+#define SV_TPB HV_TPB  // one tile = HV_TPB states
+
+struct JobDev {
+  HvArgs hv;
+  int slot, first;
+};
+typedef const JobDev __attribute__((address_space(4))) *CJob;
+typedef const uint64_t __attribute__((address_space(4))) *CU64;
+
+// job state per slot (device): [0] alpha, [1] beta, [2] start-vector scale
+#define JS_STRIDE 4
+
+__device__ __forceinline__ int find_job(CU64 prefix, int njobs, uint64_t tile) {
+  int lo = 0, hi = njobs;  // prefix[lo] <= tile < prefix[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= tile) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ HvArgs load_args(CJob J) {
+  HvArgs A;
+  A.dim = J->hv.dim;
+  A.list1 = J->hv.list1;
+  A.l21 = J->hv.l21;
+  A.l22 = J->hv.l22;
+  A.diag = J->hv.diag;
+  A.terms = J->hv.terms;
+  A.n_low = J->hv.n_low;
+  A.n_high = J->hv.n_high;
+  A.n_mixed = J->hv.n_mixed;
+  A.nlow = J->hv.nlow;
+  A.irght = J->hv.irght;
+  A.v0 = J->hv.v0;
+  A.v1 = J->hv.v1;
+  A.partial = nullptr;
+  A.xcd_map = 0;
+  A.size21 = J->hv.size21;
+  A.overwrite = 0;
+  return A;
+}
+
+// v1 <- (v0 - alpha v1)/beta, v0 <- -beta v1_old  (Lanczos_EigenValue.c:154-160)
+__global__ __launch_bounds__(SV_TPB) void k_mj_update(const JobDev *jobs, const uint64_t *prefix, int njobs,
+                                                      const double *js) {
+  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
+  CJob J = (CJob)jobs + jb;
+  if (J->first) return;
+  const uint64_t j = (blockIdx.x - ((CU64)prefix)[jb]) * SV_TPB + threadIdx.x;
+  if (j >= J->hv.dim) return;
+  const double alpha = js[J->slot * JS_STRIDE + 0], beta = js[J->slot * JS_STRIDE + 1];
+  double2 *v0 = (double2 *)J->hv.v0, *v1 = (double2 *)J->hv.v1;
+  const double2 a = v0[j], b = v1[j];
+  v1[j] = make_double2((a.x - alpha * b.x) / beta, (a.y - alpha * b.y) / beta);
+  v0[j] = make_double2(-beta * b.x, -beta * b.y);
+}
+
+// v0 += H v1 for one tile of one job; prdct partial per tile
+__global__ __launch_bounds__(SV_TPB) void k_mj_hv(const JobDev *jobs, const uint64_t *prefix, int njobs,
+                                                  double2 *partial) {
+  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
+  const HvArgs A = load_args((CJob)jobs + jb);
+  double pre = 0.0, pim = 0.0;
+  hv_tile<1, true>(A, blockIdx.x - ((CU64)prefix)[jb], pre, pim);
+  wg_reduce_store<1>(pre, pim, partial);
+}
+
+// |v0 - alpha v1|^2 per tile
+__global__ __launch_bounds__(SV_TPB) void k_mj_beta(const JobDev *jobs, const uint64_t *prefix, int njobs,
+                                                    const double *js, double *partial) {
+  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
+  CJob J = (CJob)jobs + jb;
+  const uint64_t j = (blockIdx.x - ((CU64)prefix)[jb]) * SV_TPB + threadIdx.x;
+  double s = 0.0;
+  if (j < J->hv.dim) {
+    const double alpha = js[J->slot * JS_STRIDE + 0];
+    const double2 a = ((const double2 *)J->hv.v0)[j], b = ((const double2 *)J->hv.v1)[j];
+    const double r = a.x - alpha * b.x, m = a.y - alpha * b.y;
+    s = r * r + m * m;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double red[SV_TPB / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < SV_TPB / 64; w++) t += red[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// per job (one workgroup each): fixed-order sum over the job's tiles
+__device__ double job_sum(const double *part, int stride, int comp, uint64_t t0, uint64_t t1) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (uint64_t t = t0 + threadIdx.x; t < t1; t += 256) a += part[t * stride + comp];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  double r = s[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_mj_alpha(const JobDev *jobs, const uint64_t *prefix, const double2 *partial,
+                                                  double *js) {
+  CJob J = (CJob)jobs + blockIdx.x;
+  double a = job_sum((const double *)partial, 2, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
+  if (threadIdx.x == 0) js[J->slot * JS_STRIDE + 0] = a;
+}
+
+__global__ __launch_bounds__(256) void k_mj_betafin(const JobDev *jobs, const uint64_t *prefix, const double *partial,
+                                                    double *js, double *out) {
+  CJob J = (CJob)jobs + blockIdx.x;
+  double b2 = job_sum(partial, 1, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
+  if (threadIdx.x == 0) {
+    const double b = sqrt(b2);
+    js[J->slot * JS_STRIDE + 1] = b;
+    out[2 * blockIdx.x] = js[J->slot * JS_STRIDE + 0];
+    out[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// start vector of one job: v1 = SplitMix64 uniform complex, normalised; v0 = 0
+__global__ void k_sv_fill(double2 *v1, double2 *v0, uint64_t dim, uint64_t seed) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  uint64_t z = (seed * 0x9E3779B97F4A7C15ull) ^ (2 * i);
+  auto mix = [](uint64_t x) {
+    uint64_t y = x + 0x9E3779B97F4A7C15ull;
+    y = (y ^ (y >> 30)) * 0xBF58476D1CE4E5B9ull;
+    y = (y ^ (y >> 27)) * 0x94D049BB133111EBull;
+    return y ^ (y >> 31);
+  };
+  const double u1 = (double)(mix(z) >> 11) * (1.0 / 9007199254740992.0);
+  const double u2 = (double)(mix((seed * 0x9E3779B97F4A7C15ull) ^ (2 * i + 1)) >> 11) * (1.0 / 9007199254740992.0);
+  v1[i] = make_double2(2.0 * u1 - 1.0, 2.0 * u2 - 1.0);
+  v0[i] = make_double2(0.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_sv_norm(const double2 *v, uint64_t dim, double *js, int slot) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (uint64_t i = threadIdx.x; i < dim; i += 256) a += v[i].x * v[i].x + v[i].y * v[i].y;
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) js[slot * JS_STRIDE + 2] = 1.0 / sqrt(s[0]);
+}
+
+__global__ void k_sv_scale(double2 *v, uint64_t dim, const double *js, int slot) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  const double f = js[slot * JS_STRIDE + 2];
+  v[i] = make_double2(v[i].x * f, v[i].y * f);
+}
+
+// ==================================================================== host
+namespace {
+
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> fre;  // offset -> length
+  void init(char *b, size_t n) {
+    base = b;
+    size = n;
+    fre.clear();
+    fre[0] = n;
+  }
+  bool alloc(size_t n, size_t *off) {
+    n = (n + 255) & ~(size_t)255;
+    for (auto it = fre.begin(); it != fre.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        size_t rest = it->second - n, o = it->first;
+        fre.erase(it);
+        if (rest) fre[o + n] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    n = (n + 255) & ~(size_t)255;
+    auto it = fre.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != fre.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      fre.erase(nx);
+    }
+    if (it != fre.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        fre.erase(it);
+      }
+    }
+  }
+};
+
+struct Job {
+  int id = 0;
+  uint64_t seed = 0;
+  double t_arrival = 0, t_admit = 0, t_done = 0;
+  hphi_engine h;  // host-side preparation only (no device members used)
+  size_t off = 0, bytes = 0;
+  int slot = -1;
+  JobDev dev{};
+  std::vector<double> alpha, beta;
+  int step = 0, nprev = 0;
+  double prev[16];
+  hphi_job_result res{};
+};
+
+size_t job_footprint(const hphi_engine &h) {
+  auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  return al(8 * h.dim) + al(4 * h.size21) + al(8 * h.size22) + al(8 * (h.sdim + 1)) + al(8 * h.dim) +
+         2 * al(16 * h.dim) + al(sizeof(GTerm) * std::max<size_t>(h.h_terms_dev.size(), 1)) +
+         al(sizeof(DTerm) * std::max<size_t>(h.h_dterms.size(), 1));
+}
+
+double mono_now() {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+}  // namespace
+
+struct hphi_server {
+  hphi_server_params p{};
+  hipStream_t stream = nullptr;
+  Arena arena;
+  char *d_arena = nullptr;
+  double *d_js = nullptr, *d_out = nullptr, *h_out = nullptr, *d_part = nullptr;
+  JobDev *d_jobs = nullptr;
+  uint64_t *d_prefix = nullptr;
+  char *h_stage = nullptr;  // pinned staging for term uploads
+  size_t stage_size = 0, stage_used = 0;
+  std::deque<Job *> waiting;
+  std::vector<Job *> active;
+  std::vector<int> free_slots;
+  std::vector<hphi_job_result> done;
+  int next_id = 0;
+  double t0 = 0;
+  size_t max_tiles = 0;
+};
+
+extern "C" double hphi_server_now(const hphi_server *s) { return s ? mono_now() - s->t0 : 0.0; }
+
+extern "C" int hphi_server_create(const hphi_server_params *p, hphi_server **out) {
+  if (!p || !out) return hphi_set_error("hphi_server_create: NULL argument");
+  *out = nullptr;
+  if (hphi_ensure_binom_table()) return -1;
+  hphi_server *s = new hphi_server();
+  s->p = *p;
+  if (s->p.max_jobs <= 0) s->p.max_jobs = 64;
+  if (s->p.max_jobs > 4096) s->p.max_jobs = 4096;
+  if (s->p.n_eigen <= 0) s->p.n_eigen = 4;
+  if (s->p.n_eigen > 16) s->p.n_eigen = 16;
+  if (s->p.eps <= 0) s->p.eps = 1e-12;
+  if (s->p.max_iter <= 0) s->p.max_iter = 2000;
+  if (s->p.check_every <= 0) s->p.check_every = 5;
+  size_t budget = s->p.hbm_budget ? s->p.hbm_budget : ((size_t)16 << 30);
+  s->max_tiles = (size_t)1 << 22;  // 4M tiles x 256 states = 1.07e9 states in flight
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&s->d_arena, budget) != hipSuccess ||
+      hipMalloc(&s->d_js, sizeof(double) * JS_STRIDE * s->p.max_jobs) != hipSuccess ||
+      hipMalloc(&s->d_out, sizeof(double) * 2 * s->p.max_jobs) != hipSuccess ||
+      hipMalloc(&s->d_jobs, sizeof(JobDev) * s->p.max_jobs) != hipSuccess ||
+      hipMalloc(&s->d_prefix, sizeof(uint64_t) * (s->p.max_jobs + 1)) != hipSuccess ||
+      hipMalloc(&s->d_part, sizeof(double2) * s->max_tiles) != hipSuccess ||
+      hipHostMalloc(&s->h_out, sizeof(double) * 2 * s->p.max_jobs, hipHostMallocDefault) != hipSuccess)
+    rc = hphi_set_error("hphi_server_create: device allocation failed");
+  s->stage_size = (size_t)8 << 20;
+  if (!rc && hipHostMalloc(&s->h_stage, s->stage_size, hipHostMallocDefault) != hipSuccess)
+    rc = hphi_set_error("hphi_server_create: pinned staging allocation failed");
+  if (rc) {
+    hphi_server_destroy(s);
+    return -1;
+  }
+  s->arena.init(s->d_arena, budget);
+  for (int i = s->p.max_jobs - 1; i >= 0; i--) s->free_slots.push_back(i);
+  s->t0 = mono_now();
+  *out = s;
+  return 0;
+}
+
+extern "C" void hphi_server_destroy(hphi_server *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (Job *j : s->active) delete j;
+  for (Job *j : s->waiting) delete j;
+  (void)hipFree(s->d_arena);
+  (void)hipFree(s->d_js);
+  (void)hipFree(s->d_out);
+  (void)hipFree(s->d_jobs);
+  (void)hipFree(s->d_prefix);
+  (void)hipFree(s->d_part);
+  if (s->h_out) (void)hipHostFree(s->h_out);
+  if (s->h_stage) (void)hipHostFree(s->h_stage);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+extern "C" int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint64_t seed, double t_arrival,
+                                  int *id) {
+  if (!s || !d) return hphi_set_error("hphi_server_submit: NULL argument");
+  if (d->model != HPHI_MODEL_SPIN && d->model != HPHI_MODEL_HUBBARD)
+    return hphi_set_error("the service runs canonical Spin and Hubbard models");
+  Job *j = new Job();
+  if (hphi_engine_prepare(d, &j->h)) {
+    delete j;
+    return -1;
+  }
+  j->bytes = job_footprint(j->h);
+  if (j->bytes > s->arena.size) {
+    delete j;
+    return hphi_set_error("job footprint %zu exceeds the HBM budget", j->bytes);
+  }
+  j->id = s->next_id++;
+  j->seed = seed;
+  j->t_arrival = t_arrival;
+  s->waiting.push_back(j);
+  if (id) *id = j->id;
+  return 0;
+}
+
+static int admit(hphi_server *s, Job *j) {
+  hphi_engine &h = j->h;
+  char *b = s->d_arena + j->off;
+  auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  uint64_t *list1 = (uint64_t *)b;
+  b += al(8 * h.dim);
+  uint32_t *l21 = (uint32_t *)b;
+  b += al(4 * h.size21);
+  uint64_t *l22 = (uint64_t *)b;
+  b += al(8 * h.size22);
+  uint64_t *jb = (uint64_t *)b;
+  b += al(8 * (h.sdim + 1));
+  double *diag = (double *)b;
+  b += al(8 * h.dim);
+  double *v0 = (double *)b;
+  b += al(16 * h.dim);
+  double *v1 = (double *)b;
+  b += al(16 * h.dim);
+  GTerm *terms = (GTerm *)b;
+  b += al(sizeof(GTerm) * std::max<size_t>(h.h_terms_dev.size(), 1));
+  DTerm *dterms = (DTerm *)b;
+  // term tables through pinned staging (asynchronous copies)
+  size_t tb = sizeof(GTerm) * h.h_terms_dev.size(), db = sizeof(DTerm) * h.h_dterms.size();
+  if (s->stage_used + tb + db > s->stage_size) return 1;  // staging full: admit next tick
+  char *st = s->h_stage + s->stage_used;
+  if (tb) memcpy(st, h.h_terms_dev.data(), tb);
+  if (db) memcpy(st + tb, h.h_dterms.data(), db);
+  s->stage_used += (tb + db + 255) & ~(size_t)255;
+  if (tb) HIPCHK(hipMemcpyAsync(terms, st, tb, hipMemcpyHostToDevice, s->stream));
+  if (db) HIPCHK(hipMemcpyAsync(dterms, st + tb, db, hipMemcpyHostToDevice, s->stream));
+  SzParams P = hphi_sz_params(&h);
+  if (hphi_launch_tables(P, h.sdim, h.size21, jb, l21, l22, list1, s->stream)) return -1;
+  if (hphi_launch_diag(h.dim, list1, 0, dterms, (int)h.h_dterms.size(), diag, s->stream)) return -1;
+  unsigned g = (unsigned)((h.dim + 255) / 256);
+  hipLaunchKernelGGL(k_sv_fill, dim3(g), dim3(256), 0, s->stream, (double2 *)v1, (double2 *)v0, h.dim, j->seed);
+  hipLaunchKernelGGL(k_sv_norm, dim3(1), dim3(256), 0, s->stream, (const double2 *)v1, h.dim, s->d_js, j->slot);
+  hipLaunchKernelGGL(k_sv_scale, dim3(g), dim3(256), 0, s->stream, (double2 *)v1, h.dim, (const double *)s->d_js,
+                     j->slot);
+  HIPCHK(hipGetLastError());
+  HvArgs &A = j->dev.hv;
+  memset(&A, 0, sizeof A);
+  A.dim = h.dim;
+  A.list1 = list1;
+  A.l21 = l21;
+  A.l22 = l22;
+  A.diag = diag;
+  A.terms = terms;
+  A.n_low = h.n_low;
+  A.n_high = h.n_high;
+  A.n_mixed = h.n_mixed;
+  A.nlow = h.nlow;
+  A.irght = h.irght;
+  A.v0 = v0;
+  A.v1 = v1;
+  A.size21 = h.size21;
+  j->dev.slot = j->slot;
+  j->dev.first = 1;
+  j->step = 0;
+  j->alpha.assign(1, 0.0);
+  j->beta.assign(1, 0.0);
+  return 0;
+}
+
+static void retire(hphi_server *s, Job *j, double now, int converged, const double *ev, int n) {
+  j->res.id = j->id;
+  j->res.iterations = j->step;
+  j->res.converged = converged;
+  j->res.n_found = n;
+  for (int i = 0; i < n && i < 16; i++) j->res.eigenvalues[i] = ev[i];
+  j->res.t_arrival = j->t_arrival;
+  j->res.t_admit = j->t_admit;
+  j->res.t_done = now;
+  j->res.dim = j->h.dim;
+  s->done.push_back(j->res);
+  s->arena.release(j->off, j->bytes);
+  s->free_slots.push_back(j->slot);
+}
+
+extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
+  if (!s) return hphi_set_error("hphi_server_tick: NULL argument");
+  double now = hphi_server_now(s);
+  s->stage_used = 0;
+  // 1. admission in arrival order
+  while (!s->waiting.empty() && !s->free_slots.empty()) {
+    Job *j = s->waiting.front();
+    if (j->t_arrival > now) break;
+    size_t off;
+    if (!s->arena.alloc(j->bytes, &off)) break;
+    j->off = off;
+    j->slot = s->free_slots.back();
+    int rc = admit(s, j);
+    if (rc == 1) {
+      s->arena.release(off, j->bytes);
+      break;
+    }
+    if (rc) return -1;
+    s->free_slots.pop_back();
+    s->waiting.pop_front();
+    j->t_admit = now;
+    s->active.push_back(j);
+  }
+  if (!s->active.empty()) {
+    // 2. one Lanczos step for every running job, packed launches
+    const int n = (int)s->active.size();
+    std::vector<JobDev> jd(n);
+    std::vector<uint64_t> prefix(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+      jd[i] = s->active[i]->dev;
+      prefix[i + 1] = prefix[i] + (s->active[i]->h.dim + SV_TPB - 1) / SV_TPB;
+    }
+    const uint64_t T = prefix[n];
+    if (T > s->max_tiles) return hphi_set_error("too many tiles in flight");
+    HIPCHK(hipMemcpyAsync(s->d_jobs, jd.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_prefix, prefix.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_mj_update, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
+                       (const double *)s->d_js);
+    hipLaunchKernelGGL(k_mj_hv, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
+                       (double2 *)s->d_part);
+    hipLaunchKernelGGL(k_mj_alpha, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
+                       (const double2 *)s->d_part, s->d_js);
+    hipLaunchKernelGGL(k_mj_beta, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
+                       (const double *)s->d_js, s->d_part);
+    hipLaunchKernelGGL(k_mj_betafin, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
+                       (const double *)s->d_part, s->d_js, s->d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    now = hphi_server_now(s);
+    // 3. convergence and retirement
+    std::vector<Job *> keep;
+    keep.reserve(n);
+    for (int i = 0; i < n; i++) {
+      Job *j = s->active[i];
+      j->step++;
+      j->alpha.push_back(s->h_out[2 * i]);
+      j->beta.push_back(s->h_out[2 * i + 1]);
+      j->dev.first = 0;
+      const int m = j->step;
+      const bool exhausted = (uint64_t)m >= j->h.dim || j->beta[m] < 1e-14;
+      if (m % s->p.check_every == 0 || exhausted || m >= s->p.max_iter) {
+        double cur[16];
+        int nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m, s->p.n_eigen, cur);
+        int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
+        bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) ||
+                    (exhausted && nc >= 1);
+        if (conv || m >= s->p.max_iter) {
+          retire(s, j, now, conv ? 1 : 0, cur, nc);
+          delete j;
+          continue;
+        }
+        memcpy(j->prev, cur, sizeof cur);
+        j->nprev = nc;
+      }
+      keep.push_back(j);
+    }
+    s->active.swap(keep);
+  }
+  if (n_active) *n_active = (int)s->active.size();
+  if (n_waiting) *n_waiting = (int)s->waiting.size();
+  return 0;
+}
+
+extern "C" int hphi_server_results(hphi_server *s, hphi_job_result *out, int max, int *n) {
+  if (!s || !n) return hphi_set_error("hphi_server_results: NULL argument");
+  int k = (int)std::min<size_t>(s->done.size(), (size_t)std::max(max, 0));
+  for (int i = 0; i < k; i++) out[i] = s->done[i];
+  s->done.erase(s->done.begin(), s->done.begin() + k);
+  *n = k;
+  return 0;
+}

@@ -158,6 +158,8 @@ typedef struct {
   int target;        /* LanczosTarget (0-based index of the converged value tested) */
   uint64_t seed;     /* initial vector seed */
   int check_every;   /* tridiagonal solve interval (1 = every step, as the reference) */
+  int mode;          /* 0: the reference's test on E[target]; 1: the lowest n_eigen distinct
+                        eigenvalues (Cullum-Willoughby filtered) change by < eps relative */
 } hphi_lanczos_params;
 
 typedef struct {
@@ -167,6 +169,11 @@ typedef struct {
 } hphi_lanczos_result;
 
 int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r);
+/* Lowest `want` distinct eigenvalues of the plain-Lanczos tridiagonal
+ * (alpha[0..m), beta[0..m-1)) after Cullum-Willoughby removal of spurious
+ * values and merging of ghost copies; returns how many were found. */
+int hphi_lanczos_good_eigenvalues(const double *alpha, const double *beta, int m, int want, double *out);
+int hphi_lanczos_converged(const double *prev, int nprev, const double *cur, int ncur, int want, double tol);
 /* Microcanonical TPQ (CalcByTPQ, src/CalcByTPQ.c:52-294) for k samples at once.
  * Sample c starts from a random vector of seed + c (init_kind 0: uniform
  * complex, -1: complex Gaussian); l = large_value, or 3 max|list_Diagonal|/Nsite
@@ -189,6 +196,35 @@ typedef struct {
 int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out, hphi_tpq_result *r);
 /* max_j |list_Diagonal[j]| */
 int hphi_diag_maxabs(hphi_engine *e, double *out);
+
+/* ---- continuous-batching service ----------------------------------------
+ * No counterpart in the reference (which solves one model per process).
+ * Jobs (canonical Spin / Hubbard models) are submitted with an arrival time
+ * on the server clock (hphi_server_now); each hphi_server_tick admits arrived
+ * jobs in order while their HBM footprint fits the budget and a slot is
+ * free, advances every running job by one plain-Lanczos step in packed
+ * launches, and retires jobs whose lowest n_eigen distinct eigenvalues
+ * (Cullum-Willoughby filtered, checked every check_every steps) changed by
+ * less than eps relative.  A job's numbers do not depend on the other jobs. */
+typedef struct hphi_server hphi_server;
+typedef struct {
+  uint64_t hbm_budget; /* arena bytes (0: 16 GiB) */
+  int max_jobs, n_eigen;
+  double eps;
+  int max_iter, check_every;
+} hphi_server_params;
+typedef struct {
+  int id, n_found, iterations, converged;
+  uint64_t dim;
+  double eigenvalues[16];
+  double t_arrival, t_admit, t_done;
+} hphi_job_result;
+int hphi_server_create(const hphi_server_params *p, hphi_server **out);
+void hphi_server_destroy(hphi_server *s);
+int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint64_t seed, double t_arrival, int *id);
+int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting);
+int hphi_server_results(hphi_server *s, hphi_job_result *out, int max, int *n);
+double hphi_server_now(const hphi_server *s);
 
 /* k lowest eigenvalues (ascending) of the symmetric tridiagonal matrix with
  * diagonal a[0..n) and off-diagonal b[0..n-1), by Sturm bisection (host only;

@@ -1,0 +1,87 @@
+"""GPU tests of the continuous-batching service and the distinct-eigenvalue
+Lanczos mode: eigenvalues against an independent solver, a job's result
+identical alone and in a mix, admission by HBM footprint."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from hphi_amd import lattice as L
+
+pytestmark = pytest.mark.gpu
+
+EV_TOL = 1e-10
+
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "serve_eigen.json")))
+
+
+def mixed_requests():
+    """(fixture name, spec) pairs; reference eigenvalues in golden/serve_eigen.json."""
+    from tests.golden.gen_serve_eigen import serve_models
+    return list(serve_models().items())
+
+
+@pytest.fixture(scope="module")
+def A():
+    from hphi_amd import capi
+    capi.lib()
+    return capi
+
+
+def run_server(A, specs, seeds, budget=4 << 30, max_jobs=64):
+    srv = A.Server(hbm_budget=budget, max_jobs=max_jobs, n_eigen=4, eps=1e-12, check_every=5)
+    ids = [srv.submit(s, seed, 0.0) for s, seed in zip(specs, seeds)]
+    res = {}
+    for _ in range(100000):
+        a, w = srv.tick()
+        for r in srv.results():
+            res[r.id] = r
+        if a == 0 and w == 0:
+            break
+    srv.close()
+    return [res[i] for i in ids]
+
+
+def test_server_eigenvalues_match_independent_solver(A):
+    named = mixed_requests()
+    specs = [s for _, s in named]
+    print("[gpu] serve mixed", flush=True)
+    out = run_server(A, specs, list(range(len(specs))))
+    for (name, spec), r in zip(named, out):
+        ref = np.array(GOLD[name][:4])
+        got = np.array(r.eigenvalues[: r.n_found])
+        assert r.converged and r.n_found >= min(4, len(ref)), spec.name
+        assert np.max(np.abs(got[: len(ref)] - ref)) <= EV_TOL, (spec.name, got, ref)
+
+
+def test_server_result_same_alone_and_in_mix(A):
+    specs = [s for _, s in mixed_requests()]
+    target = L.j1j2_square(4, 3, 1.0, 0.55)
+    alone = run_server(A, [target], [77])[0]
+    mix = run_server(A, specs[:3] + [target] + specs[3:], [1, 2, 3, 77, 4, 5, 6, 7, 8])[3]
+    assert alone.iterations == mix.iterations
+    assert list(alone.eigenvalues) == list(mix.eigenvalues)
+
+
+def test_server_admits_by_hbm_footprint(A):
+    specs = [L.heisenberg_chain(16, 1.0, 0.2 * i) for i in range(6)]  # dim 12870 each
+    # budget fits about two jobs at a time: all must still complete
+    out = run_server(A, specs, list(range(6)), budget=1 << 20, max_jobs=8)
+    assert all(r.converged for r in out)
+    assert max(r.t_admit for r in out) > 0.0
+
+
+def test_lanczos_distinct_mode_matches_independent_solver(A):
+    named = dict(mixed_requests())
+    for name in ("chain12_j2_0.3", "hubbard_chain8_u3.3"):
+        spec = named[name]
+        eng = A.Engine.from_spec(spec)
+        r = eng.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=5)
+        ref = np.array(GOLD[name][:4])
+        got = np.array(r.eigenvalues[: r.n_found])
+        assert r.converged
+        assert np.max(np.abs(got[: len(ref)] - ref)) <= EV_TOL, (spec.name, got, ref)
+        eng.close()
