@@ -32,7 +32,7 @@ EXPORTED = [
     "hphi_read_def", "hphi_model_desc_free", "hphi_tridiag_lowest",
     "hphi_lanczos_good_eigenvalues", "hphi_lanczos_converged",
     "hphi_server_create", "hphi_server_destroy", "hphi_server_submit", "hphi_server_tick",
-    "hphi_server_results", "hphi_server_now",
+    "hphi_server_results", "hphi_server_now", "hphi_server_stats",
 ]
 
 
@@ -137,6 +137,7 @@ def lib():
         L.hphi_server_results.argtypes = [_vp, C.POINTER(JobResult), C.c_int, C.POINTER(C.c_int)]
         L.hphi_server_now.argtypes = [_vp]
         L.hphi_server_now.restype = C.c_double
+        L.hphi_server_stats.argtypes = [_vp, _f64p]
         _lib = L
     return _lib
 
@@ -380,6 +381,12 @@ class Server:
         a, w = C.c_int(), C.c_int()
         check(lib().hphi_server_tick(self.h, C.byref(a), C.byref(w)))
         return a.value, w.value
+
+    def stats(self):
+        out = np.zeros(5)
+        check(lib().hphi_server_stats(self.h, _p(out, _f64p)))
+        return {"admit_s": out[0], "gpu_s": out[1], "check_s": out[2], "ticks": int(out[3]),
+                "job_steps": int(out[4])}
 
     def results(self, max_n: int = 4096):
         buf = (JobResult * max_n)()

@@ -16,7 +16,7 @@
 
 #include "hphi_amd.h"
 
-#define CW_EXTRA 12
+#define CW_EXTRA 6
 
 /* lowest `want` good distinct eigenvalues of T(alpha[0..m), beta[0..m-1)); returns count */
 int hphi_lanczos_good_eigenvalues(const double *alpha, const double *beta, int m, int want, double *out) {

@@ -41,6 +41,8 @@
 struct JobDev {
   HvArgs hv;
   int slot, first;
+  int step, check;       // Lanczos step computed this tick (1-based); run the eigenvalue check
+  double *ahist, *bhist;  // alpha[1..max_iter], beta[1..max_iter] on the device
 };
 typedef const JobDev __attribute__((address_space(4))) *CJob;
 typedef const uint64_t __attribute__((address_space(4))) *CU64;
@@ -149,7 +151,10 @@ __global__ __launch_bounds__(256) void k_mj_alpha(const JobDev *jobs, const uint
                                                   double *js) {
   CJob J = (CJob)jobs + blockIdx.x;
   double a = job_sum((const double *)partial, 2, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
-  if (threadIdx.x == 0) js[J->slot * JS_STRIDE + 0] = a;
+  if (threadIdx.x == 0) {
+    js[J->slot * JS_STRIDE + 0] = a;
+    J->ahist[J->step] = a;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_mj_betafin(const JobDev *jobs, const uint64_t *prefix, const double *partial,
@@ -159,8 +164,93 @@ __global__ __launch_bounds__(256) void k_mj_betafin(const JobDev *jobs, const ui
   if (threadIdx.x == 0) {
     const double b = sqrt(b2);
     js[J->slot * JS_STRIDE + 1] = b;
+    J->bhist[J->step] = b;
     out[2 * blockIdx.x] = js[J->slot * JS_STRIDE + 0];
     out[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// Lowest good eigenvalues of each checking job's tridiagonal T_m (alpha[1..m],
+// beta[1..m-1]): thread i < L bisects eigenvalue i of T_m, thread 32+i
+// eigenvalue i of T_m without its first row and column; thread 0 then applies
+// the Cullum-Willoughby test and merges ghost copies (the rules of
+// hphi_lanczos_good_eigenvalues).  out: [16 values, count] per job.
+#define CK_EXTRA 6
+__device__ int ck_sturm(const double *a, const double *b, int n, double x) {
+  int c = 0;
+  double q = a[0] - x;
+  if (q < 0) c++;
+  for (int i = 1; i < n; i++) {
+    if (fabs(q) < 1e-300) q = -1e-300;
+    q = (a[i] - x) - b[i - 1] * b[i - 1] / q;
+    if (q < 0) c++;
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(64) void k_mj_check(const JobDev *jobs, int want, double *out) {
+  CJob J = (CJob)jobs + blockIdx.x;
+  if (!J->check) return;
+  const int m = J->step;
+  const double *a = J->ahist + 1, *b = J->bhist + 1;
+  __shared__ double ev[32], evh[32];
+  __shared__ double lo_s, hi_s, scale_s;
+  int L = want + CK_EXTRA;
+  if (L > m) L = m;
+  if (L > 32) L = 32;
+  int Lh = L < m - 1 ? L : m - 1;
+  if (threadIdx.x == 0) {
+    double lo = a[0], hi = a[0], sc = 0.0;
+    for (int i = 0; i < m; i++) {
+      double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
+      lo = fmin(lo, a[i] - r);
+      hi = fmax(hi, a[i] + r);
+      sc = fmax(sc, fabs(a[i]) + r);
+    }
+    double span = hi - lo;
+    lo_s = lo - 1e-12 * span - 1e-300;
+    hi_s = hi + 1e-12 * span + 1e-300;
+    scale_s = sc;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  const bool main = t < L, hat = (t >= 32) && (t - 32 < Lh);
+  if (main || hat) {
+    const int idx = main ? t : t - 32;
+    const double *aa = main ? a : a + 1, *bb = main ? b : b + 1;
+    const int n = main ? m : m - 1;
+    double l = lo_s, h = hi_s;
+    for (int it = 0; it < 200; it++) {
+      double mid = 0.5 * (l + h);
+      if (mid <= l || mid >= h) break;
+      if (ck_sturm(aa, bb, n, mid) > idx) h = mid;
+      else l = mid;
+    }
+    if (main) ev[idx] = 0.5 * (l + h);
+    else evh[idx] = 0.5 * (l + h);
+  }
+  __syncthreads();
+  if (t == 0) {
+    double *o = out + (uint64_t)blockIdx.x * 17;
+    const double tol = 1e-11 * (scale_s > 1.0 ? scale_s : 1.0);
+    int got = 0;
+    if (m == 1) {
+      o[0] = a[0];
+      got = 1;
+    }
+    for (int i = 0; i < L && got < want && m > 1; i++) {
+      int mult = 0;
+      for (int j = 0; j < L; j++)
+        if (fabs(ev[j] - ev[i]) <= tol) mult++;
+      bool spurious = false;
+      if (mult == 1 && i < L - 1)
+        for (int j = 0; j < Lh; j++)
+          if (fabs(evh[j] - ev[i]) <= tol) spurious = true;
+      if (spurious) continue;
+      if (got > 0 && fabs(ev[i] - o[got - 1]) <= tol) continue;
+      o[got++] = ev[i];
+    }
+    o[16] = (double)got;
   }
 }
 
@@ -259,11 +349,11 @@ struct Job {
   hphi_job_result res{};
 };
 
-size_t job_footprint(const hphi_engine &h) {
+size_t job_footprint(const hphi_engine &h, int max_iter) {
   auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
   return al(8 * h.dim) + al(4 * h.size21) + al(8 * h.size22) + al(8 * (h.sdim + 1)) + al(8 * h.dim) +
          2 * al(16 * h.dim) + al(sizeof(GTerm) * std::max<size_t>(h.h_terms_dev.size(), 1)) +
-         al(sizeof(DTerm) * std::max<size_t>(h.h_dterms.size(), 1));
+         al(sizeof(DTerm) * std::max<size_t>(h.h_dterms.size(), 1)) + 2 * al(8 * (size_t)(max_iter + 2));
 }
 
 double mono_now() {
@@ -280,6 +370,7 @@ struct hphi_server {
   Arena arena;
   char *d_arena = nullptr;
   double *d_js = nullptr, *d_out = nullptr, *h_out = nullptr, *d_part = nullptr;
+  double *d_ev = nullptr, *h_ev = nullptr;  // per-job check results [17]
   JobDev *d_jobs = nullptr;
   uint64_t *d_prefix = nullptr;
   char *h_stage = nullptr;  // pinned staging for term uploads
@@ -291,6 +382,8 @@ struct hphi_server {
   int next_id = 0;
   double t0 = 0;
   size_t max_tiles = 0;
+  double st_admit = 0, st_gpu = 0, st_check = 0;  // seconds spent per phase
+  uint64_t st_ticks = 0, st_job_steps = 0;
 };
 
 extern "C" double hphi_server_now(const hphi_server *s) { return s ? mono_now() - s->t0 : 0.0; }
@@ -318,7 +411,9 @@ extern "C" int hphi_server_create(const hphi_server_params *p, hphi_server **out
       hipMalloc(&s->d_jobs, sizeof(JobDev) * s->p.max_jobs) != hipSuccess ||
       hipMalloc(&s->d_prefix, sizeof(uint64_t) * (s->p.max_jobs + 1)) != hipSuccess ||
       hipMalloc(&s->d_part, sizeof(double2) * s->max_tiles) != hipSuccess ||
-      hipHostMalloc(&s->h_out, sizeof(double) * 2 * s->p.max_jobs, hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc(&s->h_out, sizeof(double) * 2 * s->p.max_jobs, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&s->d_ev, sizeof(double) * 17 * s->p.max_jobs) != hipSuccess ||
+      hipHostMalloc(&s->h_ev, sizeof(double) * 17 * s->p.max_jobs, hipHostMallocDefault) != hipSuccess)
     rc = hphi_set_error("hphi_server_create: device allocation failed");
   s->stage_size = (size_t)8 << 20;
   if (!rc && hipHostMalloc(&s->h_stage, s->stage_size, hipHostMallocDefault) != hipSuccess)
@@ -346,6 +441,8 @@ extern "C" void hphi_server_destroy(hphi_server *s) {
   (void)hipFree(s->d_prefix);
   (void)hipFree(s->d_part);
   if (s->h_out) (void)hipHostFree(s->h_out);
+  (void)hipFree(s->d_ev);
+  if (s->h_ev) (void)hipHostFree(s->h_ev);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -361,7 +458,7 @@ extern "C" int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint
     delete j;
     return -1;
   }
-  j->bytes = job_footprint(j->h);
+  j->bytes = job_footprint(j->h, s->p.max_iter);
   if (j->bytes > s->arena.size) {
     delete j;
     return hphi_set_error("job footprint %zu exceeds the HBM budget", j->bytes);
@@ -395,6 +492,10 @@ static int admit(hphi_server *s, Job *j) {
   GTerm *terms = (GTerm *)b;
   b += al(sizeof(GTerm) * std::max<size_t>(h.h_terms_dev.size(), 1));
   DTerm *dterms = (DTerm *)b;
+  b += al(sizeof(DTerm) * std::max<size_t>(h.h_dterms.size(), 1));
+  double *ahist = (double *)b;
+  b += al(8 * (size_t)(s->p.max_iter + 2));
+  double *bhist = (double *)b;
   // term tables through pinned staging (asynchronous copies)
   size_t tb = sizeof(GTerm) * h.h_terms_dev.size(), db = sizeof(DTerm) * h.h_dterms.size();
   if (s->stage_used + tb + db > s->stage_size) return 1;  // staging full: admit next tick
@@ -431,6 +532,8 @@ static int admit(hphi_server *s, Job *j) {
   A.size21 = h.size21;
   j->dev.slot = j->slot;
   j->dev.first = 1;
+  j->dev.ahist = ahist;
+  j->dev.bhist = bhist;
   j->step = 0;
   j->alpha.assign(1, 0.0);
   j->beta.assign(1, 0.0);
@@ -455,6 +558,7 @@ static void retire(hphi_server *s, Job *j, double now, int converged, const doub
 extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
   if (!s) return hphi_set_error("hphi_server_tick: NULL argument");
   double now = hphi_server_now(s);
+  const double ta = now;
   s->stage_used = 0;
   // 1. admission in arrival order
   while (!s->waiting.empty() && !s->free_slots.empty()) {
@@ -475,14 +579,20 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
     j->t_admit = now;
     s->active.push_back(j);
   }
+  s->st_admit += hphi_server_now(s) - ta;
   if (!s->active.empty()) {
+    const double tg = hphi_server_now(s);
     // 2. one Lanczos step for every running job, packed launches
     const int n = (int)s->active.size();
     std::vector<JobDev> jd(n);
     std::vector<uint64_t> prefix(n + 1, 0);
     for (int i = 0; i < n; i++) {
-      jd[i] = s->active[i]->dev;
-      prefix[i + 1] = prefix[i] + (s->active[i]->h.dim + SV_TPB - 1) / SV_TPB;
+      Job *j = s->active[i];
+      const int m = j->step + 1;  // step computed this tick
+      j->dev.step = m;
+      j->dev.check = (m % s->p.check_every == 0) || (uint64_t)m >= j->h.dim || m >= s->p.max_iter;
+      jd[i] = j->dev;
+      prefix[i + 1] = prefix[i] + (j->h.dim + SV_TPB - 1) / SV_TPB;
     }
     const uint64_t T = prefix[n];
     if (T > s->max_tiles) return hphi_set_error("too many tiles in flight");
@@ -498,10 +608,15 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
                        (const double *)s->d_js, s->d_part);
     hipLaunchKernelGGL(k_mj_betafin, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
                        (const double *)s->d_part, s->d_js, s->d_out);
+    hipLaunchKernelGGL(k_mj_check, dim3(n), dim3(64), 0, s->stream, s->d_jobs, s->p.n_eigen, s->d_ev);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_ev, s->d_ev, sizeof(double) * 17 * n, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     now = hphi_server_now(s);
+    s->st_gpu += now - tg;
+    s->st_ticks++;
+    s->st_job_steps += n;
     // 3. convergence and retirement
     std::vector<Job *> keep;
     keep.reserve(n);
@@ -513,9 +628,16 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
       j->dev.first = 0;
       const int m = j->step;
       const bool exhausted = (uint64_t)m >= j->h.dim || j->beta[m] < 1e-14;
-      if (m % s->p.check_every == 0 || exhausted || m >= s->p.max_iter) {
+      if (j->dev.check || exhausted) {
         double cur[16];
-        int nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m, s->p.n_eigen, cur);
+        int nc;
+        if (j->dev.check) {  // computed on the device this tick
+          const double *o = s->h_ev + 17 * (size_t)i;
+          nc = (int)o[16];
+          for (int q = 0; q < nc; q++) cur[q] = o[q];
+        } else {
+          nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m, s->p.n_eigen, cur);
+        }
         int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
         bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) ||
                     (exhausted && nc >= 1);
@@ -530,9 +652,20 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
       keep.push_back(j);
     }
     s->active.swap(keep);
+    s->st_check += hphi_server_now(s) - now;
   }
   if (n_active) *n_active = (int)s->active.size();
   if (n_waiting) *n_waiting = (int)s->waiting.size();
+  return 0;
+}
+
+extern "C" int hphi_server_stats(const hphi_server *s, double *out) {
+  if (!s || !out) return hphi_set_error("hphi_server_stats: NULL argument");
+  out[0] = s->st_admit;
+  out[1] = s->st_gpu;
+  out[2] = s->st_check;
+  out[3] = (double)s->st_ticks;
+  out[4] = (double)s->st_job_steps;
   return 0;
 }
 
