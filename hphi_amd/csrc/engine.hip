@@ -662,6 +662,20 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
   } else if (k == "grid_per_cu") {
     if (value < 1 || value > 64) return hphi_set_error("grid_per_cu out of range");
     e->grid_per_cu = value;
+  } else if (k == "use_blk") {
+    e->use_blk = value ? 1 : 0;
+  } else if (k == "blk_tpb") {
+    if (value != 0 && value != 256 && value != 1024) return hphi_set_error("blk_tpb must be 0, 256 or 1024");
+    e->bk_tpb = value;
+  } else if (k == "blk_wgs_per_cu") {
+    if (value < 0 || value > 64) return hphi_set_error("blk_wgs_per_cu out of range");
+    e->bk_wgs_per_cu = value;
+  } else if (k.rfind("blk_cpl", 0) == 0) {
+    int kk = atoi(k.c_str() + 7), slot = kk == 2 ? 0 : kk == 4 ? 1 : kk == 8 ? 2 : kk == 16 ? 3 : -1;
+    if (slot < 0) return hphi_set_error("unknown tuning knob %s", knob);
+    if (value != 0 && !hphi_blk_cpl_ok(kk, value))
+      return hphi_set_error("%s: %d columns per lane not available", knob, value);
+    e->bk_cpl[slot] = value;
   } else {
     return hphi_set_error("unknown tuning knob %s", knob);
   }
@@ -677,7 +691,14 @@ extern "C" int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int 
   else if (k == "xcd_map") *value = e->xcd_map;
   else if (k == "use_c2") *value = e->use_c2 && e->c2_ok;
   else if (k == "grid_per_cu") *value = e->grid_per_cu;
-  else return hphi_set_error("unknown tuning knob %s", knob);
+  else if (k == "use_blk") *value = e->use_blk && hphi_blk_ok(e);
+  else if (k == "blk_tpb" || k == "blk_wgs_per_cu" || k.rfind("blk_cpl", 0) == 0) {
+    int kk = k.rfind("blk_cpl", 0) == 0 ? atoi(k.c_str() + 7) : 16;
+    if (kk != 2 && kk != 4 && kk != 8 && kk != 16) return hphi_set_error("unknown tuning knob %s", knob);
+    int cpl, tpb, wpc;
+    hphi_blk_shape(e, kk, &cpl, &tpb, &wpc);
+    *value = k == "blk_tpb" ? tpb : k == "blk_wgs_per_cu" ? wpc : cpl;
+  } else return hphi_set_error("unknown tuning knob %s", knob);
   return 0;
 }
 
@@ -836,6 +857,14 @@ int hphi_hv_launch_ex(hphi_engine *e, int k, double *v0, const double *v1, int o
     if (hphi_hv_launch_canon(e, A, g)) return -1;
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, e->stream, (const double2 *)e->d_partial,
                        g, 1, (double2 *)e->d_prdct);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  if (k > 1 && e->use_blk && hphi_blk_ok(e)) {
+    int g = 0;
+    if (hphi_hv_launch_blk(e, A, k, &g)) return -1;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, e->stream, (const double2 *)e->d_partial,
+                       g, k, (double2 *)e->d_prdct);
     HIPCHK(hipGetLastError());
     return 0;
   }

@@ -99,6 +99,10 @@ struct hphi_engine {
   // canonical single-vector kernel (hv_canon.hip) configuration
   int c2_ok = 0, c2_realc = 1, c2_lds21 = 0, c2_max_lds = 0;
   int c2_tpb = 1024, c2_wgs_per_cu = 1, use_c2 = 1;
+  // block kernel (hv_block.hip): columns per lane for k = 2, 4, 8, 16 (0: default),
+  // workgroup size (0: by table size), workgroups per CU (0: by workgroup size)
+  int bk_cpl[4] = {0, 0, 0, 0};
+  int bk_tpb = 0, bk_wgs_per_cu = 0, use_blk = 1;
 };
 
 extern "C" int hphi_set_error(const char *fmt, ...);
@@ -115,3 +119,7 @@ SzParams hphi_sz_params(const hphi_engine *e);
 int hphi_launch_tables(const SzParams &P, uint64_t sdim, uint64_t size21, uint64_t *jb, uint32_t *l21, uint64_t *l22, uint64_t *list1, hipStream_t st);
 int hphi_launch_diag(uint64_t dim, const uint64_t *list1, int gc, const DTerm *dterms, int nd, double *diag, hipStream_t st);
 int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g);
+int hphi_blk_ok(const hphi_engine *e);
+int hphi_blk_cpl_ok(int k, int cpl);
+void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_per_cu);
+int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid);

@@ -34,21 +34,17 @@ __device__ __forceinline__ bool term_active(const GTerm &T, uint64_t x) {
   return (T.kind == P_EQ) ? (ym == T.tval) : ((T.kind == P_DIFF) ? (pc == 1) : (pc != 1));
 }
 
-// one tile of HV_TPB/K states x K vectors; adds this lane's share of prdct to (pre, pim)
+// (H v1)_j of column `vec` for state j < A.dim: the diagonal plus every
+// off-diagonal term in gather form.  a = v1[j] of that column.
 template <int K, bool CANON>
-__device__ __forceinline__ void hv_tile(const HvArgs &A, uint64_t tile, double &pre, double &pim) {
-  constexpr int SPT = HV_TPB / K;  // states per tile
-  const int vec = threadIdx.x & (K - 1);
-  const int sl = threadIdx.x / K;
+__device__ __forceinline__ void hv_row(const HvArgs &A, uint64_t j, int vec, double2 &a, double &ar, double &ai) {
   const double2 *__restrict__ v1 = (const double2 *)A.v1;
-  double2 *__restrict__ v0 = (double2 *)A.v0;
   {
-    const uint64_t j = tile * SPT + sl;
-    if (j >= A.dim) return;
     const uint64_t x = CANON ? A.list1[j] : j;
-    const double2 a = v1[j * K + vec];
+    a = v1[j * K + vec];
     const double d = A.diag[j];
-    double ar = d * a.x, ai = d * a.y;
+    ar = d * a.x;
+    ai = d * a.y;
     if (CANON) {
       const uint64_t ia0 = x & A.irght, ib0 = x >> A.nlow;
       const int64_t b21 = (int64_t)A.l21[ia0], b22 = (int64_t)A.l22[ib0];
@@ -150,6 +146,22 @@ __device__ __forceinline__ void hv_tile(const HvArgs &A, uint64_t tile, double &
         }
       }
     }
+  }
+}
+
+// one tile of HV_TPB/K states x K vectors; adds this lane's share of prdct to (pre, pim)
+template <int K, bool CANON>
+__device__ __forceinline__ void hv_tile(const HvArgs &A, uint64_t tile, double &pre, double &pim) {
+  constexpr int SPT = HV_TPB / K;  // states per tile
+  const int vec = threadIdx.x & (K - 1);
+  const int sl = threadIdx.x / K;
+  double2 *__restrict__ v0 = (double2 *)A.v0;
+  {
+    const uint64_t j = tile * SPT + sl;
+    if (j >= A.dim) return;
+    double2 a;
+    double ar, ai;
+    hv_row<K, CANON>(A, j, vec, a, ar, ai);
     double2 o = make_double2(ar, ai);
     if (!A.overwrite) {
       const double2 p = v0[j * K + vec];

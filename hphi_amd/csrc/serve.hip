@@ -7,12 +7,24 @@
 //      (tables + diagonal + two Lanczos vectors) fits the arena and a slot is
 //      free, and builds their tables, diagonal and start vector on the device;
 //   2. advances every running job by one step of the reference's plain
-//      Lanczos recurrence (src/Lanczos_EigenValue.c:153-177) with five packed
-//      launches for all jobs together (update, H·v, alpha, beta, beta-final);
-//   3. copies the new (alpha, beta) of all jobs to the host in one transfer,
-//      checks convergence of the lowest n distinct Ritz values
-//      (Cullum-Willoughby filtered) and retires converged jobs, freeing their
-//      arena memory for the next admissions.
+//      Lanczos recurrence (src/Lanczos_EigenValue.c:153-177) with four packed
+//      launches for all jobs together (residual + norm, beta, H·v, alpha)
+//      and one launch of the batched eigenvalue check;
+//   3. copies the new (alpha, beta) and the check results of all jobs to the
+//      host in one transfer each, tests convergence of the lowest n distinct
+//      Ritz values (Cullum-Willoughby filtered) and retires converged jobs,
+//      freeing their arena memory for the next admissions.
+//
+// The recurrence is rearranged so that every vector is touched by two passes
+// per step instead of four.  A job keeps two arrays, R and Q.  At step m,
+// R holds r = v0 after the previous H·v and Q holds w_{m-1}, the previous
+// Lanczos vector before normalisation (v_{m-1} = s_{m-1} w_{m-1}).
+//   pass 1: w_m = r - alpha_{m-1} s_{m-1} w_{m-1}, written over R, and |w_m|^2;
+//   beta_{m-1} = |w_m|, s_m = 1/beta_{m-1};
+//   pass 2: Q <- -beta_{m-1} s_{m-1} Q + s_m (H w_m)  (= -beta v_{m-1} + H v_m)
+//           and alpha_m = s_m^2 <w_m|H w_m>;
+// then R and Q swap roles.  This is the reference's v1 <- (v0 - alpha v1)/beta,
+// v0 <- -beta v1 + H v1 with the division by beta folded into the H·v.
 // Tiles map to jobs through a prefix table; a job's reductions run over its
 // own tiles in a fixed order, so its result is bit-identical whether it runs
 // alone or among other jobs.
@@ -39,15 +51,15 @@
 #define SV_TPB HV_TPB  // one tile = HV_TPB states
 
 struct JobDev {
-  HvArgs hv;
-  int slot, first;
-  int step, check;       // Lanczos step computed this tick (1-based); run the eigenvalue check
+  HvArgs hv;              // hv.v1 = R (holds w_m after pass 1), hv.v0 = Q
+  int slot, pad;
+  int step, check;        // Lanczos step computed this tick (1-based); run the eigenvalue check
   double *ahist, *bhist;  // alpha[1..max_iter], beta[1..max_iter] on the device
 };
 typedef const JobDev __attribute__((address_space(4))) *CJob;
 typedef const uint64_t __attribute__((address_space(4))) *CU64;
 
-// job state per slot (device): [0] alpha, [1] beta, [2] start-vector scale
+// job state per slot (device): [0] alpha_{m-1}, [1] s_{m-1}, [2] beta_{m-1}, [3] s_m
 #define JS_STRIDE 4
 
 __device__ __forceinline__ int find_job(CU64 prefix, int njobs, uint64_t tile) {
@@ -82,44 +94,7 @@ __device__ __forceinline__ HvArgs load_args(CJob J) {
   return A;
 }
 
-// v1 <- (v0 - alpha v1)/beta, v0 <- -beta v1_old  (Lanczos_EigenValue.c:154-160)
-__global__ __launch_bounds__(SV_TPB) void k_mj_update(const JobDev *jobs, const uint64_t *prefix, int njobs,
-                                                      const double *js) {
-  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
-  CJob J = (CJob)jobs + jb;
-  if (J->first) return;
-  const uint64_t j = (blockIdx.x - ((CU64)prefix)[jb]) * SV_TPB + threadIdx.x;
-  if (j >= J->hv.dim) return;
-  const double alpha = js[J->slot * JS_STRIDE + 0], beta = js[J->slot * JS_STRIDE + 1];
-  double2 *v0 = (double2 *)J->hv.v0, *v1 = (double2 *)J->hv.v1;
-  const double2 a = v0[j], b = v1[j];
-  v1[j] = make_double2((a.x - alpha * b.x) / beta, (a.y - alpha * b.y) / beta);
-  v0[j] = make_double2(-beta * b.x, -beta * b.y);
-}
-
-// v0 += H v1 for one tile of one job; prdct partial per tile
-__global__ __launch_bounds__(SV_TPB) void k_mj_hv(const JobDev *jobs, const uint64_t *prefix, int njobs,
-                                                  double2 *partial) {
-  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
-  const HvArgs A = load_args((CJob)jobs + jb);
-  double pre = 0.0, pim = 0.0;
-  hv_tile<1, true>(A, blockIdx.x - ((CU64)prefix)[jb], pre, pim);
-  wg_reduce_store<1>(pre, pim, partial);
-}
-
-// |v0 - alpha v1|^2 per tile
-__global__ __launch_bounds__(SV_TPB) void k_mj_beta(const JobDev *jobs, const uint64_t *prefix, int njobs,
-                                                    const double *js, double *partial) {
-  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
-  CJob J = (CJob)jobs + jb;
-  const uint64_t j = (blockIdx.x - ((CU64)prefix)[jb]) * SV_TPB + threadIdx.x;
-  double s = 0.0;
-  if (j < J->hv.dim) {
-    const double alpha = js[J->slot * JS_STRIDE + 0];
-    const double2 a = ((const double2 *)J->hv.v0)[j], b = ((const double2 *)J->hv.v1)[j];
-    const double r = a.x - alpha * b.x, m = a.y - alpha * b.y;
-    s = r * r + m * m;
-  }
+__device__ __forceinline__ void tile_sum_store(double s, double *partial) {
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   __shared__ double red[SV_TPB / 64];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -129,6 +104,51 @@ __global__ __launch_bounds__(SV_TPB) void k_mj_beta(const JobDev *jobs, const ui
     for (int w = 0; w < SV_TPB / 64; w++) t += red[w];
     partial[blockIdx.x] = t;
   }
+}
+
+// pass 1: w = r - alpha_{m-1} s_{m-1} q over R, |w|^2 per tile
+// (the residual of Lanczos_EigenValue.c:154-160,170-177)
+__global__ __launch_bounds__(SV_TPB) void k_mj_resid(const JobDev *jobs, const uint64_t *prefix, int njobs,
+                                                     const double *js, double *partial) {
+  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
+  CJob J = (CJob)jobs + jb;
+  const uint64_t j = (blockIdx.x - ((CU64)prefix)[jb]) * SV_TPB + threadIdx.x;
+  double s = 0.0;
+  if (j < J->hv.dim) {
+    const double f = js[J->slot * JS_STRIDE + 0] * js[J->slot * JS_STRIDE + 1];
+    double2 *R = (double2 *)J->hv.v1;
+    double2 w = R[j];
+    if (f != 0.0) {
+      const double2 q = ((const double2 *)J->hv.v0)[j];
+      w = make_double2(w.x - f * q.x, w.y - f * q.y);
+      R[j] = w;
+    }
+    s = w.x * w.x + w.y * w.y;
+  }
+  tile_sum_store(s, partial);
+}
+
+// pass 2: Q <- c Q + s_m (H w) for one tile of one job, <w|H w> per tile
+__global__ __launch_bounds__(SV_TPB) void k_mj_hv(const JobDev *jobs, const uint64_t *prefix, int njobs,
+                                                  const double *js, double2 *partial) {
+  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
+  CJob J = (CJob)jobs + jb;
+  const HvArgs A = load_args(J);
+  const double *st = js + J->slot * JS_STRIDE;
+  const double sc = st[3], c = -st[2] * st[1];
+  const uint64_t j = (blockIdx.x - ((CU64)prefix)[jb]) * SV_TPB + threadIdx.x;
+  double pre = 0.0, pim = 0.0;
+  if (j < A.dim) {
+    double2 a;
+    double ar, ai;
+    hv_row<1, true>(A, j, 0, a, ar, ai);
+    double2 *Q = (double2 *)A.v0;
+    const double2 p = Q[j];
+    Q[j] = make_double2(c * p.x + sc * ar, c * p.y + sc * ai);
+    pre = a.x * ar + a.y * ai;
+    pim = a.x * ai - a.y * ar;
+  }
+  wg_reduce_store<1>(pre, pim, partial);
 }
 
 // per job (one workgroup each): fixed-order sum over the job's tiles
@@ -147,26 +167,35 @@ __device__ double job_sum(const double *part, int stride, int comp, uint64_t t0,
   return r;
 }
 
-__global__ __launch_bounds__(256) void k_mj_alpha(const JobDev *jobs, const uint64_t *prefix, const double2 *partial,
-                                                  double *js) {
+// beta_{m-1} = |w_m| and s_m = 1/beta_{m-1}; a beta below 1e-14 ends the
+// recurrence (the Krylov space is exhausted) and s_m = 0 keeps pass 2 finite
+__global__ __launch_bounds__(256) void k_mj_norm(const JobDev *jobs, const uint64_t *prefix, const double *partial,
+                                                 double *js) {
   CJob J = (CJob)jobs + blockIdx.x;
-  double a = job_sum((const double *)partial, 2, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
+  const double b2 = job_sum(partial, 1, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
   if (threadIdx.x == 0) {
-    js[J->slot * JS_STRIDE + 0] = a;
-    J->ahist[J->step] = a;
+    const double b = sqrt(b2);
+    double *st = js + J->slot * JS_STRIDE;
+    st[2] = b;
+    st[3] = (b > 0.0 && (J->step == 1 || b >= 1e-14)) ? 1.0 / b : 0.0;
+    if (J->step >= 2) J->bhist[J->step - 1] = b;
   }
 }
 
-__global__ __launch_bounds__(256) void k_mj_betafin(const JobDev *jobs, const uint64_t *prefix, const double *partial,
-                                                    double *js, double *out) {
+// alpha_m = s_m^2 Re <w|H w>; out = (alpha_m, beta_{m-1}) per job
+__global__ __launch_bounds__(256) void k_mj_alpha(const JobDev *jobs, const uint64_t *prefix, const double2 *partial,
+                                                  double *js, double *out) {
   CJob J = (CJob)jobs + blockIdx.x;
-  double b2 = job_sum(partial, 1, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
+  const double a = job_sum((const double *)partial, 2, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
   if (threadIdx.x == 0) {
-    const double b = sqrt(b2);
-    js[J->slot * JS_STRIDE + 1] = b;
-    J->bhist[J->step] = b;
-    out[2 * blockIdx.x] = js[J->slot * JS_STRIDE + 0];
-    out[2 * blockIdx.x + 1] = b;
+    double *st = js + J->slot * JS_STRIDE;
+    const double sc = st[3];
+    const double al = a * sc * sc;
+    st[0] = al;
+    st[1] = sc;
+    J->ahist[J->step] = al;
+    out[2 * blockIdx.x] = al;
+    out[2 * blockIdx.x + 1] = st[2];
   }
 }
 
@@ -254,9 +283,11 @@ __global__ __launch_bounds__(64) void k_mj_check(const JobDev *jobs, int want, d
   }
 }
 
-// start vector of one job: v1 = SplitMix64 uniform complex, normalised; v0 = 0
-__global__ void k_sv_fill(double2 *v1, double2 *v0, uint64_t dim, uint64_t seed) {
+// start vector of one job: R = SplitMix64 uniform complex (normalised by the
+// first pass), Q = 0, slot state zero
+__global__ void k_sv_fill(double2 *v1, double2 *v0, uint64_t dim, uint64_t seed, double *js, int slot) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < JS_STRIDE) js[slot * JS_STRIDE + i] = 0.0;
   if (i >= dim) return;
   uint64_t z = (seed * 0x9E3779B97F4A7C15ull) ^ (2 * i);
   auto mix = [](uint64_t x) {
@@ -269,26 +300,6 @@ __global__ void k_sv_fill(double2 *v1, double2 *v0, uint64_t dim, uint64_t seed)
   const double u2 = (double)(mix((seed * 0x9E3779B97F4A7C15ull) ^ (2 * i + 1)) >> 11) * (1.0 / 9007199254740992.0);
   v1[i] = make_double2(2.0 * u1 - 1.0, 2.0 * u2 - 1.0);
   v0[i] = make_double2(0.0, 0.0);
-}
-
-__global__ __launch_bounds__(256) void k_sv_norm(const double2 *v, uint64_t dim, double *js, int slot) {
-  __shared__ double s[256];
-  double a = 0.0;
-  for (uint64_t i = threadIdx.x; i < dim; i += 256) a += v[i].x * v[i].x + v[i].y * v[i].y;
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) js[slot * JS_STRIDE + 2] = 1.0 / sqrt(s[0]);
-}
-
-__global__ void k_sv_scale(double2 *v, uint64_t dim, const double *js, int slot) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= dim) return;
-  const double f = js[slot * JS_STRIDE + 2];
-  v[i] = make_double2(v[i].x * f, v[i].y * f);
 }
 
 // ==================================================================== host
@@ -509,10 +520,8 @@ static int admit(hphi_server *s, Job *j) {
   if (hphi_launch_tables(P, h.sdim, h.size21, jb, l21, l22, list1, s->stream)) return -1;
   if (hphi_launch_diag(h.dim, list1, 0, dterms, (int)h.h_dterms.size(), diag, s->stream)) return -1;
   unsigned g = (unsigned)((h.dim + 255) / 256);
-  hipLaunchKernelGGL(k_sv_fill, dim3(g), dim3(256), 0, s->stream, (double2 *)v1, (double2 *)v0, h.dim, j->seed);
-  hipLaunchKernelGGL(k_sv_norm, dim3(1), dim3(256), 0, s->stream, (const double2 *)v1, h.dim, s->d_js, j->slot);
-  hipLaunchKernelGGL(k_sv_scale, dim3(g), dim3(256), 0, s->stream, (double2 *)v1, h.dim, (const double *)s->d_js,
-                     j->slot);
+  hipLaunchKernelGGL(k_sv_fill, dim3(g), dim3(256), 0, s->stream, (double2 *)v1, (double2 *)v0, h.dim, j->seed,
+                     s->d_js, j->slot);
   HIPCHK(hipGetLastError());
   HvArgs &A = j->dev.hv;
   memset(&A, 0, sizeof A);
@@ -531,7 +540,6 @@ static int admit(hphi_server *s, Job *j) {
   A.v1 = v1;
   A.size21 = h.size21;
   j->dev.slot = j->slot;
-  j->dev.first = 1;
   j->dev.ahist = ahist;
   j->dev.bhist = bhist;
   j->step = 0;
@@ -598,16 +606,14 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
     if (T > s->max_tiles) return hphi_set_error("too many tiles in flight");
     HIPCHK(hipMemcpyAsync(s->d_jobs, jd.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipMemcpyAsync(s->d_prefix, prefix.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_mj_update, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
-                       (const double *)s->d_js);
-    hipLaunchKernelGGL(k_mj_hv, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
-                       (double2 *)s->d_part);
-    hipLaunchKernelGGL(k_mj_alpha, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
-                       (const double2 *)s->d_part, s->d_js);
-    hipLaunchKernelGGL(k_mj_beta, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
+    hipLaunchKernelGGL(k_mj_resid, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
                        (const double *)s->d_js, s->d_part);
-    hipLaunchKernelGGL(k_mj_betafin, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
-                       (const double *)s->d_part, s->d_js, s->d_out);
+    hipLaunchKernelGGL(k_mj_norm, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
+                       (const double *)s->d_part, s->d_js);
+    hipLaunchKernelGGL(k_mj_hv, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
+                       (const double *)s->d_js, (double2 *)s->d_part);
+    hipLaunchKernelGGL(k_mj_alpha, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
+                       (const double2 *)s->d_part, s->d_js, s->d_out);
     hipLaunchKernelGGL(k_mj_check, dim3(n), dim3(64), 0, s->stream, s->d_jobs, s->p.n_eigen, s->d_ev);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s->stream));
@@ -623,15 +629,22 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
     for (int i = 0; i < n; i++) {
       Job *j = s->active[i];
       j->step++;
-      j->alpha.push_back(s->h_out[2 * i]);
-      j->beta.push_back(s->h_out[2 * i + 1]);
-      j->dev.first = 0;
       const int m = j->step;
-      const bool exhausted = (uint64_t)m >= j->h.dim || j->beta[m] < 1e-14;
+      j->alpha.push_back(s->h_out[2 * i]);                  // alpha[m]
+      if (m >= 2) j->beta.push_back(s->h_out[2 * i + 1]);  // beta[m-1]
+      double *r = (double *)j->dev.hv.v1;                   // R and Q swap roles
+      j->dev.hv.v1 = j->dev.hv.v0;
+      j->dev.hv.v0 = r;
+      // beta[m-1] ~ 0: the recurrence ended at step m-1 (alpha[m] is void)
+      const bool broken = m >= 2 && j->beta[m - 1] < 1e-14;
+      const bool exhausted = broken || (uint64_t)m >= j->h.dim;
       if (j->dev.check || exhausted) {
         double cur[16];
         int nc;
-        if (j->dev.check) {  // computed on the device this tick
+        if (broken) {
+          j->step = m - 1;
+          nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m - 1, s->p.n_eigen, cur);
+        } else if (j->dev.check) {  // computed on the device this tick
           const double *o = s->h_ev + 17 * (size_t)i;
           nc = (int)o[16];
           for (int q = 0; q < nc; q++) cur[q] = o[q];
@@ -641,7 +654,7 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
         int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
         bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) ||
                     (exhausted && nc >= 1);
-        if (conv || m >= s->p.max_iter) {
+        if (conv || j->step >= s->p.max_iter) {
           retire(s, j, now, conv ? 1 : 0, cur, nc);
           delete j;
           continue;

@@ -101,13 +101,20 @@ __global__ __launch_bounds__(VO_TPB) void k_axpby(const double2 *x, double2 *y, 
   }
 }
 
-__global__ void k_sum_partials(const double *partial, int nblk, int m, double *out) {
-  // out[c] = sum_b partial[b*m + c], fixed order, one thread per c
-  int c = threadIdx.x;
-  if (c >= m) return;
+// out[c] = sum_b partial[b*m + c]: one workgroup per c, strided per-thread
+// sums then a tree, the same order on every call
+__global__ __launch_bounds__(VO_TPB) void k_sum_partials(const double *partial, int nblk, int m, double *out) {
+  __shared__ double sh[VO_TPB];
+  const int c = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblk; b++) s += partial[(uint64_t)b * m + c];
-  out[c] = s;
+  for (int b = threadIdx.x; b < nblk; b += VO_TPB) s += partial[(uint64_t)b * m + c];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = VO_TPB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = sh[0];
 }
 
 static int vo_grid(uint64_t n) {
@@ -121,7 +128,7 @@ static int vo_grid(uint64_t n) {
 static int finish_sum(hphi_engine *e, int g, int m, double *host_out) {
   double *part = (double *)e->d_partial;
   double *out = (double *)e->d_prdct;
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, e->stream, part, g, m, out);
+  hipLaunchKernelGGL(k_sum_partials, dim3(m), dim3(VO_TPB), 0, e->stream, part, g, m, out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(e->h_prdct, out, sizeof(double) * m, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));

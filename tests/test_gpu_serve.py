@@ -85,3 +85,28 @@ def test_lanczos_distinct_mode_matches_independent_solver(A):
         assert r.converged
         assert np.max(np.abs(got[: len(ref)] - ref)) <= EV_TOL, (spec.name, got, ref)
         eng.close()
+
+
+def _distinct(ev, tol=1e-9):
+    out = []
+    for e in np.sort(ev):
+        if not out or e - out[-1] > tol:
+            out.append(e)
+    return np.array(out)
+
+
+def test_server_small_sectors_exhaust_krylov_space(A):
+    """dim 6..70: the recurrence runs out of Krylov space (beta -> 0 or
+    m = dim) before the convergence test fires; the eigenvalues then come
+    from the exhausted tridiagonal and must equal dense diagonalisation."""
+    from tests import brute
+    specs = [L.heisenberg_chain(4, 1.0, 0.0), L.heisenberg_chain(6, 1.0, 0.4), L.heisenberg_chain(8, 1.0, 0.25),
+             L.hubbard_chain(4, 1.0, 2.5)]
+    print("[gpu] serve small sectors", [s.dim for s in specs], flush=True)
+    out = run_server(A, specs, [11, 12, 13, 14])
+    for spec, r in zip(specs, out):
+        ref = _distinct(np.linalg.eigvalsh(brute.dense(spec)))[:4]
+        got = np.array(r.eigenvalues[: r.n_found])
+        assert r.converged and r.n_found >= min(4, len(ref)), (spec.name, r.n_found)
+        assert r.iterations <= spec.dim
+        assert np.max(np.abs(got[: len(ref)] - ref)) <= EV_TOL, (spec.name, got, ref)
