@@ -79,15 +79,103 @@ def gpu_leg(args):
     wall = time.time() - t
     # Lanczos step (H·v + update + beta + tridiagonal), device-resident driver
     lz = eng.lanczos(n_eigen=1, max_iter=args.lanczos_steps, eps=1e-300, target=2, seed=2)
+    v0.free()
+    v1.free()
+    blk = None
+    if args.block_k > 1:  # block H·v: the same sector for k columns in one launch
+        k = args.block_k
+        V1 = capi.DeviceArray(dim * k)
+        V0 = capi.DeviceArray(dim * k)
+        eng.fill_random(V1, k, 2)
+        V0.zero()
+        blk = eng.time_mltply(k, V0, V1, 1, 2) / k
+        V0.free()
+        V1.free()
     res = {
         "dim": dim, "nnz": nnz, "balg": balg, "ms": ms, "wall_s": wall, "setup_s": t_setup,
         "lanczos_step_ms": 1e3 * lz.seconds / max(lz.iterations, 1),
         "device_gb": (eng.device_bytes + 2 * 16 * dim) / 1e9,
+        "block_ms_per_column": blk,
     }
+    eng.close()
+    return res
+
+
+def pmc_probe(args):
+    """A few plain H·v launches on the 8x4 J1-J2 sector, then the same number of
+    launches of a streaming kernel of known traffic (y = a x + b y over two
+    601,080,390-element complex vectors: 32 B read and 16 B written per
+    element) that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    from hphi_amd import capi, lattice as L
+    import numpy as np
+
+    eng = capi.Engine.from_spec(L.baseline_config(3))
+    eng.sz(copy=False)
+    eng.diagonalcalc(copy=False)
+    dim = eng.dim
+    v1 = capi.DeviceArray(dim)
+    v0 = capi.DeviceArray(dim)
+    eng.fill_random(v1, 1, 2)
+    v0.zero()
+    ms = eng.time_mltply(1, v0, v1, 0, args.steps)
+    ab = np.array([0.5, 0.0, 0.25, 0.0])
+    for _ in range(args.steps):
+        capi.check(capi.lib().hphi_vec_axpby(eng.h, v1.ptr, v0.ptr, 1, ab.ctypes.data_as(C.POINTER(C.c_double))))
+    print(json.dumps({"probe": "hv_j1j2_8x4", "dim": dim, "hv_ms": ms, "steps": args.steps,
+                      "calib_read_bytes": 32 * dim, "calib_write_bytes": 16 * dim}), flush=True)
     v0.free()
     v1.free()
     eng.close()
-    return res
+
+
+def _pmc_means(directory: str, counter: str):
+    """Mean counter value per kernel from rocprofv3 --output-format csv."""
+    import csv
+    import glob
+
+    sums = {}
+    for f in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        for row in csv.DictReader(open(f)):
+            if row.get("Counter_Name") != counter:
+                continue
+            name = row["Kernel_Name"]
+            key = "k_hv_canon" if "k_hv_canon" in name else ("k_axpby" if "k_axpby" in name else None)
+            if key:
+                a = sums.setdefault(key, [0.0, 0])
+                a[0] += float(row["Counter_Value"])
+                a[1] += 1
+    return {k: v[0] / v[1] for k, v in sums.items() if v[1]}
+
+
+def reduce_pmc(fetch_dir: str, write_dir: str, out: str, cmd: str):
+    """HBM bytes per H·v launch from a FETCH_SIZE and a WRITE_SIZE run of
+    --pmc-probe, each scaled by what the counter reads for the calibration
+    kernel of known traffic."""
+    from hphi_amd import lattice as L
+
+    dim = L.baseline_config(3).dim
+    f, w = _pmc_means(fetch_dir, "FETCH_SIZE"), _pmc_means(write_dir, "WRITE_SIZE")
+    ff = (32 * dim / 1024) / f["k_axpby"]
+    fw = (16 * dim / 1024) / w["k_axpby"]
+    fetch_b = f["k_hv_canon"] * 1024 * ff
+    write_b = w["k_hv_canon"] * 1024 * fw
+    nnz = 128 * 2 * math.comb(30, 15)
+    lines = [
+        "# HBM traffic of one plain H·v launch (k_hv_canon), J1-J2 8x4 Sz=0, dim 601,080,390",
+        f"# command: {cmd}",
+        f"FETCH_SIZE k_hv_canon (KiB/dispatch, raw): {f['k_hv_canon']:.1f}",
+        f"FETCH_SIZE k_axpby calibration (KiB/dispatch, raw): {f['k_axpby']:.1f}  known: {32 * dim / 1024:.1f}"
+        f"  factor: {ff:.4f}",
+        f"WRITE_SIZE k_hv_canon (KiB/dispatch, raw): {w['k_hv_canon']:.1f}",
+        f"WRITE_SIZE k_axpby calibration (KiB/dispatch, raw): {w['k_axpby']:.1f}  known: {16 * dim / 1024:.1f}"
+        f"  factor: {fw:.4f}",
+        f"fetch_bytes_per_launch: {fetch_b:.6e}",
+        f"write_bytes_per_launch: {write_b:.6e}",
+        f"algorithmic_bytes_per_launch: {b_alg(dim, nnz):.6e}",
+        f"traffic_bytes_per_launch: {fetch_b + write_b:.6e}",
+    ]
+    open(out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
 
 
 def cpu_leg(args):
@@ -188,8 +276,13 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--lanczos-steps", type=int, default=5)
-    ap.add_argument("--cpu-sample", type=float, default=2e7)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=float, default=4e8)
+    ap.add_argument("--cpu-reps", type=int, default=1)
+    ap.add_argument("--block-k", type=int, default=4, help="also time the block H·v with this many columns")
+    ap.add_argument("--pmc-probe", action="store_true", help="H·v + calibration launches for counter runs")
+    ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/traffic_hv_j1j2_8x4.txt from two rocprofv3 --pmc runs of --pmc-probe")
+    ap.add_argument("--pmc-cmd", type=str, default="", help="command line recorded in the traffic file")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", action="store_true", help="served-stream benchmark instead of plain H·v")
     ap.add_argument("--requests", type=int, default=2000)
@@ -203,6 +296,13 @@ def main():
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.reduce_pmc:
+        reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1], os.path.join(ROOT, "profiles", "traffic_hv_j1j2_8x4.txt"),
+                   args.pmc_cmd)
+        return
+    if args.pmc_probe:
+        pmc_probe(args)
+        return
     if args.serve:
         sv = serve_leg(args)
         best = max(sv["points"], key=lambda p: p["solves_per_s"])
@@ -228,6 +328,10 @@ def main():
         "config": {"workload": WORKLOAD, "dim": g["dim"], "nnz_offdiag": g["nnz"],
                    "bytes_alg_per_hv": g["balg"], "hbm_frac": round(value / HBM_PEAK_GBS, 4),
                    "lanczos_step_ms": round(g["lanczos_step_ms"], 3),
+                   "block_k": args.block_k if g["block_ms_per_column"] else None,
+                   "block_ms_per_column": round(g["block_ms_per_column"], 3) if g["block_ms_per_column"] else None,
+                   "block_hbm_frac_per_column": round(g["balg"] / (g["block_ms_per_column"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                   if g["block_ms_per_column"] else None,
                    "device_gb": round(g["device_gb"], 2), "setup_s": round(g["setup_s"], 2)},
         "roofline": {"bound": "hbm", "achieved": round(value, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(value / HBM_PEAK_GBS, 4),

@@ -32,7 +32,7 @@ EXPORTED = [
     "hphi_read_def", "hphi_model_desc_free", "hphi_tridiag_lowest",
     "hphi_lanczos_good_eigenvalues", "hphi_lanczos_converged",
     "hphi_server_create", "hphi_server_destroy", "hphi_server_submit", "hphi_server_tick",
-    "hphi_server_results", "hphi_server_now", "hphi_server_stats",
+    "hphi_server_results", "hphi_server_now", "hphi_server_stats", "hphi_lobpcg", "hphi_herm_eig",
 ]
 
 
@@ -70,6 +70,17 @@ class TpqParams(C.Structure):
 
 class TpqResult(C.Structure):
     _fields_ = [("seconds", C.c_double), ("large_value", C.c_double), ("steps", C.c_int)]
+
+
+class LobpcgParams(C.Structure):
+    _fields_ = [("k", C.c_int), ("max_iter", C.c_int), ("eps", C.c_double), ("precg", C.c_int),
+                ("large_value", C.c_double), ("seed", C.c_uint64)]
+
+
+class LobpcgResult(C.Structure):
+    _fields_ = [("eigenvalues", C.c_double * 16), ("residuals", C.c_double * 16), ("max_residual", C.c_double),
+                ("iterations", C.c_int), ("converged", C.c_int), ("seconds", C.c_double),
+                ("hv_seconds", C.c_double)]
 
 
 class ServerParams(C.Structure):
@@ -129,6 +140,8 @@ def lib():
         L.hphi_autotune.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_char_p, C.c_size_t]
         L.hphi_tpq.argtypes = [_vp, C.POINTER(TpqParams), _f64p, C.POINTER(TpqResult)]
         L.hphi_diag_maxabs.argtypes = [_vp, _f64p]
+        L.hphi_lobpcg.argtypes = [_vp, C.POINTER(LobpcgParams), _f64p, C.POINTER(LobpcgResult)]
+        L.hphi_herm_eig.argtypes = [C.c_int, _f64p, _f64p, _f64p]
         L.hphi_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(_vp)]
         L.hphi_server_destroy.argtypes = [_vp]
         L.hphi_server_destroy.restype = None
@@ -321,6 +334,16 @@ class Engine:
         check(lib().hphi_autotune(self.h, k, V0.ptr, V1.ptr, reps, buf, 8192))
         return buf.value.decode()
 
+    def lobpcg(self, k: int, max_iter: int = 500, eps: float = 1e-7, precg: bool = False,
+               large_value: float = 0.0, seed: int = 0):
+        """Block LOBPCG for the k lowest eigenpairs; returns (LobpcgResult,
+        history[iterations, 1 + k] = (max residual, mu_0..mu_{k-1}) per step)."""
+        p = LobpcgParams(k, max_iter, eps, 1 if precg else 0, large_value, seed)
+        hist = np.zeros((max(max_iter, 1), k + 1))
+        r = LobpcgResult()
+        check(lib().hphi_lobpcg(self.h, C.byref(p), _p(hist, _f64p), C.byref(r)))
+        return r, hist[: r.iterations].copy()
+
     def tpq(self, k: int, steps: int, seed: int = 0, init_kind: int = -1, layout: int = 1,
             use_graph: bool = True, large_value: float = 0.0):
         """Microcanonical TPQ for k samples; returns (out[steps, k, 8], TpqResult)."""
@@ -344,6 +367,17 @@ class Engine:
         r = LanczosResult()
         check(lib().hphi_lanczos(self.h, C.byref(p), C.byref(r)))
         return r
+
+
+def herm_eig(a: np.ndarray):
+    """Eigenvalues (ascending) and eigenvectors of a small Hermitian matrix
+    (the library's subspace solver)."""
+    a = np.asfortranarray(np.asarray(a, np.complex128))
+    n = a.shape[0]
+    w = np.zeros(n)
+    v = np.zeros((n, n), np.complex128, order="F")
+    check(lib().hphi_herm_eig(n, a.ctypes.data_as(_f64p), _p(w, _f64p), v.ctypes.data_as(_f64p)))
+    return w, v
 
 
 def tridiag_lowest(a: np.ndarray, b: np.ndarray, k: int) -> np.ndarray:

@@ -91,7 +91,9 @@ int hphi_engine_info(const hphi_engine *e, uint64_t *dim, uint64_t *size21, uint
 /* Launch-shape knobs of the H·v kernels (set by the autotuner or by hand):
  *   "tpb" 256|512|1024, "wgs_per_cu" 1..64, "lds21" 0|1 (list_2_1 staged in
  *   LDS), "xcd_map" 0|1 (contiguous state ranges per XCD), "use_c2" 0|1
- *   (32-bit canonical kernel), "grid_per_cu" 1..64 (generic kernel). */
+ *   (32-bit canonical kernel), "grid_per_cu" 1..64 (generic kernel);
+ *   block kernel (k > 1): "use_blk" 0|1, "blk_cpl2|4|8|16" columns per lane
+ *   (1, 4 or k; 0 = default), "blk_tpb" 0|256|1024, "blk_wgs_per_cu" 0..64. */
 int hphi_engine_tune(hphi_engine *e, const char *knob, int value);
 int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int *value);
 /* Times every candidate launch shape on the caller's device vectors (k = block
@@ -196,6 +198,34 @@ typedef struct {
 int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out, hphi_tpq_result *r);
 /* max_j |list_Diagonal[j]| */
 int hphi_diag_maxabs(hphi_engine *e, double *out);
+
+/* Block LOBPCG (CalcByLOBPCG.c:328-632) for the k lowest eigenpairs: the
+ * reference's residual, optional point-Jacobi preconditioner with adaptive
+ * shift, 3k x 3k Gram matrices, Loewdin-orthogonalised subspace problem,
+ * mu <- (mu + mu_sub)/2 and recombination, with one block H·v per step.
+ * Column c of the initial block is the synthetic vector of seed + c.
+ * eigenvalues = Rayleigh quotients of the final block; history (may be NULL,
+ * max_iter rows of 1 + k doubles): (max residual, mu_0 .. mu_{k-1}) at the
+ * start of each step, the reference's step log (:395-404). */
+typedef struct {
+  int k;              /* block size = eigenpairs (k_exct): 1, 2, 4, 8 or 16 */
+  int max_iter;       /* Lanczos_max */
+  double eps;         /* residual threshold: 10^(-LanczosEps/2) in the reference (:384) */
+  int precg;          /* PreCG */
+  double large_value; /* LargeValue: the preconditioner's positive shift is LargeValue*Nsite (:339) */
+  uint64_t seed;
+} hphi_lobpcg_params;
+typedef struct {
+  double eigenvalues[16], residuals[16];
+  double max_residual;
+  int iterations, converged;
+  double seconds, hv_seconds;
+} hphi_lobpcg_result;
+int hphi_lobpcg(hphi_engine *e, const hphi_lobpcg_params *p, double *history, hphi_lobpcg_result *r);
+/* Eigen-decomposition of an n x n Hermitian matrix (column-major, interleaved
+ * complex): w ascending, V (may be NULL) the eigenvectors as columns; the
+ * subspace solver of hphi_lobpcg (replaces LAPACK zheevd, :60,100). */
+int hphi_herm_eig(int n, const double *A, double *w, double *V);
 
 /* ---- continuous-batching service ----------------------------------------
  * No counterpart in the reference (which solves one model per process).

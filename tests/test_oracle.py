@@ -161,3 +161,75 @@ def test_planted_mistakes_are_caught(oracle_lib):
     H2 = brute.dense(spec2)
     col2 = np.array([orc2.hv(e)[0] for e in np.eye(orc2.dim, dtype=np.complex128)]).T
     assert np.max(np.abs(col2 - H2)) > 0.4
+
+
+def test_tpq_restatement_pinned_by_mathematics(oracle_lib):
+    """Pins the numpy restatement of the reference's mTPQ step (the checker of
+    the device TPQ): repeated (l - H/Ns) drives the state to the ground state,
+    so E(step) falls monotonically to the dense E0 and inv_temp grows; at step
+    0 the row is <v|H|v>, <v|H^2|v> and |v| of the normalised start vector.
+    A wrong sign of H/Ns (the state climbs to the top of the spectrum), a
+    missing 1/Ns or a wrong inv_temp formula fails one of these."""
+    from tests.tpq_restated import tpq_restated
+    O = oracle_lib
+    spec = L.heisenberg_chain(10, 1.0, 0.2)
+    orc = _oracle(O, spec)
+    H = brute.dense(spec)
+    w = np.linalg.eigvalsh(H)
+    v = O.fill_vector(5, orc.dim)
+    Ns = spec.nsite
+    lv = 3.0 * np.max(np.abs(orc.diag[1:])) / Ns
+    rows = tpq_restated(lambda x: orc.hv(x)[0], v, 3000, lv, Ns)
+    u = v / np.linalg.norm(v)
+    assert abs(rows[0, 1] - np.vdot(u, H @ u).real) <= 1e-12 * np.abs(w).max()
+    assert abs(rows[0, 2] - np.vdot(H @ u, H @ u).real) <= 1e-12 * np.abs(w).max() ** 2
+    assert abs(rows[0, 3] - np.linalg.norm(v)) <= 1e-12 * np.linalg.norm(v)
+    E = rows[:, 1]
+    assert np.all(np.diff(E) <= 1e-12)              # energy decreases step by step
+    assert abs(E[-1] - w[0]) <= 1e-6 * abs(w[0])   # ... to the ground state
+    beta = rows[1:, 0]
+    assert np.all(np.diff(beta) > 0)
+    s = np.arange(1, len(rows))
+    assert np.allclose(beta, 2.0 * s / Ns / (lv - E[1:] / Ns), rtol=1e-14)
+    # the energy fluctuation <H^2> - <H>^2 vanishes at the ground state
+    assert rows[-1, 2] - E[-1] ** 2 <= 1e-5
+
+
+@pytest.mark.parametrize("precg", [False, True])
+def test_lobpcg_restatement_pinned_by_mathematics(oracle_lib, precg):
+    """Pins the numpy restatement of the reference's LOBPCG (the checker of the
+    device LOBPCG): it reaches the k lowest eigenvalues of the dense H (with
+    multiplicity), the residual falls below the threshold, and at step 1 the
+    logged mu are the Rayleigh quotients of the normalised start block."""
+    from tests.lobpcg_restated import lobpcg_restated
+    O = oracle_lib
+    spec = L.triangular_xxz_gc(4, 2, 1.0, 0.8, 0.1)
+    orc = _oracle(O, spec)
+    H = brute.dense(spec)
+    w = np.linalg.eigvalsh(H)
+    k = 4
+    x0 = np.stack([O.fill_vector(21 + c, orc.dim) for c in range(k)], axis=1)
+    ev, hist = lobpcg_restated(lambda v: orc.hv(v)[0], x0, 300, 1e-7, diag=orc.diag[1:], precg=precg,
+                               eig_pos_shift=0.0)
+    assert hist[-1, 0] < 1e-7 * max(1.0, np.abs(w[:k]).max())
+    assert np.allclose(np.sort(ev), w[:k], atol=1e-10, rtol=0)
+    u = x0 / np.linalg.norm(x0, axis=0)
+    rq = [np.vdot(u[:, c], H @ u[:, c]).real for c in range(k)]
+    assert np.allclose(hist[0, 1:], rq, atol=1e-12)
+
+
+def test_herm_eig_matches_numpy():
+    """The library's small Hermitian eigensolver (LOBPCG subspace problems)."""
+    from hphi_amd import capi
+    rng = np.random.default_rng(4)
+    for n in (1, 3, 12, 24, 48):
+        a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        a = a + a.conj().T
+        w, v = capi.herm_eig(a)
+        assert np.allclose(w, np.linalg.eigvalsh(a), atol=1e-12 * np.abs(w).max())
+        assert np.allclose(a @ v, v * w, atol=1e-12 * np.abs(w).max())
+        assert np.allclose(v.conj().T @ v, np.eye(n), atol=1e-13)
+    b = np.eye(24) + 1e-9 * (rng.normal(size=(24, 24)) + 1j * rng.normal(size=(24, 24)))
+    b = 0.5 * (b + b.conj().T)  # nearly degenerate, like an overlap matrix
+    w, v = capi.herm_eig(b)
+    assert np.allclose(b @ v, v * w, atol=1e-14)
