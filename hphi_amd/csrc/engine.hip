@@ -212,7 +212,8 @@ __global__ __launch_bounds__(256) void k_list1(SzParams P, const uint64_t *jb, u
 // (diagonalcalc.c:105-180: CoulombIntra, Chemi, CoulombInter, Hund) with the
 // same expressions, so the result is bit-identical.
 __global__ void k_diag(uint64_t dim, const uint64_t *__restrict__ list1, int gc,
-                       const DTerm *__restrict__ terms, int nterms, double *__restrict__ diag) {
+                       const DTerm *__restrict__ terms, int nterms, double *__restrict__ diag, int stride,
+                       int col) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= dim) return;
   uint64_t x = gc ? j : list1[j];
@@ -250,7 +251,7 @@ __global__ void k_diag(uint64_t dim, const uint64_t *__restrict__ list1, int gc,
       } break;
     }
   }
-  diag[j] = D;
+  diag[j * stride + col] = D;
 }
 
 // ------------------------------------------------------------------- H·v --
@@ -733,8 +734,9 @@ int hphi_launch_tables(const SzParams &P, uint64_t sdim, uint64_t size21, uint64
 }
 
 int hphi_launch_diag(uint64_t dim, const uint64_t *list1, int gc, const DTerm *dterms, int nd, double *diag,
-                     hipStream_t st) {
-  hipLaunchKernelGGL(k_diag, dim3(grid_for(dim, 256)), dim3(256), 0, st, dim, list1, gc, dterms, nd, diag);
+                     hipStream_t st, int stride, int col) {
+  hipLaunchKernelGGL(k_diag, dim3(grid_for(dim, 256)), dim3(256), 0, st, dim, list1, gc, dterms, nd, diag, stride,
+                     col);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -117,7 +117,9 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e);
 int hphi_ensure_binom_table();
 SzParams hphi_sz_params(const hphi_engine *e);
 int hphi_launch_tables(const SzParams &P, uint64_t sdim, uint64_t size21, uint64_t *jb, uint32_t *l21, uint64_t *l22, uint64_t *list1, hipStream_t st);
-int hphi_launch_diag(uint64_t dim, const uint64_t *list1, int gc, const DTerm *dterms, int nd, double *diag, hipStream_t st);
+// diag[j * stride + col] for j < dim
+int hphi_launch_diag(uint64_t dim, const uint64_t *list1, int gc, const DTerm *dterms, int nd, double *diag,
+                     hipStream_t st, int stride = 1, int col = 0);
 int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g);
 int hphi_blk_ok(const hphi_engine *e);
 int hphi_blk_cpl_ok(int k, int cpl);
