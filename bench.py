@@ -101,6 +101,48 @@ def gpu_leg(args):
     return res
 
 
+def cfg4_leg(args):
+    """Workload 4: triangular 13x2 XXZ in a transverse field (SpinGC, dim 2^26).
+    mTPQ with k = 16 complex Gaussian samples: per-step time of the block
+    layout (one block H·v for all samples, hipGraph-captured step) and of the
+    per-sample layout (k single-vector H·v); then LOBPCG with a block of 8."""
+    from hphi_amd import capi, lattice as L
+
+    spec = L.baseline_config(4)
+    eng = capi.Engine.from_spec(spec)
+    eng.sz(copy=False)
+    eng.diagonalcalc(copy=False)
+    dim = eng.dim
+    k = 16
+    nnz_state = None
+    out = {"dim": dim, "k": k}
+    for layout, name in ((1, "block"), (0, "per_sample")):
+        t = []
+        for steps in (args.tpq_steps // 4, args.tpq_steps):
+            _, r = eng.tpq(k, steps, seed=0, init_kind=-1, layout=layout, use_graph=True)
+            t.append(r.seconds)
+        ms = 1e3 * (t[1] - t[0]) / (args.tpq_steps - args.tpq_steps // 4)
+        out[f"tpq_{name}_ms_per_step"] = round(ms, 3)
+        out[f"tpq_{name}_ms_per_sample_step"] = round(ms / k, 3)
+    # off-diagonal elements per H·v: 78 exchange entries act where the two spins differ
+    # (half of the states), 52 transverse entries act where their spin matches (half)
+    nnz = 78 * dim // 2 + 52 * dim // 2
+    nnz_state = nnz / dim
+    balg = k * (16 * nnz + 16 * dim) + 8 * dim + 48 * dim * k  # overwrite H·v + diag + fused step pass
+    out["tpq_block_gbs"] = round(balg / (out["tpq_block_ms_per_step"] / 1e3) / 1e9, 1)
+    out["offdiag_per_state"] = nnz_state
+    r, hist = eng.lobpcg(8, max_iter=args.lobpcg_iters, eps=1e-7, seed=0)
+    out["lobpcg_k"] = 8
+    out["lobpcg_iterations"] = r.iterations
+    out["lobpcg_converged"] = bool(r.converged)
+    out["lobpcg_ms_per_iteration"] = round(1e3 * r.seconds / max(r.iterations, 1), 2)
+    out["lobpcg_hv_fraction"] = round(r.hv_seconds / max(r.seconds, 1e-9), 3)
+    out["lobpcg_eigenvalues"] = [round(x, 10) for x in r.eigenvalues[:8]]
+    out["lobpcg_max_residual"] = r.max_residual
+    eng.close()
+    return out
+
+
 def pmc_probe(args):
     """A few plain H·v launches on the 8x4 J1-J2 sector, then the same number of
     launches of a streaming kernel of known traffic (y = a x + b y over two
@@ -280,6 +322,9 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--block-k", type=int, default=4, help="also time the block H·v with this many columns")
     ap.add_argument("--pmc-probe", action="store_true", help="H·v + calibration launches for counter runs")
+    ap.add_argument("--cfg4", action="store_true", help="TPQ (k=16) and LOBPCG (block 8) on the 26-site SpinGC workload")
+    ap.add_argument("--tpq-steps", type=int, default=40)
+    ap.add_argument("--lobpcg-iters", type=int, default=60)
     ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic_hv_j1j2_8x4.txt from two rocprofv3 --pmc runs of --pmc-probe")
     ap.add_argument("--pmc-cmd", type=str, default="", help="command line recorded in the traffic file")
@@ -302,6 +347,14 @@ def main():
         return
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.cfg4:
+        c4 = cfg4_leg(args)
+        print(json.dumps({"metric": "mTPQ ms per step (k=16) and LOBPCG ms per iteration (block 8)",
+                          "value": c4["tpq_block_ms_per_step"], "unit": "ms/step", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "c128", "data": "synthetic",
+                          "config": {"workload": "triangular 13x2 XXZ Jz=1 Jxy=0.8 hx=0.1, SpinGC dim 2^26", **c4}}),
+              flush=True)
         return
     if args.serve:
         sv = serve_leg(args)

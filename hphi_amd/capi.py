@@ -417,10 +417,10 @@ class Server:
         return a.value, w.value
 
     def stats(self):
-        out = np.zeros(5)
+        out = np.zeros(8)
         check(lib().hphi_server_stats(self.h, _p(out, _f64p)))
         return {"admit_s": out[0], "gpu_s": out[1], "check_s": out[2], "ticks": int(out[3]),
-                "job_steps": int(out[4])}
+                "job_steps": int(out[4]), "slab_steps": int(out[5])}
 
     def results(self, max_n: int = 4096):
         buf = (JobResult * max_n)()

