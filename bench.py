@@ -278,6 +278,7 @@ def serve_leg(args):
         eng.close()
     seq_s = (time.time() - t) / len(idx)
     out["sequential_solves_per_s"] = 1.0 / seq_s
+    print(f"[serve] sequential: {len(idx)} solves, {1.0 / seq_s:.1f}/s", file=sys.stderr, flush=True)
     out["sequential_sample"] = int(len(idx))
     rates = [float(x) for x in args.rates.split(",")]
     points = []
@@ -293,16 +294,23 @@ def serve_leg(args):
         ticks = 0
         while srv.now() < start:
             time.sleep(0.001)
+        last = time.time()
         while len(res) < len(reqs):
             srv.tick()
             ticks += 1
             res += srv.results()
+            if time.time() - last > 10.0:
+                last = time.time()
+                print(f"[serve] rate {rate}: {len(res)}/{len(reqs)} done", file=sys.stderr, flush=True)
         st = srv.stats()
         srv.close()
         lat = np.array([r.t_done - r.t_arrival for r in res])
         makespan = max(r.t_done for r in res) - start
         conv = sum(r.converged for r in res)
+        unconv = [{"id": int(r.id), "dim": int(r.dim), "iterations": int(r.iterations),
+                   "model": reqs[r.id].name} for r in res if not r.converged]
         points.append({"rate": rate, "solves_per_s": len(res) / makespan, "p50_s": float(np.percentile(lat, 50)),
+                       "unconverged": unconv[:8],
                        "p99_s": float(np.percentile(lat, 99)), "makespan_s": makespan,
                        "converged": int(conv), "mean_iterations": float(np.mean([r.iterations for r in res])),
                        **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}})

@@ -270,8 +270,8 @@ int hphi_lobpcg(hphi_engine *e, const hphi_lobpcg_params *prm, double *history, 
       for (int c = 0; c < k; c++) sw[c] = 1.0 / sqrt(nrm[c]);
     }
     {
-      double t = wall();
-      if (hphi_lb_hv(e, k, W, w)) goto done;
+      double t = wall(); /* the stream is idle here: the last reduction synchronised it */
+      if (hphi_lb_hv(e, k, W, w) || hphi_engine_sync(e)) goto done;
       thv += wall() - t;
     }
     /* Gram matrices of the stored vectors, then the column scales */

@@ -16,14 +16,16 @@
 //      SL_K columns of diagonal and two Lanczos vectors, alpha/beta histories)
 //      fits the arena; a new column gets its couplings, its diagonal and its
 //      start vector on the device;
-//   2. advances every running job by one step of the reference's plain
-//      Lanczos recurrence (src/Lanczos_EigenValue.c:153-177) with four packed
-//      launches over all slabs (residual + norm, beta, H·v, alpha) and one
-//      launch of the batched eigenvalue check;
-//   3. copies the new (alpha, beta) and the check results of all jobs to the
-//      host in one transfer each, tests convergence of the lowest n distinct
-//      Ritz values (Cullum-Willoughby filtered) and retires converged jobs,
-//      freeing a slab's arena memory when its last job leaves.
+//   2. advances every running job by S steps (S = the convergence-check
+//      interval, at most SL_SMAX; fewer when a job reaches its dimension or
+//      the step limit) of the reference's plain Lanczos recurrence
+//      (src/Lanczos_EigenValue.c:153-177): per step four packed launches over
+//      all slabs (residual + norm, beta, H·v, alpha), then one launch of the
+//      batched eigenvalue check, with no host round trip between the steps;
+//   3. copies the (alpha, beta) of every step and the check results of all
+//      jobs to the host in one transfer each, tests convergence of the lowest
+//      n distinct Ritz values (Cullum-Willoughby filtered) and retires
+//      converged jobs, freeing a slab's arena memory when its last job leaves.
 //
 // The recurrence is rearranged so that every vector is touched by two passes
 // per step.  A slab keeps two arrays, R and Q.  At step m of a column, R
@@ -63,6 +65,8 @@
 #define SV_TPB 256  // one tile = SV_TPB states of one slab
 #define SL_K 4      // columns (jobs) per slab
 #define SL_U 2      // terms per gather batch (SL_U * SL_K gathers in flight)
+#define SL_SMAX 16  // Lanczos steps per tick at most
+#define SL_TPW 4    // tiles per workgroup (a chunk)
 
 struct SlabDev {
   uint64_t dim;
@@ -79,9 +83,10 @@ struct SlabDev {
   int realc;  // columns whose couplings are all real
   int slot[SL_K];
 };
-struct ColDev {  // one running job
+struct ColDev {  // one running job in one step of a tick
   int slab, col, slot;
-  int step, check;  // Lanczos step computed this tick (1-based); run the eigenvalue check
+  int step, check;  // Lanczos step computed (1-based); run the eigenvalue check after it
+  int job;          // index of the job in the tick (rows of the outputs)
   double *ahist, *bhist;
 };
 typedef const SlabDev __attribute__((address_space(4))) *CSlab;
@@ -102,50 +107,44 @@ __device__ __forceinline__ int find_slab(CU64 prefix, int n, uint64_t tile) {
   return lo;
 }
 
-// per-column sums over the workgroup (fixed order), stored at partial[blk*SL_K + c]
-__device__ __forceinline__ void sl_store4(const double *v, double *partial) {
-  __shared__ double red[SV_TPB / 64][SL_K];
-  double s[SL_K];
-#pragma unroll
-  for (int c = 0; c < SL_K; c++) {
-    s[c] = v[c];
-    for (int off = 32; off > 0; off >>= 1) s[c] += __shfl_xor(s[c], off, 64);
+// pass 1: w = r - alpha_{m-1} s_{m-1} q over R per running column, |w|^2 per
+// chunk of SL_TPW tiles.  Lane (state, column) = (base + t / SL_K, t % SL_K), so
+// every load and store of a wave covers one contiguous range of rows.
+__global__ __launch_bounds__(SV_TPB) void k_sl_resid(const SlabDev *slabs, const uint64_t *prefix, int ns,
+                                                     const double *js, double *partial) {
+  const int sb = find_slab((CU64)prefix, ns, blockIdx.x);
+  CSlab S = (CSlab)slabs + sb;
+  const int c = threadIdx.x % SL_K;
+  const bool run = (S->mask >> c) & 1;
+  const int sl = S->slot[c];
+  const double f = js[sl * JS_STRIDE + 0] * js[sl * JS_STRIDE + 1];
+  const uint64_t e0 = (blockIdx.x - ((CU64)prefix)[sb]) * SL_TPW * SV_TPB * SL_K + threadIdx.x;
+  const uint64_t n = S->dim * SL_K, cend = e0 - threadIdx.x + (uint64_t)SL_TPW * SV_TPB * SL_K;
+  const uint64_t eend = n < cend ? n : cend;
+  double2 *__restrict__ R = S->R;
+  const double2 *__restrict__ Q = S->Q;
+  double acc = 0.0;
+  if (run) {
+    for (uint64_t e = e0; e < eend; e += SV_TPB) {
+      double2 w = R[e];
+      if (f != 0.0) {
+        const double2 q = Q[e];
+        w = make_double2(w.x - f * q.x, w.y - f * q.y);
+        R[e] = w;
+      }
+      acc += w.x * w.x + w.y * w.y;
+    }
   }
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int c = 0; c < SL_K; c++) red[threadIdx.x >> 6][c] = s[c];
+  // lanes with equal t % SL_K hold one column
+  for (int off = 32; off >= SL_K; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  __shared__ double red[SV_TPB / 64][SL_K];
+  if ((threadIdx.x & 63) < SL_K) red[threadIdx.x >> 6][threadIdx.x & 63] = acc;
   __syncthreads();
   if (threadIdx.x < SL_K) {
     double t = 0.0;
     for (int w = 0; w < SV_TPB / 64; w++) t += red[w][threadIdx.x];
     partial[(uint64_t)blockIdx.x * SL_K + threadIdx.x] = t;
   }
-}
-
-// pass 1: w = r - alpha_{m-1} s_{m-1} q over R per running column, |w|^2 per tile
-// (the residual of Lanczos_EigenValue.c:154-160,170-177)
-__global__ __launch_bounds__(SV_TPB) void k_sl_resid(const SlabDev *slabs, const uint64_t *prefix, int ns,
-                                                     const double *js, double *partial) {
-  const int sb = find_slab((CU64)prefix, ns, blockIdx.x);
-  CSlab S = (CSlab)slabs + sb;
-  const uint64_t j = (blockIdx.x - ((CU64)prefix)[sb]) * SV_TPB + threadIdx.x;
-  const int mask = S->mask;
-  double acc[SL_K];
-#pragma unroll
-  for (int c = 0; c < SL_K; c++) {
-    acc[c] = 0.0;
-    if (!((mask >> c) & 1) || j >= S->dim) continue;
-    const int sl = S->slot[c];
-    const double f = js[sl * JS_STRIDE + 0] * js[sl * JS_STRIDE + 1];
-    double2 w = S->R[j * SL_K + c];
-    if (f != 0.0) {
-      const double2 q = S->Q[j * SL_K + c];
-      w = make_double2(w.x - f * q.x, w.y - f * q.y);
-      S->R[j * SL_K + c] = w;
-    }
-    acc[c] = w.x * w.x + w.y * w.y;
-  }
-  sl_store4(acc, partial);
 }
 
 __device__ __forceinline__ bool sl_pred(int kind, uint32_t tmask, uint32_t tval, uint32_t x) {
@@ -183,20 +182,17 @@ __device__ __forceinline__ void sl_load(const double2 *__restrict__ row, bool ac
   }
 }
 
-// pass 2: Q_c <- cc_c Q_c + s_c (H w)_c for the running columns of one tile;
-// <w_c|H w_c> per tile.  Gather form of the reference's per-term scatter
-// (mltplySpinCore.c:273-307, mltplyHubbardCore.c:305-347).
-__global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const uint64_t *prefix, int ns,
-                                                  const double *js, double2 *partial) {
-  const int sb = find_slab((CU64)prefix, ns, blockIdx.x);
-  CSlab S = (CSlab)slabs + sb;
+// (H w)_c of one state j of slab S for the running columns, then
+// Q_c <- cc_c Q_c + s_c (H w)_c and <w_c|H w_c> accumulated into (pre, pim).
+// Gather form of the reference's per-term scatter (mltplySpinCore.c:273-307,
+// mltplyHubbardCore.c:305-347); list_2_1 is read from the LDS copy s21.
+__device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t *s21, const double *js, double *pre,
+                                            double *pim) {
   const uint64_t dim = S->dim;
-  const uint64_t j = (blockIdx.x - ((CU64)prefix)[sb]) * SV_TPB + threadIdx.x;
   const bool valid = j < dim;
   const uint64_t jj = valid ? j : dim - 1;
   const int mask = S->mask, realc = S->realc;
   const double2 *__restrict__ R = S->R;
-  const uint32_t *__restrict__ l21 = S->l21;
   const uint64_t *__restrict__ l22 = S->l22;
   const uint32_t irght = (uint32_t)S->irght;
   const int nlow = S->nlow;
@@ -218,7 +214,7 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const ui
     ai[c] = d * own[c].y;
   }
   const uint32_t ia = x & irght, ib = x >> nlow;
-  const int64_t b21 = (int64_t)l21[ia];
+  const int64_t b21 = (int64_t)s21[ia];
   // ---- low-half terms: same block
   for (int t = 0; t < S->n_low; t += SL_U) {
     uint64_t idx[SL_U];
@@ -227,7 +223,7 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const ui
     for (int u = 0; u < SL_U; u++) {
       const GTerm T = load_term(tl + t + u);
       const uint32_t y = x ^ (uint32_t)T.flip;
-      const int64_t l = (int64_t)l21[y & irght];
+      const int64_t l = (int64_t)s21[y & irght];
       idx[u] = (uint64_t)((int64_t)j + l - b21);
       act[u] = valid & sl_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x) & (l != 0) & (idx[u] < dim);
     }
@@ -275,7 +271,7 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const ui
         const GTerm T = load_term(tm + t + u);
         const uint32_t y = x ^ (uint32_t)T.flip;
         const int64_t lb = (int64_t)l22[(xh ^ (uint32_t)T.flip) >> nlow];
-        const int64_t la = (int64_t)l21[y & irght];
+        const int64_t la = (int64_t)s21[y & irght];
         idx[u] = (uint64_t)(la + lb - 3);
         act[u] = valid & sl_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x) & (la != 0) & (lb != 0) &
                  (idx[u] < dim);
@@ -301,7 +297,7 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const ui
         int64_t la = b21, lb = 0;
         if (a) {
           lb = (int64_t)l22[y >> nlow];
-          if ((y & irght) != ia) la = (int64_t)l21[y & irght];
+          if ((y & irght) != ia) la = (int64_t)s21[y & irght];
         }
         idx[u] = (uint64_t)(la + lb - 3);
         act[u] = a & (la != 0) & (lb != 0) & (idx[u] < dim);
@@ -316,18 +312,38 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const ui
       }
     }
   }
-  double pre[SL_K], pim[SL_K];
 #pragma unroll
   for (int c = 0; c < SL_K; c++) {
-    pre[c] = pim[c] = 0.0;
     if (!((mask >> c) & 1) || !valid) continue;
     const int sl = S->slot[c];
     const double sc = js[sl * JS_STRIDE + 3], cc = -js[sl * JS_STRIDE + 2] * js[sl * JS_STRIDE + 1];
     double2 *q = S->Q + j * SL_K + c;
     const double2 p = *q;
     *q = make_double2(cc * p.x + sc * ar[c], cc * p.y + sc * ai[c]);
-    pre[c] = own[c].x * ar[c] + own[c].y * ai[c];
-    pim[c] = own[c].x * ai[c] - own[c].y * ar[c];
+    pre[c] += own[c].x * ar[c] + own[c].y * ai[c];
+    pim[c] += own[c].x * ai[c] - own[c].y * ar[c];
+  }
+}
+
+// pass 2 over one chunk of SL_TPW tiles of one slab: Q_c <- cc_c Q_c + s_c (H w)_c
+// for the running columns; <w_c|H w_c> per chunk
+__global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const uint64_t *prefix, int ns,
+                                                  const double *js, double2 *partial) {
+  extern __shared__ uint16_t s21[];
+  const int sb = find_slab((CU64)prefix, ns, blockIdx.x);
+  CSlab S = (CSlab)slabs + sb;
+  const uint32_t n21 = (uint32_t)S->size21;
+  for (uint32_t i = threadIdx.x; i < n21; i += SV_TPB) s21[i] = (uint16_t)S->l21[i];
+  __syncthreads();
+  double pre[SL_K], pim[SL_K];
+#pragma unroll
+  for (int c = 0; c < SL_K; c++) pre[c] = pim[c] = 0.0;
+  const uint64_t t0 = (blockIdx.x - ((CU64)prefix)[sb]) * SL_TPW;
+  const uint64_t dim = S->dim;
+  for (int t = 0; t < SL_TPW; t++) {
+    const uint64_t j = (t0 + t) * SV_TPB + threadIdx.x;
+    if ((t0 + t) * SV_TPB >= dim) break;  // uniform: the chunk's last tiles may lie past the slab
+    sl_hv_state(S, j, s21, js, pre, pim);
   }
   // per-column prdct partials: [blk][SL_K] double2
   __shared__ double2 red[SV_TPB / 64][SL_K];
@@ -396,8 +412,8 @@ __global__ __launch_bounds__(256) void k_sl_alpha(const ColDev *cols, const uint
     st[0] = al;
     st[1] = sc;
     J->ahist[J->step] = al;
-    out[2 * blockIdx.x] = al;
-    out[2 * blockIdx.x + 1] = st[2];
+    out[2 * J->job] = al;
+    out[2 * J->job + 1] = st[2];
   }
 }
 
@@ -462,7 +478,7 @@ __global__ __launch_bounds__(64) void k_sl_check(const ColDev *cols, int want, d
   }
   __syncthreads();
   if (t == 0) {
-    double *o = out + (uint64_t)blockIdx.x * 17;
+    double *o = out + (uint64_t)J->job * 17;
     const double tol = 1e-11 * (scale_s > 1.0 ? scale_s : 1.0);
     int got = 0;
     if (m == 1) {
@@ -561,7 +577,7 @@ struct Job {
   int col = -1, slot = -1;
   int realc = 1;
   std::vector<double> alpha, beta;
-  int step = 0, nprev = 0, check = 0;
+  int step = 0, nprev = 0, tick_steps = 0;
   double prev[16];
   hphi_job_result res{};
 };
@@ -661,12 +677,12 @@ extern "C" int hphi_server_create(const hphi_server_params *p, hphi_server **out
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&s->d_arena, budget) != hipSuccess ||
       hipMalloc(&s->d_js, sizeof(double) * JS_STRIDE * mj) != hipSuccess ||
-      hipMalloc(&s->d_out, sizeof(double) * 2 * mj) != hipSuccess ||
-      hipMalloc(&s->d_slabs, sizeof(SlabDev) * mj) != hipSuccess ||
-      hipMalloc(&s->d_cols, sizeof(ColDev) * mj) != hipSuccess ||
-      hipMalloc(&s->d_prefix, sizeof(uint64_t) * (mj + 1)) != hipSuccess ||
+      hipMalloc(&s->d_out, sizeof(double) * 2 * mj * SL_SMAX) != hipSuccess ||
+      hipMalloc(&s->d_slabs, sizeof(SlabDev) * mj * SL_SMAX) != hipSuccess ||
+      hipMalloc(&s->d_cols, sizeof(ColDev) * mj * SL_SMAX) != hipSuccess ||
+      hipMalloc(&s->d_prefix, sizeof(uint64_t) * (mj + 1) * SL_SMAX) != hipSuccess ||
       hipMalloc(&s->d_part, sizeof(double2) * SL_K * s->max_tiles) != hipSuccess ||
-      hipHostMalloc(&s->h_out, sizeof(double) * 2 * mj, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&s->h_out, sizeof(double) * 2 * mj * SL_SMAX, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&s->d_ev, sizeof(double) * 17 * mj) != hipSuccess ||
       hipHostMalloc(&s->h_ev, sizeof(double) * 17 * mj, hipHostMallocDefault) != hipSuccess)
     rc = hphi_set_error("hphi_server_create: device allocation failed");
@@ -881,112 +897,148 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
   s->st_admit += hphi_server_now(s) - ta;
   if (!s->active.empty()) {
     const double tg = hphi_server_now(s);
-    // 2. one Lanczos step for every running job, packed launches over the slabs
-    std::vector<SlabDev> sd;
-    std::vector<uint64_t> prefix(1, 0);
-    std::map<const Slab *, int> index;
-    for (Slab *b : s->slabs) {
-      if (!b->used()) continue;
-      const hphi_engine &h = b->proto;
-      SlabDev d{};
-      d.dim = h.dim;
-      d.list1 = b->list1;
-      d.l21 = b->l21;
-      d.l22 = b->l22;
-      d.terms = b->terms;
-      d.coef = b->coef;
-      d.diag = b->diag;
-      d.R = b->R;
-      d.Q = b->Q;
-      d.irght = h.irght;
-      d.size21 = h.size21;
-      d.n_low = h.n_low;
-      d.n_high = h.n_high;
-      d.n_mixed = h.n_mixed;
-      d.nlow = h.nlow;
-      for (int c = 0; c < SL_K; c++) {
-        d.slot[c] = b->col[c] ? b->col[c]->slot : 0;
-        if (b->col[c]) {
-          d.mask |= 1 << c;
-          d.realc |= b->col[c]->realc << c;
-        }
-      }
-      index[b] = (int)sd.size();
-      sd.push_back(d);
-      prefix.push_back(prefix.back() + (h.dim + SV_TPB - 1) / SV_TPB);
-    }
-    const int ns = (int)sd.size(), n = (int)s->active.size();
-    std::vector<ColDev> cd(n);
-    const size_t hl = (size_t)s->p.max_iter + 2;
+    // 2. S Lanczos steps for every running job, packed launches over the slabs
+    const int S = std::min(s->p.check_every, SL_SMAX);
+    const int n = (int)s->active.size();
+    std::vector<int> nst(n);  // steps of each job this tick
     for (int i = 0; i < n; i++) {
       Job *j = s->active[i];
-      const int m = j->step + 1;  // step computed this tick
-      j->check = (m % s->p.check_every == 0) || (uint64_t)m >= j->h.dim || m >= s->p.max_iter;
-      double *hb = j->slab->hist + (size_t)j->col * 2 * hl;
-      cd[i] = ColDev{index[j->slab], j->col, j->slot, m, j->check, hb, hb + hl};
+      const uint64_t lim = std::min<uint64_t>(j->h.dim, (uint64_t)s->p.max_iter);
+      nst[i] = (int)std::min<uint64_t>((uint64_t)S, lim - (uint64_t)j->step);
+      j->tick_steps = nst[i];
     }
-    const uint64_t T = prefix[ns];
-    if (T > s->max_tiles) return hphi_set_error("too many tiles in flight");
-    HIPCHK(hipMemcpyAsync(s->d_slabs, sd.data(), sizeof(SlabDev) * ns, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->d_cols, cd.data(), sizeof(ColDev) * n, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->d_prefix, prefix.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_sl_resid, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_slabs, s->d_prefix, ns,
-                       (const double *)s->d_js, s->d_part);
-    hipLaunchKernelGGL(k_sl_norm, dim3(n), dim3(256), 0, s->stream, s->d_cols, s->d_prefix,
-                       (const double *)s->d_part, s->d_js);
-    hipLaunchKernelGGL(k_sl_hv, dim3((unsigned)T), dim3(SV_TPB), 0, s->stream, s->d_slabs, s->d_prefix, ns,
-                       (const double *)s->d_js, (double2 *)s->d_part);
-    hipLaunchKernelGGL(k_sl_alpha, dim3(n), dim3(256), 0, s->stream, s->d_cols, s->d_prefix,
-                       (const double2 *)s->d_part, s->d_js, s->d_out);
-    hipLaunchKernelGGL(k_sl_check, dim3(n), dim3(64), 0, s->stream, s->d_cols, s->p.n_eigen, s->d_ev);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s->stream));
+    std::vector<int> s_chk(S, 0);  // jobs whose last step of the tick is step st
+    for (int i = 0; i < n; i++) s_chk[nst[i] - 1]++;
+    std::vector<SlabDev> sd;
+    std::vector<ColDev> cd;
+    std::vector<uint64_t> prefix;
+    std::vector<int> s_ns(S), s_nc(S), s_sd0(S), s_cd0(S), s_pf0(S);
+    std::map<const Slab *, int> index;
+    const size_t hl = (size_t)s->p.max_iter + 2;
+    uint64_t tiles = 0, max21 = 0;
+    for (int st = 0; st < S; st++) {
+      s_sd0[st] = (int)sd.size();
+      s_cd0[st] = (int)cd.size();
+      s_pf0[st] = (int)prefix.size();
+      prefix.push_back(0);
+      index.clear();
+      for (Slab *b : s->slabs) {
+        const hphi_engine &h = b->proto;
+        SlabDev d{};
+        for (int c = 0; c < SL_K; c++) {
+          const Job *j = b->col[c];
+          d.slot[c] = j ? j->slot : 0;
+          if (j && j->tick_steps > st) {
+            d.mask |= 1 << c;
+            d.realc |= j->realc << c;
+          }
+        }
+        if (!d.mask) continue;
+        d.dim = h.dim;
+        d.list1 = b->list1;
+        d.l21 = b->l21;
+        d.l22 = b->l22;
+        d.terms = b->terms;
+        d.coef = b->coef;
+        d.diag = b->diag;
+        d.R = b->R;
+        d.Q = b->Q;
+        d.irght = h.irght;
+        d.size21 = h.size21;
+        d.n_low = h.n_low;
+        d.n_high = h.n_high;
+        d.n_mixed = h.n_mixed;
+        d.nlow = h.nlow;
+        index[b] = (int)sd.size() - s_sd0[st];
+        sd.push_back(d);
+        prefix.push_back(prefix.back() + (h.dim + (uint64_t)SL_TPW * SV_TPB - 1) / ((uint64_t)SL_TPW * SV_TPB));
+        max21 = std::max<uint64_t>(max21, h.size21);
+        std::swap(b->R, b->Q);  // roles after this step
+      }
+      for (int i = 0; i < n; i++) {
+        if (nst[i] <= st) continue;
+        Job *j = s->active[i];
+        double *hb = j->slab->hist + (size_t)j->col * 2 * hl;
+        cd.push_back(ColDev{index[j->slab], j->col, j->slot, j->step + st + 1, st == nst[i] - 1, i, hb, hb + hl});
+      }
+      s_ns[st] = (int)sd.size() - s_sd0[st];
+      s_nc[st] = (int)cd.size() - s_cd0[st];
+      tiles = std::max<uint64_t>(tiles, prefix.back());
+    }
+    if (tiles > s->max_tiles) return hphi_set_error("too many tiles in flight");
+    const size_t lds = 2 * (size_t)max21;  // list_2_1 as uint16 (rank + 2 < 2^16 for <= 16 low bits)
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void *)k_sl_hv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    HIPCHK(hipMemcpyAsync(s->d_slabs, sd.data(), sizeof(SlabDev) * sd.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_cols, cd.data(), sizeof(ColDev) * cd.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_prefix, prefix.data(), sizeof(uint64_t) * prefix.size(), hipMemcpyHostToDevice,
+                          s->stream));
+    for (int st = 0; st < S; st++) {
+      const int ns = s_ns[st], nc = s_nc[st];
+      if (!nc) continue;
+      const SlabDev *dsl = s->d_slabs + s_sd0[st];
+      const ColDev *dcl = s->d_cols + s_cd0[st];
+      const uint64_t *dpf = s->d_prefix + s_pf0[st];
+      const unsigned T = (unsigned)prefix[s_pf0[st] + ns];
+      double *out = s->d_out + (size_t)st * 2 * n;
+      hipLaunchKernelGGL(k_sl_resid, dim3(T), dim3(SV_TPB), 0, s->stream, dsl, dpf, ns, (const double *)s->d_js,
+                         s->d_part);
+      hipLaunchKernelGGL(k_sl_norm, dim3(nc), dim3(256), 0, s->stream, dcl, dpf, (const double *)s->d_part,
+                         s->d_js);
+      hipLaunchKernelGGL(k_sl_hv, dim3(T), dim3(SV_TPB), lds, s->stream, dsl, dpf, ns, (const double *)s->d_js,
+                         (double2 *)s->d_part);
+      hipLaunchKernelGGL(k_sl_alpha, dim3(nc), dim3(256), 0, s->stream, dcl, dpf, (const double2 *)s->d_part,
+                         s->d_js, out);
+      if (s_chk[st]) hipLaunchKernelGGL(k_sl_check, dim3(nc), dim3(64), 0, s->stream, dcl, s->p.n_eigen, s->d_ev);
+      HIPCHK(hipGetLastError());
+      s->st_slab_steps += ns;
+    }
+    HIPCHK(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * 2 * n * S, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(s->h_ev, s->d_ev, sizeof(double) * 17 * n, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     now = hphi_server_now(s);
     s->st_gpu += now - tg;
     s->st_ticks++;
-    s->st_job_steps += n;
-    s->st_slab_steps += ns;
-    // R and Q swap roles in every slab that stepped
-    for (Slab *b : s->slabs)
-      if (b->used()) std::swap(b->R, b->Q);
     // 3. convergence and retirement
     std::vector<Job *> keep;
     keep.reserve(n);
     for (int i = 0; i < n; i++) {
       Job *j = s->active[i];
-      j->step++;
-      const int m = j->step;
-      j->alpha.push_back(s->h_out[2 * i]);                  // alpha[m]
-      if (m >= 2) j->beta.push_back(s->h_out[2 * i + 1]);  // beta[m-1]
-      // beta[m-1] ~ 0: the recurrence ended at step m-1 (alpha[m] is void)
-      const bool broken = m >= 2 && j->beta[m - 1] < 1e-14;
-      const bool exhausted = broken || (uint64_t)m >= j->h.dim;
-      if (j->check || exhausted) {
-        double cur[16];
-        int nc;
-        if (broken) {
+      s->st_job_steps += nst[i];
+      bool broken = false;
+      for (int st = 0; st < nst[i]; st++) {
+        const double *o = s->h_out + ((size_t)st * n + i) * 2;
+        const int m = j->step + 1;
+        j->alpha.push_back(o[0]);                  // alpha[m]
+        if (m >= 2) j->beta.push_back(o[1]);      // beta[m-1]
+        j->step = m;
+        // beta[m-1] ~ 0: the recurrence ended at step m-1 (alpha[m] and later are void)
+        if (m >= 2 && j->beta[m - 1] < 1e-14) {
+          broken = true;
           j->step = m - 1;
-          nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m - 1, s->p.n_eigen, cur);
-        } else if (j->check) {  // computed on the device this tick
-          const double *o = s->h_ev + 17 * (size_t)i;
-          nc = (int)o[16];
-          for (int q = 0; q < nc; q++) cur[q] = o[q];
-        } else {
-          nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m, s->p.n_eigen, cur);
+          break;
         }
-        const int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
-        const bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) ||
-                          (exhausted && nc >= 1);
-        if (conv || j->step >= s->p.max_iter) {
-          retire(s, j, now, conv ? 1 : 0, cur, nc);
-          delete j;
-          continue;
-        }
-        memcpy(j->prev, cur, sizeof cur);
-        j->nprev = nc;
       }
+      const int m = j->step;
+      const bool exhausted = broken || (uint64_t)m >= j->h.dim;
+      double cur[16];
+      int nc;
+      if (broken) {
+        nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m, s->p.n_eigen, cur);
+      } else {  // the check after the job's last step of the tick ran on the device
+        const double *o = s->h_ev + 17 * (size_t)i;
+        nc = (int)o[16];
+        for (int q = 0; q < nc; q++) cur[q] = o[q];
+      }
+      const int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
+      const bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) || (exhausted && nc >= 1);
+      if (conv || m >= s->p.max_iter) {
+        retire(s, j, now, conv ? 1 : 0, cur, nc);
+        delete j;
+        continue;
+      }
+      memcpy(j->prev, cur, sizeof cur);
+      j->nprev = nc;
       keep.push_back(j);
     }
     s->active.swap(keep);
