@@ -44,7 +44,10 @@ class ModelDesc(C.Structure):
                 ("n_chemi", C.c_int), ("chemi_site", _i32p), ("chemi_spin", _i32p), ("chemi_mu", _f64p),
                 ("n_intra", C.c_int), ("intra_site", _i32p), ("intra_U", _f64p),
                 ("n_inter", C.c_int), ("inter_ij", _i32p), ("inter_V", _f64p),
-                ("n_hund", C.c_int), ("hund_ij", _i32p), ("hund_J", _f64p)]
+                ("n_hund", C.c_int), ("hund_ij", _i32p), ("hund_J", _f64p),
+                ("n_interall", C.c_int), ("interall_idx", _i32p), ("interall_V", _f64p),
+                ("n_interall_diag", C.c_int), ("interall_diag_idx", _i32p), ("interall_diag_V", _f64p),
+                ("n_pairhop", C.c_int), ("pairhop_ij", _i32p), ("pairhop_J", _f64p)]
 
 
 class ModPara(C.Structure):
@@ -213,17 +216,7 @@ class Engine:
 
     @classmethod
     def from_terms(cls, model: int, nsite: int, nup: int, ndown: int, t: Dict[str, np.ndarray]) -> "Engine":
-        keep = {k: np.ascontiguousarray(v) for k, v in t.items()}
-        d = ModelDesc()
-        d.model, d.nsite, d.nup, d.ndown = model, nsite, nup, ndown
-        d.n_exchange, d.exchange_ij, d.exchange_J = len(keep["exchange_J"]), _p(keep["exchange_ij"], _i32p), _p(keep["exchange_J"], _f64p)
-        d.n_pairlift, d.pairlift_ij, d.pairlift_J = len(keep["pairlift_J"]), _p(keep["pairlift_ij"], _i32p), _p(keep["pairlift_J"], _f64p)
-        d.n_transfer, d.transfer_idx, d.transfer_t = len(keep["trans_t"]), _p(keep["trans_idx"], _i32p), _p(keep["trans_t"], _f64p)
-        d.n_chemi, d.chemi_site, d.chemi_spin, d.chemi_mu = (len(keep["chemi_mu"]), _p(keep["chemi_site"], _i32p),
-                                                             _p(keep["chemi_spin"], _i32p), _p(keep["chemi_mu"], _f64p))
-        d.n_intra, d.intra_site, d.intra_U = len(keep["intra_U"]), _p(keep["intra_site"], _i32p), _p(keep["intra_U"], _f64p)
-        d.n_inter, d.inter_ij, d.inter_V = len(keep["inter_V"]), _p(keep["inter_ij"], _i32p), _p(keep["inter_V"], _f64p)
-        d.n_hund, d.hund_ij, d.hund_J = len(keep["hund_J"]), _p(keep["hund_ij"], _i32p), _p(keep["hund_J"], _f64p)
+        d, keep = cls.desc_from_terms(model, nsite, nup, ndown, t)
         h = _vp()
         check(lib().hphi_engine_create(C.byref(d), C.byref(h)))
         return cls(h)
@@ -242,6 +235,18 @@ class Engine:
         d.n_intra, d.intra_site, d.intra_U = len(keep["intra_U"]), _p(keep["intra_site"], _i32p), _p(keep["intra_U"], _f64p)
         d.n_inter, d.inter_ij, d.inter_V = len(keep["inter_V"]), _p(keep["inter_ij"], _i32p), _p(keep["inter_V"], _f64p)
         d.n_hund, d.hund_ij, d.hund_J = len(keep["hund_J"]), _p(keep["hund_ij"], _i32p), _p(keep["hund_J"], _f64p)
+        for k, w in (("interall_idx", 8), ("interall_diag_idx", 4), ("pairhop_ij", 2)):
+            keep.setdefault(k, np.zeros((0, w), np.int32))
+        for k, w in (("interall_V", 2),):
+            keep.setdefault(k, np.zeros((0, w), np.float64))
+        for k in ("interall_diag_V", "pairhop_J"):
+            keep.setdefault(k, np.zeros(0, np.float64))
+        d.n_interall, d.interall_idx, d.interall_V = (len(keep["interall_V"]), _p(keep["interall_idx"], _i32p),
+                                                      _p(keep["interall_V"], _f64p))
+        d.n_interall_diag, d.interall_diag_idx, d.interall_diag_V = (
+            len(keep["interall_diag_V"]), _p(keep["interall_diag_idx"], _i32p), _p(keep["interall_diag_V"], _f64p))
+        d.n_pairhop, d.pairhop_ij, d.pairhop_J = (len(keep["pairhop_J"]), _p(keep["pairhop_ij"], _i32p),
+                                                  _p(keep["pairhop_J"], _f64p))
         return d, keep
 
     @classmethod

@@ -32,11 +32,12 @@ int hphi_set_error(const char *fmt, ...);
 #define MAXLINE 1024
 
 enum { KW_CALCMOD, KW_MODPARA, KW_LOCSPIN, KW_TRANS, KW_INTRA, KW_INTER, KW_HUND, KW_EXCHANGE,
-       KW_ISING, KW_PAIRLIFT, KW_N };
+       KW_ISING, KW_PAIRLIFT, KW_PAIRHOP, KW_INTERALL, KW_N };
 static const char *kw_names[KW_N] = {"CalcMod", "ModPara", "LocSpin", "Trans", "CoulombIntra",
-                                     "CoulombInter", "Hund", "Exchange", "Ising", "PairLift"};
+                                     "CoulombInter", "Hund", "Exchange", "Ising", "PairLift",
+                                     "PairHop", "InterAll"};
 /* keywords of the reference reader (readdef.c:46-71) that this engine does not implement */
-static const char *kw_unsupported[] = {"PairHop", "InterAll", "OneBodyG", "TwoBodyG", "Boost",
+static const char *kw_unsupported[] = {"OneBodyG", "TwoBodyG", "Boost",
                                        "SingleExcitation", "PairExcitation", "SpectrumVec", "Laser",
                                        "TEOneBody", "TETwoBody", "ThreeBodyG", "FourBodyG", "SixBodyG"};
 
@@ -97,7 +98,9 @@ static int read_terms(const char *path, int nint, int nval, tlist *out, int *cou
     int r;
     if (nint == 1) r = sscanf(line, "%d %lf %lf", &ix[0], &v[0], &v[1]);
     else if (nint == 2) r = sscanf(line, "%d %d %lf %lf", &ix[0], &ix[1], &v[0], &v[1]);
-    else r = sscanf(line, "%d %d %d %d %lf %lf", &ix[0], &ix[1], &ix[2], &ix[3], &v[0], &v[1]);
+    else if (nint == 4) r = sscanf(line, "%d %d %d %d %lf %lf", &ix[0], &ix[1], &ix[2], &ix[3], &v[0], &v[1]);
+    else r = sscanf(line, "%d %d %d %d %d %d %d %d %lf %lf", &ix[0], &ix[1], &ix[2], &ix[3], &ix[4], &ix[5],
+                    &ix[6], &ix[7], &v[0], &v[1]);
     if (r < nint + nval) {
       fclose(f);
       return hphi_set_error("%s: bad entry line '%s'", path, line);
@@ -150,7 +153,7 @@ static int read_locspn(const char *path, int nsite, int *nlocspn, int *kind) {
 /* owned storage behind a descriptor returned by hphi_read_def */
 typedef struct {
   hphi_model_desc d;
-  void *blocks[16];
+  void *blocks[32];
   int nblocks;
 } desc_store;
 
@@ -305,7 +308,7 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
   }
 
   tlist tr = {0}, intra = {0}, inter = {0}, hund = {0}, ex = {0}, ising = {0}, pl = {0};
-  tlist edt = {0}, chemi = {0};
+  tlist edt = {0}, chemi = {0}, ph = {0}, ia = {0}, iaoff = {0}, iadiag = {0}, ph2 = {0};
   int rc = 0;
   if (files[KW_TRANS][0] && read_terms(files[KW_TRANS], 4, 2, &tr, NULL)) rc = -1;
   if (!rc && files[KW_INTRA][0] && read_terms(files[KW_INTRA], 1, 1, &intra, NULL)) rc = -1;
@@ -314,6 +317,10 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
   if (!rc && files[KW_EXCHANGE][0] && read_terms(files[KW_EXCHANGE], 2, 1, &ex, NULL)) rc = -1;
   if (!rc && files[KW_ISING][0] && read_terms(files[KW_ISING], 2, 1, &ising, NULL)) rc = -1;
   if (!rc && files[KW_PAIRLIFT][0] && read_terms(files[KW_PAIRLIFT], 2, 1, &pl, NULL)) rc = -1;
+  if (!rc && files[KW_PAIRHOP][0] && read_terms(files[KW_PAIRHOP], 2, 1, &ph, NULL)) rc = -1;
+  if (!rc && files[KW_INTERALL][0] && read_terms(files[KW_INTERALL], 8, 2, &ia, NULL)) rc = -1;
+  if (!rc && (ph.n || ia.n) && calc_model != HPHI_MODEL_HUBBARD)
+    rc = hphi_set_error("PairHop and InterAll are implemented for the canonical Hubbard model");
 
   if (!rc) {
     /* merge duplicates, keeping first-occurrence order (readdef.c:1130-1150) */
@@ -363,6 +370,80 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
     tl_free(&m);
   }
   if (!rc) {
+    /* PairHop: each entry and its reverse (readdef.c:1257-1267) */
+    ph2.w = 2;
+    for (int a = 0; a < ph.n; a++) {
+      int rev[2] = {ph.idx[2 * a + 1], ph.idx[2 * a]};
+      tl_push(&ph2, ph.idx + 2 * a, ph.val[2 * a], 0.0);
+      tl_push(&ph2, rev, ph.val[2 * a], 0.0);
+    }
+    /* InterAll: duplicates summed in first-occurrence order (InputInterAllInfo,
+     * readdef.c:3054-3090), then split (GetDiagonalInterAll, readdef.c:2415-2530) */
+    tlist m = {0};
+    m.w = 8;
+    for (int a = 0; a < ia.n; a++) {
+      int hit = -1;
+      for (int b = 0; b < m.n && hit < 0; b++)
+        if (!memcmp(m.idx + 8 * b, ia.idx + 8 * a, 8 * sizeof(int))) hit = b;
+      if (hit >= 0) {
+        m.val[2 * hit] += ia.val[2 * a];
+        m.val[2 * hit + 1] += ia.val[2 * a + 1];
+      } else {
+        tl_push(&m, ia.idx + 8 * a, ia.val[2 * a], ia.val[2 * a + 1]);
+      }
+    }
+    iaoff.w = 8;
+    iadiag.w = 4;
+    chemi.w = 2;
+    for (int a = 0; a < m.n && !rc; a++) {
+      const int *q = m.idx + 8 * a; /* i s1 j s2 k s3 l s4 */
+      const double vr = m.val[2 * a], vi = m.val[2 * a + 1];
+      if (q[0] == q[2] && q[4] == q[6] && q[1] == q[3] && q[5] == q[7]) {
+        int d4[4] = {q[0], q[1], q[4], q[5]};
+        tl_push(&iadiag, d4, vr, 0.0);
+      } else if (q[0] == q[6] && q[2] == q[4] && q[1] == q[7] && q[3] == q[5]) {
+        int d4[4] = {q[0], q[1], q[2], q[3]}, cs[2] = {q[0], q[1]};
+        tl_push(&iadiag, d4, -vr, 0.0);
+        tl_push(&chemi, cs, -vr, 0.0);
+      } else if (q[1] == q[3] && q[5] == q[7]) {
+        tl_push(&iaoff, q, vr, vi);
+      } else if (q[1] == q[7] && q[3] == q[5]) {
+        int r8[8] = {q[0], q[1], q[6], q[1], q[4], q[3], q[2], q[3]};
+        tl_push(&iaoff, r8, -vr, -vi);
+      } else {
+        rc = hphi_set_error("InterAll (%d %d %d %d %d %d %d %d) does not conserve Sz", q[0], q[1], q[2], q[3], q[4],
+                            q[5], q[6], q[7]);
+      }
+    }
+    tl_free(&m);
+    /* Hermitian pairing: entry a followed by its reverse with the conjugate
+     * value (CheckInterAllHermite, readdef.c:2257-2360) */
+    tlist pr = {0};
+    pr.w = 8;
+    for (int a = 0; a < iaoff.n && !rc; a++) {
+      const int *A = iaoff.idx + 8 * a;
+      int hit = -1;
+      for (int b = 0; b < iaoff.n && hit < 0; b++) {
+        const int *B = iaoff.idx + 8 * b;
+        if (A[0] == B[6] && A[1] == B[7] && A[2] == B[4] && A[3] == B[5] && A[4] == B[2] && A[5] == B[3] &&
+            A[6] == B[0] && A[7] == B[1]) {
+          const double dre = iaoff.val[2 * a] - iaoff.val[2 * b], dim = iaoff.val[2 * a + 1] + iaoff.val[2 * b + 1];
+          if (sqrt(dre * dre + dim * dim) < 1e-12) hit = b;
+        }
+      }
+      if (hit < 0) {
+        rc = hphi_set_error("InterAll (%d %d %d %d %d %d %d %d) has no Hermitian partner", A[0], A[1], A[2], A[3],
+                            A[4], A[5], A[6], A[7]);
+      } else if (a <= hit) {
+        tl_push(&pr, A, iaoff.val[2 * a], iaoff.val[2 * a + 1]);
+        tl_push(&pr, iaoff.idx + 8 * hit, iaoff.val[2 * hit], iaoff.val[2 * hit + 1]);
+      }
+    }
+    if (!rc && pr.n != iaoff.n) rc = hphi_set_error("InterAll entries do not pair into Hermitian conjugates");
+    tl_free(&iaoff);
+    iaoff = pr;
+  }
+  if (!rc) {
     /* Ising -> Hund (-J/2) and CoulombInter (-J/4), appended (readdef.c:1320-1327) */
     hund.w = 2;
     inter.w = 2;
@@ -373,6 +454,7 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
   desc_store *st = NULL;
   int *ch_site = NULL, *ch_spin = NULL;
   double *ch_mu = NULL, *v_intra = NULL, *v_inter = NULL, *v_hund = NULL, *v_ex = NULL, *v_pl = NULL;
+  double *v_ph = NULL, *v_iad = NULL;
   int *s_intra = NULL;
   if (!rc) {
     ch_site = (int *)calloc(chemi.n + 1, sizeof(int));
@@ -396,7 +478,11 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
     REALS(v_hund, hund);
     REALS(v_ex, ex);
     REALS(v_pl, pl);
+    REALS(v_ph, ph2);
+    REALS(v_iad, iadiag);
 #undef REALS
+    double *v_iao = (double *)calloc(2 * iaoff.n + 2, sizeof(double));
+    for (int a = 0; a < 2 * iaoff.n; a++) v_iao[a] = iaoff.val[a];
     double *tt = (double *)calloc(2 * edt.n + 2, sizeof(double));
     for (int a = 0; a < 2 * edt.n; a++) tt[a] = edt.val[a];
     d.n_exchange = ex.n; d.exchange_ij = ex.idx; d.exchange_J = v_ex;
@@ -406,19 +492,23 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
     d.n_intra = intra.n; d.intra_site = s_intra; d.intra_U = v_intra;
     d.n_inter = inter.n; d.inter_ij = inter.idx; d.inter_V = v_inter;
     d.n_hund = hund.n; d.hund_ij = hund.idx; d.hund_J = v_hund;
+    d.n_interall = iaoff.n; d.interall_idx = iaoff.idx; d.interall_V = v_iao;
+    d.n_interall_diag = iadiag.n; d.interall_diag_idx = iadiag.idx; d.interall_diag_V = v_iad;
+    d.n_pairhop = ph2.n; d.pairhop_ij = ph2.idx; d.pairhop_J = v_ph;
     st = (desc_store *)calloc(1, sizeof(desc_store));
     st->d = d;
-    void *owned[] = {tt, ch_site, ch_spin, ch_mu, s_intra, v_intra, v_inter, v_hund, v_ex, v_pl,
-                     ex.idx, pl.idx, edt.idx, inter.idx, hund.idx};
+    void *owned[] = {tt, ch_site, ch_spin, ch_mu, s_intra, v_intra, v_inter, v_hund, v_ex, v_pl, v_ph, v_iad,
+                     v_iao, ex.idx, pl.idx, edt.idx, inter.idx, hund.idx, iaoff.idx, iadiag.idx, ph2.idx};
     for (int a = 0; a < (int)(sizeof owned / sizeof owned[0]); a++) keep(st, owned[a]);
-    ex.idx = pl.idx = edt.idx = inter.idx = hund.idx = NULL;
+    ex.idx = pl.idx = edt.idx = inter.idx = hund.idx = iaoff.idx = iadiag.idx = ph2.idx = NULL;
     *dout = &st->d;
   } else {
     free(ch_site); free(ch_spin); free(ch_mu); free(s_intra); free(v_intra);
-    free(v_inter); free(v_hund); free(v_ex); free(v_pl);
+    free(v_inter); free(v_hund); free(v_ex); free(v_pl); free(v_ph); free(v_iad);
   }
   tl_free(&tr); tl_free(&intra); tl_free(&inter); tl_free(&hund); tl_free(&ex);
   tl_free(&ising); tl_free(&pl); tl_free(&edt); tl_free(&chemi);
+  tl_free(&ph); tl_free(&ia); tl_free(&iaoff); tl_free(&iadiag); tl_free(&ph2);
   if (!rc && mp_out) *mp_out = mp;
   return rc;
 }

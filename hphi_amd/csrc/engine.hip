@@ -249,6 +249,10 @@ __global__ void k_diag(uint64_t dim, const uint64_t *__restrict__ list1, int gc,
         uint64_t ibit = x & T.m1;
         if (ibit == 0 || ibit == T.m1) D += T.V;
       } break;
+      case D_NN: {  // SetDiagonalInterAll, Hubbard (diagonalcalc.c:1113-1435)
+        uint64_t n1 = (x & T.m1) / T.m1, n2 = (x & T.m2) / T.m2;
+        D += (double)(n1 * n2) * T.V;
+      } break;
     }
   }
   diag[j * stride + col] = D;
@@ -360,8 +364,10 @@ static void classify_and_upload_terms(hphi_engine *e) {
       low.push_back(t);
       continue;
     }
+    // the high group's wave-uniform path reads predicate and sign from the high
+    // half only: terms that test or count low bits go to the general mixed group
     bool lo = (t.flip & ~e->irght) == 0;
-    bool hi = (t.flip & e->irght) == 0;
+    bool hi = (t.flip & e->irght) == 0 && (t.tmask & e->irght) == 0 && (t.sgn & e->irght) == 0;
     if (lo) low.push_back(t);
     else if (hi) high.push_back(t);
     else mixed.push_back(t);
@@ -387,6 +393,57 @@ static void classify_and_upload_terms(hphi_engine *e) {
 
 // Host-side preparation: sector sizes, split masks, gather terms (grouped
 // low/high/mixed) and diagonal terms.  No device work.
+// bits strictly between orbitals a and b
+static uint64_t between_mask(int a, int b) {
+  const int lo = std::min(a, b), hi = std::max(a, b);
+  return (hi - lo > 1) ? (((1ull << hi) - 1) & ~((1ull << (lo + 1)) - 1)) : 0ull;
+}
+
+// one hop c+_a c_b on state x as the reference's child functions apply it
+// (child_GC_CisAjt, mltplyHubbardCore.c:431-456: a empty, b occupied, sign
+// from the occupied bits strictly between; a == b is the occupation n_a,
+// child_CisAis :288-298); sign parity restricted to `inside`
+static bool apply_hop(uint64_t &x, int a, int b, uint64_t inside, int &parity) {
+  if (a == b) return (x >> a) & 1;
+  if (((x >> a) & 1) || !((x >> b) & 1)) return false;
+  parity ^= __builtin_popcountll(x & between_mask(a, b) & inside) & 1;
+  x ^= (1ull << a) | (1ull << b);
+  return true;
+}
+
+// V c+_{a1} c_{b1} c+_{a2} c_{b2} in gather form: the reference applies the
+// hop (a2 <- b2) to the source and then (a1 <- b1) to the result
+// (CisAjtCkuAlv_element, CisAisCjtAku_element, CisAjtCkuAku_element,
+// mltplyHubbardCore.c:718-830).  For every occupation pattern p of the involved
+// orbitals with a non-zero result q, the target state x (pattern q) gathers
+// from x ^ (p ^ q); the sign splits into a constant (the involved bits) and the
+// parity of the target's other bits under the XOR of the two hops' between-masks
+// (those bits do not change).
+static void compile_two_hops(int a1, int b1, int a2, int b2, double vre, double vim, std::vector<GTerm> &out) {
+  const uint64_t B = (1ull << a1) | (1ull << b1) | (1ull << a2) | (1ull << b2);
+  int orb[4], no = 0;
+  for (int k = 0; k < 64; k++)
+    if ((B >> k) & 1) orb[no++] = k;
+  const uint64_t m1 = a1 != b1 ? between_mask(a1, b1) : 0, m2 = a2 != b2 ? between_mask(a2, b2) : 0;
+  for (int p = 0; p < (1 << no); p++) {
+    uint64_t s = 0;
+    for (int k = 0; k < no; k++)
+      if ((p >> k) & 1) s |= 1ull << orb[k];
+    uint64_t x = s;
+    int parity = 0;
+    if (!apply_hop(x, a2, b2, B, parity) || !apply_hop(x, a1, b1, B, parity)) continue;
+    GTerm g{};
+    g.kind = P_EQ;
+    g.tmask = B;
+    g.tval = x;
+    g.flip = s ^ x;
+    g.sgn = (m1 ^ m2) & ~B;
+    g.cre = parity ? -vre : vre;
+    g.cim = parity ? -vim : vim;
+    out.push_back(g);
+  }
+}
+
 int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
   e->model = d->model;
   e->nsite = d->nsite;
@@ -435,10 +492,24 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
   const int gc = (d->model == HPHI_MODEL_SPINGC);
   for (int t = 0; t < d->n_exchange; t++) {  // mltplySpinCore.c:273-345
     int i = d->exchange_ij[2 * t], j = d->exchange_ij[2 * t + 1];
-    if (d->model == HPHI_MODEL_HUBBARD) {
-      return hphi_set_error("Hubbard Exchange terms are not supported by this engine");
-    }
     if (i < 0 || j < 0 || i >= N || j >= N || i == j) { return hphi_set_error("bad Exchange sites"); }
+    if (d->model == HPHI_MODEL_HUBBARD) {
+      // exchange_element (mltplyHubbardCore.c:472-525): coefficient -J, the
+      // up/down pattern of sites i, j swapped, no sign; gather form: the
+      // target has the swapped pattern
+      const uint64_t iu = 1ull << (2 * i), id = 1ull << (2 * i + 1), ju = 1ull << (2 * j), jd = 1ull << (2 * j + 1);
+      const uint64_t pat[2] = {iu | jd, id | ju};
+      for (int q = 0; q < 2; q++) {
+        GTerm g{};
+        g.flip = iu | id | ju | jd;
+        g.tmask = g.flip;
+        g.tval = pat[q];
+        g.kind = P_EQ;
+        g.cre = -d->exchange_J[t];
+        e->h_terms.push_back(g);
+      }
+      continue;
+    }
     GTerm g{};
     g.flip = (1ull << i) | (1ull << j);
     g.tmask = g.flip;
@@ -497,6 +568,29 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
     }
     e->h_terms.push_back(g);
   }
+  if ((d->n_pairhop || d->n_interall || d->n_interall_diag) && d->model != HPHI_MODEL_HUBBARD)
+    return hphi_set_error("PairHop and InterAll are implemented for the canonical Hubbard model");
+  for (int t = 0; t < d->n_pairhop; t++) {
+    // pairhopp_element (mltplyHubbardCore.c:533-572): a doublon moves from
+    // site j to the empty site i, coefficient J, no sign
+    int i = d->pairhop_ij[2 * t], j = d->pairhop_ij[2 * t + 1];
+    if (i < 0 || j < 0 || i >= N || j >= N || i == j) return hphi_set_error("bad PairHop sites");
+    GTerm g{};
+    const uint64_t di = 3ull << (2 * i), dj = 3ull << (2 * j);
+    g.flip = di | dj;
+    g.tmask = g.flip;
+    g.tval = di;
+    g.kind = P_EQ;
+    g.cre = d->pairhop_J[t];
+    e->h_terms.push_back(g);
+  }
+  for (int t = 0; t < d->n_interall; t++) {
+    const int *q = d->interall_idx + 8 * t;
+    for (int u = 0; u < 8; u += 2)
+      if (q[u] < 0 || q[u] >= N || q[u + 1] < 0 || q[u + 1] > 1) return hphi_set_error("bad InterAll indices");
+    compile_two_hops(2 * q[0] + q[1], 2 * q[2] + q[3], 2 * q[4] + q[5], 2 * q[6] + q[7], d->interall_V[2 * t],
+                     d->interall_V[2 * t + 1], e->h_terms);
+  }
   classify_and_upload_terms(e);
 
   // ---- diagonal terms, in diagonalcalc order ----
@@ -553,6 +647,17 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
       x.kind = D_HUND_S;
       x.m1 = (1ull << s1) | (1ull << s2);
     }
+    e->h_dterms.push_back(x);
+  }
+  for (int t = 0; t < d->n_interall_diag; t++) {  // diagonalcalc.c:168-181
+    const int *q = d->interall_diag_idx + 4 * t;
+    if (q[0] < 0 || q[0] >= N || q[2] < 0 || q[2] >= N || q[1] < 0 || q[1] > 1 || q[3] < 0 || q[3] > 1)
+      return hphi_set_error("bad InterAll diagonal indices");
+    DTerm x{};
+    x.kind = D_NN;
+    x.m1 = 1ull << (2 * q[0] + q[1]);
+    x.m2 = 1ull << (2 * q[2] + q[3]);
+    x.V = d->interall_diag_V[t];
     e->h_dterms.push_back(x);
   }
 

@@ -45,7 +45,7 @@ __device__ __forceinline__ GTerm load_term(CTerm p) {
 }
 
 // diagonal term kinds (diagonalcalc.c set functions, per model)
-enum { D_INTRA = 0, D_CHEMI_H, D_CHEMI_S, D_INTER_H, D_INTER_S, D_HUND_H, D_HUND_S };
+enum { D_INTRA = 0, D_CHEMI_H, D_CHEMI_S, D_INTER_H, D_INTER_S, D_HUND_H, D_HUND_S, D_NN };
 
 struct DTerm {
   int kind, pad;

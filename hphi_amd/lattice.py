@@ -72,6 +72,11 @@ class ModelSpec:
     coulomb_intra: List[Tuple[int, float]] = field(default_factory=list)
     coulomb_inter: List[Tuple[int, int, float]] = field(default_factory=list)
     hund: List[Tuple[int, int, float]] = field(default_factory=list)
+    # Hubbard two-body terms: InterAll (i, s1, j, s2, k, s3, l, s4, re, im) for
+    # V c+_{i s1} c_{j s2} c+_{k s3} c_{l s4}; PairHop (i, j, J); for Hubbard,
+    # ``exchange`` holds the Hubbard Exchange (i, j, J)
+    interall: List[Tuple[int, int, int, int, int, int, int, int, float, float]] = field(default_factory=list)
+    pairhop: List[Tuple[int, int, float]] = field(default_factory=list)
     calc_type: int = CALC_LANCZOS
     params: Dict[str, float] = field(default_factory=dict)
 
@@ -360,11 +365,16 @@ def write_expert(spec: ModelSpec, directory: str, calc_type: Optional[int] = Non
         txt = _header("NCoulombIntra", len(spec.coulomb_intra), "================== CoulombIntra ================")
         txt += "".join(f"{i} {_f(u)}\n" for i, u in spec.coulomb_intra)
         put("CoulombIntra", "coulombintra.def", txt)
+    if spec.interall:
+        txt = _header("NInterAll", len(spec.interall), "========zInterAll======")
+        txt += "".join(" ".join(str(int(v)) for v in e[:8]) + f" {_f(e[8])} {_f(e[9])}\n" for e in spec.interall)
+        put("InterAll", "interall.def", txt)
     for kw, fname, lst in (("CoulombInter", "coulombinter.def", spec.coulomb_inter),
                            ("Hund", "hund.def", spec.hund),
                            ("Exchange", "exchange.def", spec.exchange),
                            ("Ising", "ising.def", spec.ising),
-                           ("PairLift", "pairlift.def", spec.pairlift)):
+                           ("PairLift", "pairlift.def", spec.pairlift),
+                           ("PairHop", "pairhop.def", spec.pairhop)):
         if lst:
             txt = _header(f"N{kw}", len(lst), f"========{kw}======")
             txt += "".join(f"{i} {j} {_f(v)}\n" for i, j, v in lst)
@@ -419,6 +429,55 @@ def processed_terms(spec: ModelSpec) -> Dict[str, np.ndarray]:
             raise ValueError("transfer without Hermitian partner")
     hund = list(spec.hund) + [(i, j, -J / 2.0) for i, j, J in spec.ising]
     inter = list(spec.coulomb_inter) + [(i, j, -J / 4.0) for i, j, J in spec.ising]
+    # PairHop: each entry and its reverse (readdef.c:1257-1267)
+    pairhop = []
+    for i, j, J in spec.pairhop:
+        pairhop += [(i, j, J), (j, i, J)]
+    # InterAll: duplicates summed in first-occurrence order (InputInterAllInfo,
+    # readdef.c:3054-3090), then split (GetDiagonalInterAll, readdef.c:2415-2530):
+    # n_i n_k -> diagonal; c+_i c_j c+_j c_i -> diagonal -V n_i n_j plus a
+    # chemical-potential entry; spin-exchange order rewritten with -V
+    ia_merged: List[List] = []
+    for e in spec.interall:
+        for m in ia_merged:
+            if m[:8] == list(e[:8]):
+                m[8] += complex(e[8], e[9])
+                break
+        else:
+            ia_merged.append(list(e[:8]) + [complex(e[8], e[9])])
+    ia_off, ia_offV, ia_diag, ia_diagV = [], [], [], []
+    for i, s1, j, s2, k, s3, l, s4, V in ia_merged:
+        if i == j and k == l and s1 == s2 and s3 == s4:
+            ia_diag.append((i, s1, k, s3))
+            ia_diagV.append(V.real)
+        elif i == l and j == k and s1 == s4 and s2 == s3:
+            ia_diag.append((i, s1, j, s2))
+            ia_diagV.append(-V.real)
+            chemi.append((i, s1, -V.real))
+        elif s1 == s2 and s3 == s4:
+            ia_off.append((i, s1, j, s2, k, s3, l, s4))
+            ia_offV.append(V)
+        elif s1 == s4 and s2 == s3:
+            ia_off.append((i, s1, l, s1, k, s2, j, s2))
+            ia_offV.append(-V)
+        else:
+            raise ValueError("InterAll term does not conserve Sz")
+    # Hermitian pairing: entry a is followed by its reverse with the conjugate
+    # value (CheckInterAllHermite, readdef.c:2257-2360)
+    paired, pairedV = [], []
+    for a, qa in enumerate(ia_off):
+        rev = (qa[6], qa[7], qa[4], qa[5], qa[2], qa[3], qa[0], qa[1])
+        for b, qb in enumerate(ia_off):
+            if tuple(qb) == rev and abs(ia_offV[a] - ia_offV[b].conjugate()) < 1e-12:
+                if a <= b:
+                    paired += [qa, qb]
+                    pairedV += [ia_offV[a], ia_offV[b]]
+                break
+        else:
+            raise ValueError("non-Hermitian InterAll")
+    if len(paired) != len(ia_off):
+        raise ValueError("non-Hermitian InterAll")
+    ia_off, ia_offV = paired, pairedV
 
     def arr2(lst, k):
         return np.array([x[:k] for x in lst], dtype=np.int32).reshape(-1, k)
@@ -438,4 +497,9 @@ def processed_terms(spec: ModelSpec) -> Dict[str, np.ndarray]:
         "intra_U": np.array([c[1] for c in spec.coulomb_intra], dtype=np.float64),
         "inter_ij": arr2(inter, 2), "inter_V": arrv(inter, 2),
         "hund_ij": arr2(hund, 2), "hund_J": arrv(hund, 2),
+        "interall_idx": np.array(ia_off, dtype=np.int32).reshape(-1, 8),
+        "interall_V": np.array([(v.real, v.imag) for v in ia_offV], dtype=np.float64).reshape(-1, 2),
+        "interall_diag_idx": np.array(ia_diag, dtype=np.int32).reshape(-1, 4),
+        "interall_diag_V": np.array(ia_diagV, dtype=np.float64),
+        "pairhop_ij": arr2(pairhop, 2), "pairhop_J": arrv(pairhop, 2),
     }

@@ -62,6 +62,17 @@ typedef struct {
   int n_intra;     const int *intra_site;   const double *intra_U;
   int n_inter;     const int *inter_ij;     const double *inter_V;
   int n_hund;      const int *hund_ij;      const double *hund_J;
+  /* Hubbard two-body terms as the reference's reader leaves them
+   * (InputInterAllInfo readdef.c:3054, GetDiagonalInterAll :2415-2530):
+   *   interall       InterAll_OffDiagonal[n][8] (i, s1, j, s2, k, s3, l, s4) for
+   *                  V c+_{i s1} c_{j s2} c+_{k s3} c_{l s4}, V as (re, im)
+   *   interall_diag  InterAll_Diagonal[n][4] (i, s1, k, s3) for V n_{i s1} n_{k s3}
+   *   pairhop        PairHopping[n][2] (both directions, readdef.c:1257-1267), J
+   * For Hubbard, exchange_ij/exchange_J hold the Hubbard ExchangeCoupling
+   * (mltplyHubbardCore.c:472-525). */
+  int n_interall;      const int *interall_idx;      const double *interall_V;
+  int n_interall_diag; const int *interall_diag_idx; const double *interall_diag_V;
+  int n_pairhop;       const int *pairhop_ij;        const double *pairhop_J;
 } hphi_model_desc;
 
 /* Modelling parameters read from modpara.def (src/readdef.c:555-650). */

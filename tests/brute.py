@@ -81,7 +81,36 @@ def dense(spec) -> np.ndarray:
             for i, U in spec.coulomb_intra:
                 if (x >> (2 * i)) & 1 and (x >> (2 * i + 1)) & 1:
                     H[k, k] += U
+            # two-body terms from their definitions
+            # (doc/en/source/filespecification/expertmode_en/{InterAll,PairHop,Exchange}_file_en.rst)
+            terms = []
+            for i, s1, j, s2, kk, s3, l, s4, re, im in getattr(spec, "interall", []):
+                terms.append((complex(re, im), [(1, 2 * i + s1), (0, 2 * j + s2), (1, 2 * kk + s3), (0, 2 * l + s4)]))
+            for i, j, J in getattr(spec, "pairhop", []):
+                terms.append((J, [(1, 2 * i), (0, 2 * j), (1, 2 * i + 1), (0, 2 * j + 1)]))
+                terms.append((J, [(1, 2 * j), (0, 2 * i), (1, 2 * j + 1), (0, 2 * i + 1)]))
+            for i, j, J in spec.exchange:
+                terms.append((J, [(1, 2 * i), (0, 2 * j), (1, 2 * j + 1), (0, 2 * i + 1)]))
+                terms.append((J, [(1, 2 * i + 1), (0, 2 * j + 1), (1, 2 * j), (0, 2 * i)]))
+            for V, ops in terms:
+                y, sg = apply_ops(x, ops)
+                if sg and y in index:
+                    H[index[y], k] += V * sg
     return H
+
+
+def apply_ops(x: int, ops):
+    """Product of fermion operators (dagger, orbital), written left to right and
+    applied right to left, with the sign (-1)^(occupied orbitals below) of each
+    operator (orbital order 2i + sigma).  Returns (state, sign) or (None, 0)."""
+    sign = 1
+    for dag, a in reversed(ops):
+        if ((x >> a) & 1) == dag:  # create on an occupied or annihilate an empty orbital
+            return None, 0
+        if bin(x & ((1 << a) - 1)).count("1") & 1:
+            sign = -sign
+        x ^= 1 << a
+    return x, sign
 
 
 def kron_xxz_gc(nsite, bonds, Jz, Jxy, hx) -> np.ndarray:

@@ -52,7 +52,13 @@ def _arr(ptr, n, w=1):
     return np.array([ptr[i] for i in range(n * w)]).reshape(n, w) if n else np.zeros((0, w))
 
 
+def _hubbard_two_body():
+    from tests.golden.cases import hubbard_two_body
+    return hubbard_two_body()
+
+
 @pytest.mark.parametrize("spec", [
+    _hubbard_two_body(),
     L.baseline_config(1), L.hubbard_chain(5), L.hubbard_square(4, 2, ncond=8),
     L.triangular_xxz_gc(4, 2), L.j1j2_square(4, 3, 1.0, 0.31, two_sz=2),
 ], ids=lambda s: s.name)
@@ -75,6 +81,14 @@ def test_def_reader_matches_python_mirror(capi, tmp_path, spec):
     assert np.array_equal(_arr(d.inter_V, d.n_inter)[:, 0] if d.n_inter else np.zeros(0), t["inter_V"])
     assert np.array_equal(_arr(d.hund_ij, d.n_hund, 2), t["hund_ij"].reshape(-1, 2))
     assert np.array_equal(_arr(d.hund_J, d.n_hund)[:, 0] if d.n_hund else np.zeros(0), t["hund_J"])
+    assert np.array_equal(_arr(d.interall_idx, d.n_interall, 8), t["interall_idx"].reshape(-1, 8))
+    assert np.array_equal(_arr(d.interall_V, d.n_interall, 2), t["interall_V"].reshape(-1, 2))
+    assert np.array_equal(_arr(d.interall_diag_idx, d.n_interall_diag, 4), t["interall_diag_idx"].reshape(-1, 4))
+    assert np.array_equal(_arr(d.interall_diag_V, d.n_interall_diag)[:, 0] if d.n_interall_diag else np.zeros(0),
+                          t["interall_diag_V"])
+    assert np.array_equal(_arr(d.pairhop_ij, d.n_pairhop, 2), t["pairhop_ij"].reshape(-1, 2))
+    assert np.array_equal(_arr(d.pairhop_J, d.n_pairhop)[:, 0] if d.n_pairhop else np.zeros(0), t["pairhop_J"])
+    assert np.array_equal(_arr(d.chemi_site, d.n_chemi)[:, 0] if d.n_chemi else np.zeros(0), t["chemi_site"])
     assert mp.exct == int(spec.params["exct"]) and mp.lanczos_max == int(spec.params["Lanczos_max"])
     capi.lib().hphi_model_desc_free(pd)
 
@@ -114,9 +128,17 @@ def test_def_reader_errors(capi, tmp_path, mutate, msg):
 def test_def_reader_rejects_unsupported_keyword(capi, tmp_path):
     nl = L.write_expert(L.heisenberg_chain(6), str(tmp_path))
     with open(nl, "a") as f:
-        f.write("InterAll interall.def\n")
+        f.write("Boost boost.def\n")
     rc, _, _ = _read(capi, nl)
-    assert rc == -1 and "InterAll" in capi.lib().hphi_last_error().decode()
+    assert rc == -1 and "Boost" in capi.lib().hphi_last_error().decode()
+
+
+def test_def_reader_rejects_interall_outside_hubbard(capi, tmp_path):
+    s = L.heisenberg_chain(6)
+    s.interall = [(0, 0, 0, 0, 1, 1, 1, 1, 0.5, 0.0)]
+    nl = L.write_expert(s, str(tmp_path))
+    rc, _, _ = _read(capi, nl)
+    assert rc == -1 and "Hubbard" in capi.lib().hphi_last_error().decode()
 
 
 def test_tridiagonal_eigensolver(capi):

@@ -1,0 +1,126 @@
+"""Hubbard two-body terms (InterAll, PairHop, Exchange).
+
+The C restatement in oracle/ does not cover these terms; their parity rests
+on fixtures written by the reference itself (tests/golden/gen_golden.py
+--two-body, from oracle/_ref) and on a dense Hamiltonian built from the
+operator definitions (tests/brute.py).  The CPU test pins the fixtures to the
+definitions; the GPU tests put the device engine against the fixtures."""
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from hphi_amd import lattice as L
+from tests import brute
+from tests.golden.cases import SEED, hubbard_two_body
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+HV_TOL = 1e-12
+
+
+def _rec():
+    g = json.load(open(os.path.join(GOLD, "golden.json")))
+    return g["cases_two_body"]["hubbard_two_body"]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_reference_two_body_fixture_matches_definitions(oracle_lib):
+    """The reference's H·v, prdct and diagonal for the two-body model equal the
+    dense Hamiltonian from the textbook operator definitions."""
+    spec = hubbard_two_body()
+    rec = _rec()
+    arr = np.load(os.path.join(GOLD, rec["arrays"]))
+    H = brute.dense(spec)
+    v = oracle_lib.fill_vector(SEED, rec["dim"])
+    hv = H @ v
+    assert np.max(np.abs(hv - arr["hv"])) <= HV_TOL * np.max(np.abs(hv))
+    assert np.allclose(np.real(np.diag(H)), arr["list_Diagonal"][1:], atol=1e-13, rtol=0)
+    assert abs(np.vdot(v, hv) - complex(*rec["prdct"])) <= HV_TOL * np.linalg.norm(v) * np.linalg.norm(hv)
+    # the basis the reference enumerated is the ascending canonical one
+    assert np.array_equal(arr["list_1"][1:], np.array(brute.basis(spec), dtype=np.uint64))
+
+
+@pytest.fixture(scope="module")
+def A():
+    from hphi_amd import capi
+    capi.lib()
+    return capi
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("source", ["spec", "def"])
+def test_two_body_tables_diag_hv_vs_reference(A, oracle_lib, tmp_path, source):
+    print(f"[gpu] two-body from {source}", flush=True)
+    spec = hubbard_two_body()
+    rec = _rec()
+    arr = np.load(os.path.join(GOLD, rec["arrays"]))
+    eng = A.Engine.from_spec(spec) if source == "spec" else A.Engine.from_def(L.write_expert(spec, str(tmp_path)))
+    l1, l21, l22 = eng.sz()
+    assert sha(l1) == rec["sha_list_1"] and sha(l21) == rec["sha_list_2_1"] and sha(l22) == rec["sha_list_2_2"]
+    diag = eng.diagonalcalc()
+    assert sha(diag) == rec["sha_list_Diagonal"]  # bit-exact, including the InterAll diagonal
+    v = oracle_lib.fill_vector(SEED, eng.dim)
+    hv, pr = eng.hv(v)
+    ref = arr["hv"]
+    assert np.max(np.abs(hv - ref)) <= HV_TOL * np.max(np.abs(ref))
+    assert abs(pr - complex(*rec["prdct"])) <= HV_TOL * np.linalg.norm(v) * np.linalg.norm(ref)
+    eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tuning", [dict(use_c2=0), dict(use_c2=1, lds21=1, tpb=256, wgs_per_cu=4),
+                                    dict(use_c2=1, lds21=0, tpb=512, wgs_per_cu=2)],
+                         ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
+def test_two_body_every_kernel(A, oracle_lib, tuning):
+    """Single-vector kernels under each tuning, and the block kernel for k = 4
+    and 16, against the reference's H·v."""
+    spec = hubbard_two_body()
+    rec = _rec()
+    ref_cols = []
+    eng = A.Engine.from_spec(spec)
+    eng.tune(**tuning)
+    v = oracle_lib.fill_vector(SEED, eng.dim)
+    hv, _ = eng.hv(v)
+    ref = np.load(os.path.join(GOLD, rec["arrays"]))["hv"]
+    assert np.max(np.abs(hv - ref)) <= HV_TOL * np.max(np.abs(ref))
+    H = brute.dense(spec)
+    for k in (4, 16):
+        V1 = A.DeviceArray(eng.dim * k)
+        eng.fill_random(V1, k, 30)
+        V0 = A.DeviceArray(eng.dim * k).zero()
+        eng.mltply_block_dev(k, V0, V1)
+        out = V0.download().reshape(eng.dim, k)
+        for c in range(k):
+            want = H @ oracle_lib.fill_vector(30 + c, eng.dim)
+            assert np.max(np.abs(out[:, c] - want)) <= HV_TOL * np.max(np.abs(want)), (k, c)
+        V0.free()
+        V1.free()
+    eng.close()
+
+
+@pytest.mark.gpu
+def test_two_body_lanczos_and_service(A):
+    spec = hubbard_two_body()
+    w = np.linalg.eigvalsh(brute.dense(spec))
+    eng = A.Engine.from_spec(spec)
+    r = eng.lanczos(n_eigen=1, max_iter=400, eps=1e-14, target=0, seed=3)
+    assert r.converged and abs(r.eigenvalues[0] - w[0]) <= 1e-10
+    eng.close()
+    srv = A.Server(hbm_budget=1 << 28, max_jobs=8, n_eigen=2, eps=1e-12, check_every=5)
+    srv.submit(spec, 4, 0.0)
+    res = []
+    for _ in range(10000):
+        a, wq = srv.tick()
+        res += srv.results()
+        if a == 0 and wq == 0:
+            break
+    srv.close()
+    distinct = [w[0]] + [x for x in w[1:] if x - w[0] > 1e-9][:1]
+    assert res and res[0].converged
+    assert np.allclose(list(res[0].eigenvalues[:2]), distinct, atol=1e-10)
