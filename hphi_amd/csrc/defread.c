@@ -14,9 +14,11 @@
  *                   explicit Hund / CoulombInter entries      src/readdef.c:1300-1327,2788-2792
  *   sector          Spin: Nup = (NLocSpn + 2Sz)/2; Hubbard: Nup = (NCond + 2Sz)/2
  *                                                             src/readdef.c:843-905
- * Supported subset: CalcModel 0 (Hubbard), 1 (Spin, S=1/2), 4 (SpinGC, S=1/2);
- * term files Trans, CoulombIntra, CoulombInter, Hund, Exchange, Ising, PairLift.
- * Other keywords (InterAll, PairHop, Boost, ...) are rejected with a message.
+ * Supported subset: CalcModel 0 (Hubbard), 1 (Spin, S=1/2), 3 (HubbardGC),
+ * 4 (SpinGC, S=1/2);
+ * term files Trans, CoulombIntra, CoulombInter, Hund, Exchange, Ising, PairLift,
+ * InterAll (every model), PairHop (Hubbard, HubbardGC).  Other keywords (Boost, ...)
+ * are rejected with a message.
  */
 #include <complex.h>
 #include <math.h>
@@ -303,9 +305,12 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
     }
   } else if (calc_model == HPHI_MODEL_SPINGC) {
     if (lkind != 1) return hphi_set_error("only S=1/2 spins are supported (LocSpin 1)");
+  } else if (calc_model == HPHI_MODEL_HUBBARDGC) {
+    if (lkind != 0) return hphi_set_error("Kondo sites are not supported");
   } else {
-    return hphi_set_error("CalcModel %d is not supported (0 Hubbard, 1 Spin, 4 SpinGC)", calc_model);
+    return hphi_set_error("CalcModel %d is not supported (0 Hubbard, 1 Spin, 3 HubbardGC, 4 SpinGC)", calc_model);
   }
+  const int hubbard_model = calc_model == HPHI_MODEL_HUBBARD || calc_model == HPHI_MODEL_HUBBARDGC;
 
   tlist tr = {0}, intra = {0}, inter = {0}, hund = {0}, ex = {0}, ising = {0}, pl = {0};
   tlist edt = {0}, chemi = {0}, ph = {0}, ia = {0}, iaoff = {0}, iadiag = {0}, ph2 = {0};
@@ -319,8 +324,8 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
   if (!rc && files[KW_PAIRLIFT][0] && read_terms(files[KW_PAIRLIFT], 2, 1, &pl, NULL)) rc = -1;
   if (!rc && files[KW_PAIRHOP][0] && read_terms(files[KW_PAIRHOP], 2, 1, &ph, NULL)) rc = -1;
   if (!rc && files[KW_INTERALL][0] && read_terms(files[KW_INTERALL], 8, 2, &ia, NULL)) rc = -1;
-  if (!rc && (ph.n || ia.n) && calc_model != HPHI_MODEL_HUBBARD)
-    rc = hphi_set_error("PairHop and InterAll are implemented for the canonical Hubbard model");
+  if (!rc && ph.n && !hubbard_model) rc = hphi_set_error("PairHop needs a Hubbard model");
+  const int spin_model = !hubbard_model;
 
   if (!rc) {
     /* merge duplicates, keeping first-occurrence order (readdef.c:1130-1150) */
@@ -381,7 +386,16 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
      * readdef.c:3054-3090), then split (GetDiagonalInterAll, readdef.c:2415-2530) */
     tlist m = {0};
     m.w = 8;
-    for (int a = 0; a < ia.n; a++) {
+    for (int a = 0; a < ia.n && !rc; a++) {
+      /* CheckInterAllCondition (readdef.c:2996-3033): sites in range; spin
+       * entries are (i s1 i s2 k s3 k s4) */
+      const int *q = ia.idx + 8 * a;
+      for (int u = 0; u < 8; u += 2)
+        if (q[u] < 0 || q[u] >= nsite) rc = hphi_set_error("InterAll site index out of range");
+      if (!rc && spin_model && (q[0] != q[2] || q[4] != q[6]))
+        rc = hphi_set_error("spin InterAll (%d %d %d %d %d %d %d %d) needs i == j and k == l", q[0], q[1], q[2], q[3],
+                            q[4], q[5], q[6], q[7]);
+      if (rc) break;
       int hit = -1;
       for (int b = 0; b < m.n && hit < 0; b++)
         if (!memcmp(m.idx + 8 * b, ia.idx + 8 * a, 8 * sizeof(int))) hit = b;
@@ -405,11 +419,13 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
         int d4[4] = {q[0], q[1], q[2], q[3]}, cs[2] = {q[0], q[1]};
         tl_push(&iadiag, d4, -vr, 0.0);
         tl_push(&chemi, cs, -vr, 0.0);
-      } else if (q[1] == q[3] && q[5] == q[7]) {
+      } else if (spin_model || (q[1] == q[3] && q[5] == q[7])) {
         tl_push(&iaoff, q, vr, vi);
       } else if (q[1] == q[7] && q[3] == q[5]) {
         int r8[8] = {q[0], q[1], q[6], q[1], q[4], q[3], q[2], q[3]};
         tl_push(&iaoff, r8, -vr, -vi);
+      } else if (calc_model == HPHI_MODEL_HUBBARDGC) {
+        tl_push(&iaoff, q, vr, vi); /* HubbardGC keeps Sz-changing entries */
       } else {
         rc = hphi_set_error("InterAll (%d %d %d %d %d %d %d %d) does not conserve Sz", q[0], q[1], q[2], q[3], q[4],
                             q[5], q[6], q[7]);
@@ -431,13 +447,36 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
           if (sqrt(dre * dre + dim * dim) < 1e-12) hit = b;
         }
       }
-      if (hit < 0) {
-        rc = hphi_set_error("InterAll (%d %d %d %d %d %d %d %d) has no Hermitian partner", A[0], A[1], A[2], A[3],
-                            A[4], A[5], A[6], A[7]);
-      } else if (a <= hit) {
-        tl_push(&pr, A, iaoff.val[2 * a], iaoff.val[2 * a + 1]);
-        tl_push(&pr, iaoff.idx + 8 * hit, iaoff.val[2 * hit], iaoff.val[2 * hit + 1]);
+      if (hit >= 0) {
+        if (a <= hit) {
+          tl_push(&pr, A, iaoff.val[2 * a], iaoff.val[2 * a + 1]);
+          tl_push(&pr, iaoff.idx + 8 * hit, iaoff.val[2 * hit], iaoff.val[2 * hit + 1]);
+        }
+        continue;
       }
+      /* spin: the partner may list the two site operators the other way
+       * round, (i s2 i s1 k s4 k s3); the pair is then entry a and its
+       * reverse, with the partner's value (readdef.c:2321-2352) */
+      if (spin_model && A[0] != A[4] && A[1] != A[5] && A[2] != A[6] && A[3] != A[7]) {
+        for (int b = 0; b < iaoff.n && hit < 0; b++) {
+          const int *B = iaoff.idx + 8 * b;
+          if (A[0] == B[2] && A[2] == B[0] && A[4] == B[6] && A[6] == B[4] && A[1] == B[3] && A[3] == B[1] &&
+              A[5] == B[7] && A[7] == B[5]) {
+            const double dre = iaoff.val[2 * a] - iaoff.val[2 * b], dim = iaoff.val[2 * a + 1] + iaoff.val[2 * b + 1];
+            if (sqrt(dre * dre + dim * dim) < 1e-12) hit = b;
+          }
+        }
+        if (hit >= 0) {
+          if (a <= hit) {
+            const int rev[8] = {A[6], A[7], A[4], A[5], A[2], A[3], A[0], A[1]};
+            tl_push(&pr, A, iaoff.val[2 * a], iaoff.val[2 * a + 1]);
+            tl_push(&pr, rev, iaoff.val[2 * hit], iaoff.val[2 * hit + 1]);
+          }
+          continue;
+        }
+      }
+      rc = hphi_set_error("InterAll (%d %d %d %d %d %d %d %d) has no Hermitian partner", A[0], A[1], A[2], A[3], A[4],
+                          A[5], A[6], A[7]);
     }
     if (!rc && pr.n != iaoff.n) rc = hphi_set_error("InterAll entries do not pair into Hermitian conjugates");
     tl_free(&iaoff);

@@ -253,6 +253,10 @@ __global__ void k_diag(uint64_t dim, const uint64_t *__restrict__ list1, int gc,
         uint64_t n1 = (x & T.m1) / T.m1, n2 = (x & T.m2) / T.m2;
         D += (double)(n1 * n2) * T.V;
       } break;
+      case D_SS: {  // SetDiagonalInterAll, spin: [bit_i == s1][bit_k == s3] V
+        uint64_t n1 = ((x & T.m1) / T.m1) ^ T.m3, n2 = ((x & T.m2) / T.m2) ^ T.m4;
+        D += (double)(n1 * n2) * T.V;
+      } break;
     }
   }
   diag[j * stride + col] = D;
@@ -360,7 +364,7 @@ static void classify_and_upload_terms(hphi_engine *e) {
   // groups: low (flip within the low half), high (within the high half), mixed
   std::vector<GTerm> low, high, mixed;
   for (const GTerm &t : e->h_terms) {
-    if (e->model == HPHI_MODEL_SPINGC) {
+    if (hphi_model_gc(e->model)) {
       low.push_back(t);
       continue;
     }
@@ -444,6 +448,36 @@ static void compile_two_hops(int a1, int b1, int a2, int b2, double vre, double 
   }
 }
 
+// V (|s1><s2|)_A (|s3><s4|)_B for spin-1/2 sites (bit value = sigma), applied
+// to the source as the reference does: site B first, then site A
+// (GC_general_int_spin and its four element functions,
+// mltplySpinCore.c:427-583; canonical general_int_spin, mltplySpin.c:1020-1051,
+// checks bit_A == s2, bit_B == s4 and flips both, which is the same operator for
+// the Sz-conserving entries the canonical model accepts).  For every pattern p
+// of the involved sites with a non-zero result q, the target (pattern q)
+// gathers from target ^ (p ^ q); no sign.
+static void compile_spin_pair(int A, int s1, int s2, int B, int s3, int s4, double vre, double vim,
+                              std::vector<GTerm> &out) {
+  const uint64_t mA = 1ull << A, mB = 1ull << B, M = mA | mB;
+  for (uint64_t p = 0; p < 4; p++) {
+    const uint64_t src = ((p & 1) ? mA : 0) | ((p & 2) ? mB : 0);
+    if ((p & 2) && A == B) continue;  // one site: patterns 0 and 1 only
+    uint64_t x = src;
+    if ((uint64_t)((x & mB) != 0) != (uint64_t)s4) continue;
+    x = s3 ? (x | mB) : (x & ~mB);
+    if ((uint64_t)((x & mA) != 0) != (uint64_t)s2) continue;
+    x = s1 ? (x | mA) : (x & ~mA);
+    GTerm g{};
+    g.kind = P_EQ;
+    g.tmask = M;
+    g.tval = x;
+    g.flip = src ^ x;
+    g.cre = vre;
+    g.cim = vim;
+    out.push_back(g);
+  }
+}
+
 int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
   e->model = d->model;
   e->nsite = d->nsite;
@@ -479,8 +513,16 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
       e->sdim = 0;
       e->size21 = e->size22 = 0;
       break;
+    case HPHI_MODEL_HUBBARDGC:  // check.c:120-124: idim_max = 4^Nsite
+      if (N > 20) { return hphi_set_error("HubbardGC Nsite too large"); }
+      e->nbits = 2 * N;
+      e->dim = 1ull << (2 * N);
+      e->sdim = 0;
+      e->size21 = e->size22 = 0;
+      break;
     default:
-      return hphi_set_error("unsupported CalcModel %d (supported: 0 Hubbard, 1 Spin, 4 SpinGC)", d->model);
+      return hphi_set_error("unsupported CalcModel %d (supported: 0 Hubbard, 1 Spin, 3 HubbardGC, 4 SpinGC)",
+                            d->model);
   }
   e->nlow = (e->nbits + 1) / 2;
   e->ihfbit = 1ull << e->nlow;
@@ -489,11 +531,12 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
   if (e->dim == 0) { return hphi_set_error("empty sector"); }
 
   // ---- off-diagonal terms (gather form) ----
-  const int gc = (d->model == HPHI_MODEL_SPINGC);
+  const int gc = hphi_model_gc(d->model);
+  const bool hubbard = hphi_model_hubbard(d->model);
   for (int t = 0; t < d->n_exchange; t++) {  // mltplySpinCore.c:273-345
     int i = d->exchange_ij[2 * t], j = d->exchange_ij[2 * t + 1];
     if (i < 0 || j < 0 || i >= N || j >= N || i == j) { return hphi_set_error("bad Exchange sites"); }
-    if (d->model == HPHI_MODEL_HUBBARD) {
+    if (hubbard) {
       // exchange_element (mltplyHubbardCore.c:472-525): coefficient -J, the
       // up/down pattern of sites i, j swapped, no sign; gather form: the
       // target has the swapped pattern
@@ -536,13 +579,13 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
       return hphi_set_error("bad transfer indices");
     }
     GTerm g{};
-    if (d->model == HPHI_MODEL_HUBBARD) {
+    if (hubbard) {
       // CisAjt (mltplyHubbardCore.c:305-347): c+_a c_b with a = 2 i1 + s1,
       // b = 2 i2 + s2; target has bit a set and bit b clear; sign from the
       // bits strictly between a and b (general_hopp_GetInfo, :34-69)
       int a = 2 * i1 + s1, b = 2 * i2 + s2;
       if (a == b) continue;  // diagonal: EDChemi, not a transfer
-      if (s1 != s2) {
+      if (s1 != s2 && !gc) {  // HubbardGC takes spin-flip transfers (GC_CisAjt, mltplyHubbardCore.c:354-392)
         return hphi_set_error("spin-flip transfer (%d,%d,%d,%d) breaks the canonical Hubbard sector", i1, s1, i2, s2);
       }
       int lo = std::min(a, b), hi = std::max(a, b);
@@ -568,8 +611,7 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
     }
     e->h_terms.push_back(g);
   }
-  if ((d->n_pairhop || d->n_interall || d->n_interall_diag) && d->model != HPHI_MODEL_HUBBARD)
-    return hphi_set_error("PairHop and InterAll are implemented for the canonical Hubbard model");
+  if (d->n_pairhop && !hubbard) return hphi_set_error("PairHop needs a Hubbard model");
   for (int t = 0; t < d->n_pairhop; t++) {
     // pairhopp_element (mltplyHubbardCore.c:533-572): a doublon moves from
     // site j to the empty site i, coefficient J, no sign
@@ -588,13 +630,24 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
     const int *q = d->interall_idx + 8 * t;
     for (int u = 0; u < 8; u += 2)
       if (q[u] < 0 || q[u] >= N || q[u + 1] < 0 || q[u + 1] > 1) return hphi_set_error("bad InterAll indices");
+    if (!hubbard) {
+      // spin entries are (i s1 i s2 k s3 k s4) (CheckFormatForSpinInt, readdef.c:2637-2650)
+      if (q[0] != q[2] || q[4] != q[6]) return hphi_set_error("spin InterAll needs i == j and k == l");
+      // canonical: only two opposite flips on distinct sites keep the sector;
+      // the reference's canonical kernel would index outside it otherwise
+      if (!gc && (q[0] == q[4] || q[1] == q[3] || q[5] == q[7] || q[3] == q[7]))
+        return hphi_set_error("InterAll (%d %d %d %d %d %d %d %d) does not conserve Sz in the canonical Spin model",
+                              q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]);
+      compile_spin_pair(q[0], q[1], q[3], q[4], q[5], q[7], d->interall_V[2 * t], d->interall_V[2 * t + 1],
+                        e->h_terms);
+      continue;
+    }
     compile_two_hops(2 * q[0] + q[1], 2 * q[2] + q[3], 2 * q[4] + q[5], 2 * q[6] + q[7], d->interall_V[2 * t],
                      d->interall_V[2 * t + 1], e->h_terms);
   }
   classify_and_upload_terms(e);
 
   // ---- diagonal terms, in diagonalcalc order ----
-  auto hubbard = d->model == HPHI_MODEL_HUBBARD;
   for (int t = 0; t < d->n_intra; t++) {
     if (!hubbard) break;
     int s = d->intra_site[t];
@@ -654,9 +707,17 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
     if (q[0] < 0 || q[0] >= N || q[2] < 0 || q[2] >= N || q[1] < 0 || q[1] > 1 || q[3] < 0 || q[3] > 1)
       return hphi_set_error("bad InterAll diagonal indices");
     DTerm x{};
-    x.kind = D_NN;
-    x.m1 = 1ull << (2 * q[0] + q[1]);
-    x.m2 = 1ull << (2 * q[2] + q[3]);
+    if (hubbard) {
+      x.kind = D_NN;
+      x.m1 = 1ull << (2 * q[0] + q[1]);
+      x.m2 = 1ull << (2 * q[2] + q[3]);
+    } else {  // child_Spin_CisAis / child_SpinGC_CisAis products (diagonalcalc.c:1380-1420)
+      x.kind = D_SS;
+      x.m1 = 1ull << q[0];
+      x.m2 = 1ull << q[2];
+      x.m3 = (uint64_t)(1 - q[1]);
+      x.m4 = (uint64_t)(1 - q[3]);
+    }
     x.V = d->interall_diag_V[t];
     e->h_dterms.push_back(x);
   }
@@ -848,7 +909,7 @@ int hphi_launch_diag(uint64_t dim, const uint64_t *list1, int gc, const DTerm *d
 
 int hphi_build_tables(hphi_engine *e) {
   if (e->tables_ready) return 0;
-  if (e->model == HPHI_MODEL_SPINGC) {
+  if (hphi_model_gc(e->model)) {
     e->tables_ready = true;
     return 0;
   }
@@ -872,7 +933,7 @@ int hphi_build_tables(hphi_engine *e) {
 extern "C" int hphi_sz(hphi_engine *e, uint64_t *list_1, uint64_t *list_2_1, uint64_t *list_2_2) {
   if (!e) return hphi_set_error("NULL engine");
   if (hphi_build_tables(e)) return -1;
-  if (e->model == HPHI_MODEL_SPINGC) return 0;
+  if (hphi_model_gc(e->model)) return 0;
   if (list_1) {
     list_1[0] = 0;
     HIPCHK(hipMemcpy(list_1 + 1, e->d_list1, sizeof(uint64_t) * e->dim, hipMemcpyDeviceToHost));
@@ -892,7 +953,7 @@ int hphi_build_diag(hphi_engine *e) {
   if (hphi_build_tables(e)) return -1;
   HIPCHK(hipMalloc(&e->d_diag, sizeof(double) * e->dim));
   e->bytes += sizeof(double) * e->dim;
-  if (hphi_launch_diag(e->dim, e->d_list1, e->model == HPHI_MODEL_SPINGC ? 1 : 0, e->d_dterms,
+  if (hphi_launch_diag(e->dim, e->d_list1, hphi_model_gc(e->model) ? 1 : 0, e->d_dterms,
                        (int)e->h_dterms.size(), e->d_diag, e->stream))
     return -1;
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -976,7 +1037,7 @@ int hphi_hv_launch_ex(hphi_engine *e, int k, double *v0, const double *v1, int o
     return 0;
   }
   int g = hphi_hv_grid(e, k);
-  bool canon = e->model != HPHI_MODEL_SPINGC;
+  bool canon = !hphi_model_gc(e->model);
 #define LAUNCH(KK)                                                                                   \
   if (canon) hipLaunchKernelGGL((k_hv<KK, true>), dim3(g), dim3(HV_TPB), 0, e->stream, A);          \
   else hipLaunchKernelGGL((k_hv<KK, false>), dim3(g), dim3(HV_TPB), 0, e->stream, A);

@@ -13,6 +13,11 @@
 #define HV_MAX_GRID 65536 // upper bound of H·v workgroups (sizes the partial-sum buffer)
 
 // predicate kinds of a gather term, evaluated on the TARGET state x
+// grand-canonical models index the vector by the state's bits (no tables)
+static inline bool hphi_model_gc(int m) { return m == HPHI_MODEL_SPINGC || m == HPHI_MODEL_HUBBARDGC; }
+// Hubbard models: site i's orbitals are bits 2i (up) and 2i+1 (down)
+static inline bool hphi_model_hubbard(int m) { return m == HPHI_MODEL_HUBBARD || m == HPHI_MODEL_HUBBARDGC; }
+
 enum { P_EQ = 0,    // (x & tmask) == tval            (hopping, transverse field)
        P_DIFF = 1,  // popcount(x & tmask) == 1       (exchange: the two spins differ)
        P_SAME = 2   // popcount(x & tmask) != 1       (pair lift: the two spins agree)
@@ -45,7 +50,7 @@ __device__ __forceinline__ GTerm load_term(CTerm p) {
 }
 
 // diagonal term kinds (diagonalcalc.c set functions, per model)
-enum { D_INTRA = 0, D_CHEMI_H, D_CHEMI_S, D_INTER_H, D_INTER_S, D_HUND_H, D_HUND_S, D_NN };
+enum { D_INTRA = 0, D_CHEMI_H, D_CHEMI_S, D_INTER_H, D_INTER_S, D_HUND_H, D_HUND_S, D_NN, D_SS };
 
 struct DTerm {
   int kind, pad;

@@ -320,7 +320,7 @@ int hphi_blk_cpl_ok(int k, int cpl) {
 
 int hphi_blk_ok(const hphi_engine *e) {
   if (e->nbits > 32) return 0;
-  if (e->model == HPHI_MODEL_SPINGC) return 1;
+  if (hphi_model_gc(e->model)) return 1;
   return e->c2_lds21;
 }
 
@@ -335,7 +335,7 @@ void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_pe
   }
   if (c <= 0 || !hphi_blk_cpl_ok(k, c)) c = k >= 8 ? 4 : k;
   *cpl = c;
-  const bool canon = e->model != HPHI_MODEL_SPINGC;
+  const bool canon = !hphi_model_gc(e->model);
   int t = e->bk_tpb;
   if (t != 256 && t != 1024) t = (canon && 2 * e->size21 > 32768) ? 1024 : 256;
   if (!canon) t = 256;
@@ -348,7 +348,7 @@ void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_pe
 int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid) {
   int cpl, tpb, wpc;
   hphi_blk_shape(e, k, &cpl, &tpb, &wpc);
-  const bool canon = e->model != HPHI_MODEL_SPINGC;
+  const bool canon = !hphi_model_gc(e->model);
   const uint64_t spt = (uint64_t)tpb / (uint64_t)(k / cpl);
   const uint64_t ntiles = (e->dim + spt - 1) / spt;
   uint64_t g = std::min<uint64_t>(ntiles, (uint64_t)wpc * std::max(e->num_cu, 1));

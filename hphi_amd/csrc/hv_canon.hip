@@ -253,7 +253,7 @@ int hphi_canon_config(hphi_engine *e) {
   // rank + 2 must fit uint16: the largest class has C(nlow, nlow/2) members (spin)
   bool small_rank = e->nlow <= 16;
   e->c2_lds21 = small_rank && need + 1024 <= (size_t)maxlds;
-  e->c2_ok = e->nbits <= 32 && e->model != HPHI_MODEL_SPINGC;
+  e->c2_ok = e->nbits <= 32 && !hphi_model_gc(e->model);
   return 0;
 }
 

@@ -29,7 +29,7 @@
 
 #define TQ_TPB 256
 #define TQ_GRID 1024
-#define TQ_NRED 4  // per column: |Hw|^2, obs, obs^2, |w_next|^2
+#define TQ_NRED 5  // per column: |Hw|^2, obs, obs^2, |w_next|^2, particle number
 
 __device__ __forceinline__ uint64_t tq_splitmix64(uint64_t x) {
   uint64_t z = x + 0x9E3779B97F4A7C15ull;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(TQ_TPB) void k_tpq_colnorm(const double2 *v, uint64
 
 __device__ __forceinline__ double tq_obs(int kind, uint64_t x, int nsite) {
   if (kind == 1) return (double)(2 * (int)__popcll(x) - nsite);  // SpinGC: 2 Sz(x)
-  if (kind == 2) return (double)(int)__popcll(x & (x >> 1) & 0x5555555555555555ull);  // Hubbard: doublons
+  if (kind >= 2) return (double)(int)__popcll(x & (x >> 1) & 0x5555555555555555ull);  // Hubbard: doublons
   return 0.0;
 }
 
@@ -95,16 +95,18 @@ __global__ __launch_bounds__(TQ_TPB) void k_tpq_fused(double2 *v1, const double2
                                                       const uint64_t *list1, double *partial) {
   const int c = threadIdx.x % k, jl = threadIdx.x / k, spb = TQ_TPB / k;
   const double sc_c = state[16 + c];  // 1/|w_s|
-  double r[TQ_NRED] = {0.0, 0.0, 0.0, 0.0};
+  double r[TQ_NRED] = {};
   for (uint64_t j = (uint64_t)blockIdx.x * spb + jl; j < dim; j += (uint64_t)gridDim.x * spb) {
     const uint64_t idx = j * sj + c * sc;
     const double2 w = v1[idx], h = v0[idx];
     r[0] += h.x * h.x + h.y * h.y;
     if (obs_kind) {
       const double ww = w.x * w.x + w.y * w.y;
-      const double o = tq_obs(obs_kind, list1 ? list1[j] : j, nsite);
+      const uint64_t x = list1 ? list1[j] : j;
+      const double o = tq_obs(obs_kind, x, nsite);
       r[1] += ww * o;
       r[2] += ww * o * o;
+      if (obs_kind == 3) r[4] += ww * (double)(int)__popcll(x);  // HubbardGC: electron number
     }
     const double2 wn = make_double2((lv * w.x - h.x * invNs) * sc_c, (lv * w.y - h.y * invNs) * sc_c);
     v1[idx] = wn;
@@ -140,12 +142,13 @@ __global__ void k_tpq_start(const double *partial, int nblk, int k, double *stat
 
 // one block: output row of step s, next scale
 __global__ void k_tpq_finalize(const double *partial, int nblk, int k, const double2 *prdct, double *state,
-                               double lv, double Ns, double num_const, double *out, int max_steps) {
+                               double lv, double Ns, double num_const, int num_measured, double *out,
+                               int max_steps) {
   const int c = threadIdx.x;
   const int s = (int)state[32];
   __syncthreads();
   if (c < k) {
-    double r[TQ_NRED] = {0.0, 0.0, 0.0, 0.0};
+    double r[TQ_NRED] = {};
     for (int b = 0; b < nblk; b++)
       for (int q = 0; q < TQ_NRED; q++) r[q] += partial[((uint64_t)b * k + c) * TQ_NRED + q];
     const double n = state[c];
@@ -158,8 +161,8 @@ __global__ void k_tpq_finalize(const double *partial, int nblk, int k, const dou
       o[2] = var;
       o[3] = sqrt(n);
       o[4] = r[1] / n;  // SpinGC: 2<Sz>; Hubbard: <doublon>
-      o[5] = r[2] / n;  // SpinGC: 4<Sz^2>; Hubbard: <doublon^2>
-      o[6] = num_const;
+      o[5] = r[2] / n;  // SpinGC: 4<Sz^2>; Hubbard, HubbardGC: <doublon^2>
+      o[6] = num_measured ? r[4] / n : num_const;  // HubbardGC: <N>
       o[7] = prdct[c].y / n;
     }
     state[c] = r[3];
@@ -229,11 +232,11 @@ static int tpq_enqueue_step(hphi_engine *e, const TpqPlan &P) {
   }
   hipLaunchKernelGGL(k_tpq_fused, dim3(P.g), dim3(TQ_TPB), 0, e->stream, (double2 *)P.V1, (const double2 *)P.V0,
                      dim, P.k, P.sj, P.sc, (const double *)P.state, P.lv, 1.0 / P.Ns, P.obs_kind, e->nsite,
-                     (e->model == HPHI_MODEL_SPINGC) ? (const uint64_t *)nullptr : (const uint64_t *)e->d_list1,
+                     hphi_model_gc(e->model) ? (const uint64_t *)nullptr : (const uint64_t *)e->d_list1,
                      P.part);
   hipLaunchKernelGGL(k_tpq_finalize, dim3(1), dim3(64), 0, e->stream, (const double *)P.part, P.g, P.k,
                      (const double2 *)(P.layout == 1 ? e->d_prdct : e->d_coef), P.state, P.lv, P.Ns, P.num_const,
-                     P.out, P.steps);
+                     P.obs_kind == 3 ? 1 : 0, P.out, P.steps);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -259,7 +262,7 @@ extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_ho
     if (hphi_diag_maxabs(e, &m)) return -1;
     P.lv = 3.0 * m / P.Ns;  // l = 3 max_j |list_Diagonal[j]| / Ns
   }
-  P.obs_kind = (e->model == HPHI_MODEL_SPINGC) ? 1 : (e->model == HPHI_MODEL_HUBBARD ? 2 : 0);
+  P.obs_kind = e->model == HPHI_MODEL_SPINGC ? 1 : e->model == HPHI_MODEL_HUBBARD ? 2 : e->model == HPHI_MODEL_HUBBARDGC ? 3 : 0;
   P.num_const = (e->model == HPHI_MODEL_HUBBARD) ? (double)(e->nup + e->ndown) : (double)e->nsite;
   P.g = tq_grid(dim, k);
   size_t vbytes = sizeof(double2) * dim * k;

@@ -53,7 +53,7 @@ CALC_TPQ = 1
 CALC_FULLDIAG = 2
 CALC_CG = 3
 
-_MODEL_NAMES = {HUBBARD: "Hubbard", SPIN: "Spin", SPIN_GC: "SpinGC"}
+_MODEL_NAMES = {HUBBARD: "Hubbard", SPIN: "Spin", HUBBARD_GC: "HubbardGC", SPIN_GC: "SpinGC"}
 
 
 @dataclass
@@ -107,11 +107,13 @@ class ModelSpec:
             return math.comb(self.nsite, self.nup) * math.comb(self.nsite, self.ndown)
         if self.model == SPIN_GC:
             return 1 << self.nsite
+        if self.model == HUBBARD_GC:
+            return 1 << (2 * self.nsite)
         raise ValueError("unsupported model")
 
     @property
     def nbits(self) -> int:
-        return 2 * self.nsite if self.model == HUBBARD else self.nsite
+        return 2 * self.nsite if self.model in (HUBBARD, HUBBARD_GC) else self.nsite
 
 
 # ---------------------------------------------------------------------------
@@ -207,8 +209,13 @@ def spin_model(name: str, nsite: int, bonds_J: Sequence[Tuple[int, int, float]],
 
 
 def hubbard_model(name: str, nsite: int, bonds_t: Sequence[Tuple[int, int, float]],
-                  U: float, ncond: int, two_sz: int = 0) -> ModelSpec:
-    spec = ModelSpec(name=name, model=HUBBARD, nsite=nsite, two_sz=two_sz, ncond=ncond)
+                  U: float, ncond: Optional[int], two_sz: int = 0) -> ModelSpec:
+    """Hubbard model from (i, j, t) bonds; ``ncond`` None selects the
+    grand-canonical HubbardGC model (every electron number and Sz)."""
+    if ncond is None:
+        spec = ModelSpec(name=name, model=HUBBARD_GC, nsite=nsite)
+    else:
+        spec = ModelSpec(name=name, model=HUBBARD, nsite=nsite, two_sz=two_sz, ncond=ncond)
     for i, j, t in bonds_t:
         for s in (0, 1):
             spec.trans.append((i, s, j, s, t, 0.0))
@@ -253,6 +260,23 @@ def hubbard_chain(L: int, t: float = 1.0, U: float = 4.0, ncond: Optional[int] =
         two_sz = ncond % 2
     return hubbard_model(f"hubbard_chain{L}", L, [(i, j, t) for i, j in chain_bonds(L)],
                          U, ncond, two_sz)
+
+
+def hubbard_chain_gc(L: int, t: float = 1.0, U: float = 4.0, mu: float = 0.0,
+                     flip: complex = 0.0) -> ModelSpec:
+    """HubbardGC ring: hopping t, on-site U, chemical potential mu (a diagonal
+    transfer -mu n), and an optional spin-flip hopping c+_{i up} c_{i+1 down}
+    with amplitude ``flip`` (plus its Hermitian conjugate)."""
+    spec = hubbard_model(f"hubbard_gc_chain{L}", L, [(i, j, t) for i, j in chain_bonds(L)], U, None)
+    for i in range(L):
+        for sp in (0, 1):
+            if mu != 0.0:
+                spec.trans.append((i, sp, i, sp, mu, 0.0))
+    if flip != 0.0:
+        for i, j in chain_bonds(L):
+            spec.trans.append((i, 0, j, 1, flip.real, flip.imag))
+            spec.trans.append((j, 1, i, 0, flip.real, -flip.imag))
+    return spec
 
 
 def triangular_xxz_gc(Lx: int = 13, Ly: int = 2, Jz: float = 1.0, Jxy: float = 0.8,
@@ -430,6 +454,8 @@ def processed_terms(spec: ModelSpec) -> Dict[str, np.ndarray]:
     hund = list(spec.hund) + [(i, j, -J / 2.0) for i, j, J in spec.ising]
     inter = list(spec.coulomb_inter) + [(i, j, -J / 4.0) for i, j, J in spec.ising]
     # PairHop: each entry and its reverse (readdef.c:1257-1267)
+    if spec.pairhop and spec.model not in (HUBBARD, HUBBARD_GC):
+        raise ValueError("PairHop needs a Hubbard model")
     pairhop = []
     for i, j, J in spec.pairhop:
         pairhop += [(i, j, J), (j, i, J)]
@@ -437,8 +463,14 @@ def processed_terms(spec: ModelSpec) -> Dict[str, np.ndarray]:
     # readdef.c:3054-3090), then split (GetDiagonalInterAll, readdef.c:2415-2530):
     # n_i n_k -> diagonal; c+_i c_j c+_j c_i -> diagonal -V n_i n_j plus a
     # chemical-potential entry; spin-exchange order rewritten with -V
+    spin_model = spec.model not in (HUBBARD, HUBBARD_GC)
     ia_merged: List[List] = []
     for e in spec.interall:
+        # CheckInterAllCondition (readdef.c:2996-3033)
+        if any(not 0 <= e[u] < spec.nsite for u in (0, 2, 4, 6)):
+            raise ValueError("InterAll site index out of range")
+        if spin_model and (e[0] != e[2] or e[4] != e[6]):
+            raise ValueError("spin InterAll needs i == j and k == l")
         for m in ia_merged:
             if m[:8] == list(e[:8]):
                 m[8] += complex(e[8], e[9])
@@ -454,27 +486,41 @@ def processed_terms(spec: ModelSpec) -> Dict[str, np.ndarray]:
             ia_diag.append((i, s1, j, s2))
             ia_diagV.append(-V.real)
             chemi.append((i, s1, -V.real))
-        elif s1 == s2 and s3 == s4:
+        elif spin_model or (s1 == s2 and s3 == s4):
             ia_off.append((i, s1, j, s2, k, s3, l, s4))
             ia_offV.append(V)
         elif s1 == s4 and s2 == s3:
             ia_off.append((i, s1, l, s1, k, s2, j, s2))
             ia_offV.append(-V)
+        elif spec.model == HUBBARD_GC:
+            ia_off.append((i, s1, j, s2, k, s3, l, s4))
+            ia_offV.append(V)
         else:
             raise ValueError("InterAll term does not conserve Sz")
     # Hermitian pairing: entry a is followed by its reverse with the conjugate
     # value (CheckInterAllHermite, readdef.c:2257-2360)
+    # (spin: the partner may also list the two site operators the other way
+    # round; the pair is then entry a and its reverse, readdef.c:2321-2352)
     paired, pairedV = [], []
     for a, qa in enumerate(ia_off):
         rev = (qa[6], qa[7], qa[4], qa[5], qa[2], qa[3], qa[0], qa[1])
-        for b, qb in enumerate(ia_off):
-            if tuple(qb) == rev and abs(ia_offV[a] - ia_offV[b].conjugate()) < 1e-12:
-                if a <= b:
-                    paired += [qa, qb]
-                    pairedV += [ia_offV[a], ia_offV[b]]
-                break
-        else:
-            raise ValueError("non-Hermitian InterAll")
+        swp = (qa[2], qa[3], qa[0], qa[1], qa[6], qa[7], qa[4], qa[5])
+        hit = next((b for b, qb in enumerate(ia_off)
+                    if tuple(qb) == rev and abs(ia_offV[a] - ia_offV[b].conjugate()) < 1e-12), None)
+        if hit is not None:
+            if a <= hit:
+                paired += [qa, ia_off[hit]]
+                pairedV += [ia_offV[a], ia_offV[hit]]
+            continue
+        if spin_model and qa[0] != qa[4] and qa[1] != qa[5] and qa[2] != qa[6] and qa[3] != qa[7]:
+            hit = next((b for b, qb in enumerate(ia_off)
+                        if tuple(qb) == swp and abs(ia_offV[a] - ia_offV[b].conjugate()) < 1e-12), None)
+            if hit is not None:
+                if a <= hit:
+                    paired += [tuple(qa), rev]
+                    pairedV += [ia_offV[a], ia_offV[hit]]
+                continue
+        raise ValueError("non-Hermitian InterAll")
     if len(paired) != len(ia_off):
         raise ValueError("non-Hermitian InterAll")
     ia_off, ia_offV = paired, pairedV

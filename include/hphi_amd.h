@@ -37,6 +37,7 @@ extern "C" {
 /* calcmod.def CalcModel values (src/include/DefCommon.h:22-37) */
 #define HPHI_MODEL_HUBBARD 0
 #define HPHI_MODEL_SPIN 1
+#define HPHI_MODEL_HUBBARDGC 3
 #define HPHI_MODEL_SPINGC 4
 
 typedef struct hphi_engine hphi_engine;
@@ -195,8 +196,8 @@ int hphi_lanczos_converged(const double *prev, int nprev, const double *cur, int
  * use_graph captures one step in a hipGraph and replays it.
  * out[(s*k + c)*8 + q], s = 0..steps-1 (the reference's step_i), q:
  *   0 inv_temp, 1 energy, 2 <H^2>, 3 norm (global_norm; step 0: global_1st_norm),
- *   4 SpinGC 2<Sz> / Hubbard <doublon>, 5 SpinGC 4<Sz^2> / Hubbard <doublon^2>,
- *   6 particle number, 7 Im <H> (rounding check). */
+ *   4 SpinGC 2<Sz> / Hubbard, HubbardGC <doublon>, 5 SpinGC 4<Sz^2> / Hubbard,
+ *   HubbardGC <doublon^2>, 6 particle number (HubbardGC: <N>), 7 Im <H> (rounding check). */
 typedef struct {
   int k, steps, init_kind, layout, use_graph;
   double large_value;

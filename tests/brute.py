@@ -9,7 +9,9 @@ brute force, and every operator is applied from its definition
 * Trans t:     -t c+_{i s1} c_{j s2}; fermion order = bit index 2i+s, so
                c+_a c_b |x> = (-1)^(#occupied strictly between a, b) |x'>
 * spin Trans:  on one site, c+_{i s1} c_{i s2} flips the bit s2 -> s1
-* CoulombIntra U: U n_up n_down
+* CoulombIntra U: U n_up n_down; CoulombInter V n_i n_j;
+  Hund -J (n_iup n_jup + n_idn n_jdn)
+* spin InterAll: (|s1><s2|)_i (|s3><s4|)_k, the right factor applied first
 Used only for small sizes.
 """
 
@@ -25,7 +27,7 @@ from hphi_amd import lattice as L
 
 def basis(spec) -> List[int]:
     n = spec.nbits
-    if spec.model == L.SPIN_GC:
+    if spec.model in (L.SPIN_GC, L.HUBBARD_GC):
         return list(range(1 << n))
     out = []
     for x in range(1 << n):
@@ -54,6 +56,18 @@ def dense(spec) -> np.ndarray:
                 si = 0.5 if (x >> i) & 1 else -0.5
                 sj = 0.5 if (x >> j) & 1 else -0.5
                 H[k, k] += J * si * sj
+            # InterAll: c+_{i s1} c_{i s2} on a spin-1/2 site is |s1><s2| (the
+            # site's state is its bit); the right factor acts first
+            for i, s1, j, s2, kk, s3, l, s4, re, im in getattr(spec, "interall", []):
+                y = x
+                if ((y >> kk) & 1) != s4:
+                    continue
+                y = (y | (1 << kk)) if s3 else (y & ~(1 << kk))
+                if ((y >> i) & 1) != s2:
+                    continue
+                y = (y | (1 << i)) if s1 else (y & ~(1 << i))
+                if y in index:
+                    H[index[y], k] += complex(re, im)
             for i, s1, j, s2, re, im in spec.trans:
                 t = complex(re, im)
                 if i == j and s1 != s2:
@@ -81,6 +95,11 @@ def dense(spec) -> np.ndarray:
             for i, U in spec.coulomb_intra:
                 if (x >> (2 * i)) & 1 and (x >> (2 * i + 1)) & 1:
                     H[k, k] += U
+            n = [((x >> (2 * i)) & 1, (x >> (2 * i + 1)) & 1) for i in range(spec.nsite)]
+            for i, j, V in spec.coulomb_inter:  # V n_i n_j
+                H[k, k] += V * sum(n[i]) * sum(n[j])
+            for i, j, J in spec.hund:  # -J (n_iup n_jup + n_idn n_jdn)
+                H[k, k] += -J * (n[i][0] * n[j][0] + n[i][1] * n[j][1])
             # two-body terms from their definitions
             # (doc/en/source/filespecification/expertmode_en/{InterAll,PairHop,Exchange}_file_en.rst)
             terms = []

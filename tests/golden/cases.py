@@ -65,3 +65,74 @@ def hubbard_two_body():
     h.exchange = [(1, 2, 0.4), (3, 5, -0.15)]
     h.name = "hubbard_two_body"
     return h
+
+
+def spin_interall():
+    """Heisenberg ring of 8 sites, 2Sz=0, with spin InterAll entries (bit value =
+    sigma): complex S+S- flips in the low half, the high half and across the
+    halves; a pair whose partner lists the two site operators the other way
+    round; diagonal products with a duplicate to merge; a one-site product
+    split into a diagonal and a chemical-potential entry."""
+    s = L.heisenberg_chain(8, 1.0, 0.0)
+    s.interall = [
+        (0, 0, 0, 1, 2, 1, 2, 0, 0.3, 0.2),      # |0><1|_0 |1><0|_2
+        (2, 0, 2, 1, 0, 1, 0, 0, 0.3, -0.2),     # its reverse, conjugate value
+        (1, 1, 1, 0, 4, 0, 4, 1, -0.25, 0.1),    # across the halves
+        (1, 0, 1, 1, 4, 1, 4, 0, -0.25, -0.1),   # partner with the operators swapped
+        (5, 0, 5, 1, 7, 1, 7, 0, 0.15, 0.0),     # high half only
+        (7, 0, 7, 1, 5, 1, 5, 0, 0.15, 0.0),
+        (3, 1, 3, 1, 5, 1, 5, 1, 0.4, 0.0),      # diagonal
+        (3, 0, 3, 0, 5, 1, 5, 1, -0.2, 0.0),
+        (3, 1, 3, 1, 5, 1, 5, 1, 0.05, 0.0),     # duplicate: merged to 0.45
+        (6, 1, 6, 0, 6, 0, 6, 1, 0.25, 0.0),     # |1><0|0><1| on site 6
+    ]
+    s.name = "spin_interall"
+    return s
+
+
+def spingc_interall():
+    """Triangular 4x2 XXZ model with a transverse field (SpinGC) and InterAll
+    entries of every grand-canonical form: projector times flip in both
+    orders, two raising flips, two operators on one site, a swapped-order
+    partner, and a diagonal product."""
+    s = L.triangular_xxz_gc(4, 2, 1.0, 0.8, 0.1)
+    s.interall = [
+        (0, 1, 0, 1, 3, 1, 3, 0, 0.2, 0.1),      # [bit_0 = 1] |1><0|_3
+        (3, 0, 3, 1, 0, 1, 0, 1, 0.2, -0.1),
+        (1, 1, 1, 0, 4, 1, 4, 1, -0.3, 0.05),    # |1><0|_1 [bit_4 = 1]
+        (4, 1, 4, 1, 1, 0, 1, 1, -0.3, -0.05),
+        (2, 1, 2, 0, 6, 1, 6, 0, 0.15, 0.05),    # two raising flips
+        (6, 0, 6, 1, 2, 0, 2, 1, 0.15, -0.05),
+        (5, 1, 5, 1, 5, 1, 5, 0, 0.12, 0.0),     # one site: [1] after |1><0|
+        (5, 0, 5, 1, 5, 1, 5, 1, 0.12, 0.0),
+        (1, 0, 1, 1, 7, 1, 7, 0, 0.1, 0.2),
+        (1, 1, 1, 0, 7, 0, 7, 1, 0.1, -0.2),     # partner with the operators swapped
+        (0, 0, 0, 0, 7, 1, 7, 1, 0.33, 0.0),     # diagonal
+    ]
+    s.name = "spingc_interall"
+    return s
+
+
+def hubbard_gc():
+    """HubbardGC ring of 5 sites (every electron number and Sz, dim 4^5):
+    hopping, U, a chemical potential, a complex spin-flip hopping, CoulombInter
+    and Hund, an Sz-changing InterAll pair, PairHop and Exchange."""
+    h = L.hubbard_chain_gc(5, 1.0, 3.0, mu=0.4, flip=complex(0.2, 0.1))
+    h.coulomb_inter = [(0, 2, 0.3), (1, 4, -0.2)]
+    h.hund = [(2, 3, 0.25)]
+    v = complex(0.15, -0.05)
+    h.interall = [
+        (0, 0, 1, 1, 2, 0, 3, 0, v.real, v.imag),       # c+_0up c_1dn c+_2up c_3up
+        (3, 0, 2, 0, 1, 1, 0, 0, v.real, -v.imag),
+        (4, 1, 4, 1, 0, 0, 1, 0, 0.2, 0.0),             # n_4dn c+_0up c_1up
+        (1, 0, 0, 0, 4, 1, 4, 1, 0.2, 0.0),
+        (2, 0, 2, 0, 4, 1, 4, 1, -0.35, 0.0),           # diagonal
+    ]
+    h.pairhop = [(1, 3, 0.25)]
+    h.exchange = [(0, 4, -0.3)]
+    h.name = "hubbard_gc"
+    return h
+
+
+def two_body_cases():
+    return {c.name: c for c in (hubbard_two_body(), spin_interall(), spingc_interall(), hubbard_gc())}

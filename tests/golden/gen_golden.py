@@ -12,9 +12,10 @@ Full-size workloads (pass --large) store checksums only: S1 = sum v0,
 S2 = sum |v0|^2, S3 = sum ((j mod 97)+1) v0[j] and 64 sampled entries
 (the harness's `sums` mode).
 
-Two-body Hubbard terms (InterAll, PairHop, Exchange; pass --two-body) are
-stored under "cases_two_body": the C restatement in oracle/ does not cover
-them, so these fixtures come from the reference alone.
+Two-body terms (Hubbard InterAll, PairHop, Exchange; spin and SpinGC
+InterAll; pass --two-body) are stored under "cases_two_body": the C
+restatement in oracle/ does not cover them, so these fixtures come from the
+reference alone.
 
 usage: python tests/golden/gen_golden.py [--large | --two-body]
 """
@@ -36,7 +37,7 @@ sys.path.insert(0, HERE)
 
 from hphi_amd import lattice as L  # noqa: E402
 from oracle import oracle as O  # noqa: E402
-from cases import SEED, golden_cases, hubbard_two_body  # noqa: E402
+from cases import SEED, golden_cases, two_body_cases  # noqa: E402
 
 FULL_MAX = 5000
 
@@ -109,9 +110,9 @@ def main():
     gold = json.load(open(out_path)) if os.path.exists(out_path) else {}
     with tempfile.TemporaryDirectory() as work:
         if "--two-body" in sys.argv:
-            spec = hubbard_two_body()
-            gold.setdefault("cases_two_body", {})[spec.name] = run_case(spec.name, spec, work)
-            print(spec.name, "done", flush=True)
+            for name, spec in two_body_cases().items():
+                gold.setdefault("cases_two_body", {})[name] = run_case(name, spec, work)
+                print(name, "done", flush=True)
         elif "--large" in sys.argv:
             for n in [int(a) for a in sys.argv[sys.argv.index("--large") + 1:] if a.isdigit()] or [4, 2, 3]:
                 gold.setdefault("large", {})[f"cfg{n}"] = run_large(n, work)

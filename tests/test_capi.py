@@ -52,13 +52,13 @@ def _arr(ptr, n, w=1):
     return np.array([ptr[i] for i in range(n * w)]).reshape(n, w) if n else np.zeros((0, w))
 
 
-def _hubbard_two_body():
-    from tests.golden.cases import hubbard_two_body
-    return hubbard_two_body()
+def _two_body():
+    from tests.golden.cases import two_body_cases
+    return list(two_body_cases().values())
 
 
 @pytest.mark.parametrize("spec", [
-    _hubbard_two_body(),
+    *_two_body(),
     L.baseline_config(1), L.hubbard_chain(5), L.hubbard_square(4, 2, ncond=8),
     L.triangular_xxz_gc(4, 2), L.j1j2_square(4, 3, 1.0, 0.31, two_sz=2),
 ], ids=lambda s: s.name)
@@ -133,12 +133,26 @@ def test_def_reader_rejects_unsupported_keyword(capi, tmp_path):
     assert rc == -1 and "Boost" in capi.lib().hphi_last_error().decode()
 
 
-def test_def_reader_rejects_interall_outside_hubbard(capi, tmp_path):
+@pytest.mark.parametrize("what", ["spin_format", "pairhop_spin", "no_partner"])
+def test_def_reader_rejects_bad_two_body_input(capi, tmp_path, what):
+    """Spin InterAll entries must read (i s1 i s2 k s3 k s4)
+    (CheckFormatForSpinInt, readdef.c:2637-2650); PairHop needs a Hubbard
+    model; an off-diagonal entry needs its Hermitian partner."""
     s = L.heisenberg_chain(6)
-    s.interall = [(0, 0, 0, 0, 1, 1, 1, 1, 0.5, 0.0)]
+    if what == "spin_format":
+        s.interall = [(0, 0, 1, 0, 2, 1, 2, 1, 0.5, 0.0)]
+        msg = "i == j"
+    elif what == "pairhop_spin":
+        s.pairhop = [(0, 1, 0.5)]
+        msg = "Hubbard"
+    else:
+        s.interall = [(0, 0, 0, 1, 2, 1, 2, 0, 0.5, 0.1)]
+        msg = "Hermitian"
     nl = L.write_expert(s, str(tmp_path))
     rc, _, _ = _read(capi, nl)
-    assert rc == -1 and "Hubbard" in capi.lib().hphi_last_error().decode()
+    assert rc == -1 and msg in capi.lib().hphi_last_error().decode()
+    with pytest.raises(ValueError):
+        L.processed_terms(s)
 
 
 def test_tridiagonal_eigensolver(capi):

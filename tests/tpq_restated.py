@@ -6,14 +6,19 @@
 import numpy as np
 
 
-def tpq_restated(hv, v_init, steps, lv, Ns, sz_of=None):
-    """Rows (inv_temp, energy, <H^2>, norm, 2<Sz>, 4<Sz^2>) of one sample."""
+def tpq_restated(hv, v_init, steps, lv, Ns, sz_of=None, num_of=None):
+    """Rows (inv_temp, energy, <H^2>, norm, 2<Sz>, 4<Sz^2>) of one sample; with
+    ``num_of`` (the electron number of each state) a seventh column <N>."""
     rows = []
     first_norm = np.linalg.norm(v_init)
     v1 = v_init / first_norm
     h = hv(v1)
-    obs = lambda v: ((np.sum(np.abs(v) ** 2 * sz_of), np.sum(np.abs(v) ** 2 * sz_of ** 2))
-                     if sz_of is not None else (0.0, 0.0))
+
+    def obs(v):
+        w = np.abs(v) ** 2
+        o = (np.sum(w * sz_of), np.sum(w * sz_of ** 2)) if sz_of is not None else (0.0, 0.0)
+        return o + ((np.sum(w * num_of),) if num_of is not None else ())
+
     o = obs(v1)
     rows.append((0.0, np.vdot(v1, h).real, np.vdot(h, h).real, first_norm, *o))
     for s in range(1, steps):
