@@ -33,6 +33,8 @@ EXPORTED = [
     "hphi_lanczos_good_eigenvalues", "hphi_lanczos_converged",
     "hphi_server_create", "hphi_server_destroy", "hphi_server_submit", "hphi_server_tick",
     "hphi_server_results", "hphi_server_now", "hphi_server_stats", "hphi_lobpcg", "hphi_herm_eig",
+    "hphi_lanczos_eigenvector", "hphi_vec_lanczos_update_acc", "hphi_expec_energy_flct", "hphi_expec_cisajs",
+    "hphi_expec_cisajscktalt",
 ]
 
 
@@ -59,6 +61,13 @@ class ModPara(C.Structure):
 class LanczosParams(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("n_eigen", C.c_int), ("eps", C.c_double),
                 ("target", C.c_int), ("seed", C.c_uint64), ("check_every", C.c_int), ("mode", C.c_int)]
+
+
+class Phys(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("energy", "var", "doublon", "doublon2", "num", "num2", "sz", "sz2")]
+
+    def as_dict(self) -> Dict[str, float]:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class LanczosResult(C.Structure):
@@ -154,6 +163,12 @@ def lib():
         L.hphi_server_now.argtypes = [_vp]
         L.hphi_server_now.restype = C.c_double
         L.hphi_server_stats.argtypes = [_vp, _f64p]
+        L.hphi_lanczos_eigenvector.argtypes = [_vp, C.POINTER(LanczosParams), C.c_int, _vp,
+                                               C.POINTER(LanczosResult)]
+        L.hphi_vec_lanczos_update_acc.argtypes = [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_double]
+        L.hphi_expec_energy_flct.argtypes = [_vp, _vp, C.POINTER(Phys)]
+        L.hphi_expec_cisajs.argtypes = [_vp, _vp, C.c_int, C.POINTER(C.c_int), _f64p]
+        L.hphi_expec_cisajscktalt.argtypes = [_vp, _vp, C.c_int, C.POINTER(C.c_int), _f64p]
         _lib = L
     return _lib
 
@@ -372,6 +387,37 @@ class Engine:
         r = LanczosResult()
         check(lib().hphi_lanczos(self.h, C.byref(p), C.byref(r)))
         return r
+
+    def lanczos_eigenvector(self, k: int = 0, n_eigen: int = 1, max_iter: int = 2000, eps: float = 1e-14,
+                            target: int = 2, seed: int = 0, check_every: int = 1,
+                            mode: int = 0) -> Tuple["DeviceArray", LanczosResult]:
+        """Eigenvector of the k-th lowest Ritz value (a DeviceArray, normalised)."""
+        p = LanczosParams(max_iter, n_eigen, eps, target, seed, check_every, mode)
+        r = LanczosResult()
+        v = DeviceArray(self.dim)
+        check(lib().hphi_lanczos_eigenvector(self.h, C.byref(p), k, v.ptr, C.byref(r)))
+        return v, r
+
+    def expec_energy_flct(self, v: "DeviceArray") -> Dict[str, float]:
+        """energy, var, doublon, doublon2, num, num2, sz, sz2 of the device vector v."""
+        out = Phys()
+        check(lib().hphi_expec_energy_flct(self.h, v.ptr, C.byref(out)))
+        return out.as_dict()
+
+    def _green(self, fn, v: "DeviceArray", idx, width: int) -> np.ndarray:
+        idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, width))
+        n = idx.shape[0]
+        out = np.zeros(2 * n)
+        check(fn(self.h, v.ptr, n, idx.ctypes.data_as(C.POINTER(C.c_int)), _p(out, _f64p)))
+        return out[0::2] + 1j * out[1::2]
+
+    def cisajs(self, v: "DeviceArray", idx) -> np.ndarray:
+        """<v|c+_{i s1} c_{j s2}|v> for rows (i, s1, j, s2)."""
+        return self._green(lib().hphi_expec_cisajs, v, idx, 4)
+
+    def cisajscktalt(self, v: "DeviceArray", idx) -> np.ndarray:
+        """<v|c+_{i s1} c_{j s2} c+_{k s3} c_{l s4}|v> for rows of 8 indices."""
+        return self._green(lib().hphi_expec_cisajscktalt, v, idx, 8)
 
 
 def herm_eig(a: np.ndarray):

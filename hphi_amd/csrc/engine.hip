@@ -398,7 +398,7 @@ static void classify_and_upload_terms(hphi_engine *e) {
 // Host-side preparation: sector sizes, split masks, gather terms (grouped
 // low/high/mixed) and diagonal terms.  No device work.
 // bits strictly between orbitals a and b
-static uint64_t between_mask(int a, int b) {
+uint64_t hphi_between_mask(int a, int b) {
   const int lo = std::min(a, b), hi = std::max(a, b);
   return (hi - lo > 1) ? (((1ull << hi) - 1) & ~((1ull << (lo + 1)) - 1)) : 0ull;
 }
@@ -410,7 +410,7 @@ static uint64_t between_mask(int a, int b) {
 static bool apply_hop(uint64_t &x, int a, int b, uint64_t inside, int &parity) {
   if (a == b) return (x >> a) & 1;
   if (((x >> a) & 1) || !((x >> b) & 1)) return false;
-  parity ^= __builtin_popcountll(x & between_mask(a, b) & inside) & 1;
+  parity ^= __builtin_popcountll(x & hphi_between_mask(a, b) & inside) & 1;
   x ^= (1ull << a) | (1ull << b);
   return true;
 }
@@ -423,12 +423,12 @@ static bool apply_hop(uint64_t &x, int a, int b, uint64_t inside, int &parity) {
 // from x ^ (p ^ q); the sign splits into a constant (the involved bits) and the
 // parity of the target's other bits under the XOR of the two hops' between-masks
 // (those bits do not change).
-static void compile_two_hops(int a1, int b1, int a2, int b2, double vre, double vim, std::vector<GTerm> &out) {
+void hphi_compile_two_hops(int a1, int b1, int a2, int b2, double vre, double vim, std::vector<GTerm> &out) {
   const uint64_t B = (1ull << a1) | (1ull << b1) | (1ull << a2) | (1ull << b2);
   int orb[4], no = 0;
   for (int k = 0; k < 64; k++)
     if ((B >> k) & 1) orb[no++] = k;
-  const uint64_t m1 = a1 != b1 ? between_mask(a1, b1) : 0, m2 = a2 != b2 ? between_mask(a2, b2) : 0;
+  const uint64_t m1 = a1 != b1 ? hphi_between_mask(a1, b1) : 0, m2 = a2 != b2 ? hphi_between_mask(a2, b2) : 0;
   for (int p = 0; p < (1 << no); p++) {
     uint64_t s = 0;
     for (int k = 0; k < no; k++)
@@ -456,8 +456,8 @@ static void compile_two_hops(int a1, int b1, int a2, int b2, double vre, double 
 // the Sz-conserving entries the canonical model accepts).  For every pattern p
 // of the involved sites with a non-zero result q, the target (pattern q)
 // gathers from target ^ (p ^ q); no sign.
-static void compile_spin_pair(int A, int s1, int s2, int B, int s3, int s4, double vre, double vim,
-                              std::vector<GTerm> &out) {
+void hphi_compile_spin_pair(int A, int s1, int s2, int B, int s3, int s4, double vre, double vim,
+                            std::vector<GTerm> &out) {
   const uint64_t mA = 1ull << A, mB = 1ull << B, M = mA | mB;
   for (uint64_t p = 0; p < 4; p++) {
     const uint64_t src = ((p & 1) ? mA : 0) | ((p & 2) ? mB : 0);
@@ -638,11 +638,11 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
       if (!gc && (q[0] == q[4] || q[1] == q[3] || q[5] == q[7] || q[3] == q[7]))
         return hphi_set_error("InterAll (%d %d %d %d %d %d %d %d) does not conserve Sz in the canonical Spin model",
                               q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]);
-      compile_spin_pair(q[0], q[1], q[3], q[4], q[5], q[7], d->interall_V[2 * t], d->interall_V[2 * t + 1],
+      hphi_compile_spin_pair(q[0], q[1], q[3], q[4], q[5], q[7], d->interall_V[2 * t], d->interall_V[2 * t + 1],
                         e->h_terms);
       continue;
     }
-    compile_two_hops(2 * q[0] + q[1], 2 * q[2] + q[3], 2 * q[4] + q[5], 2 * q[6] + q[7], d->interall_V[2 * t],
+    hphi_compile_two_hops(2 * q[0] + q[1], 2 * q[2] + q[3], 2 * q[4] + q[5], 2 * q[6] + q[7], d->interall_V[2 * t],
                      d->interall_V[2 * t + 1], e->h_terms);
   }
   classify_and_upload_terms(e);

@@ -49,6 +49,14 @@ __device__ __forceinline__ GTerm load_term(CTerm p) {
   return t;
 }
 
+// host-side term compilers shared by the Hamiltonian and the Green's functions:
+// bits strictly between orbitals a and b; V c+_{a1} c_{b1} c+_{a2} c_{b2}
+// (fermions, Jordan-Wigner sign); V (|s1><s2|)_A (|s3><s4|)_B (spin-1/2)
+uint64_t hphi_between_mask(int a, int b);
+void hphi_compile_two_hops(int a1, int b1, int a2, int b2, double vre, double vim, std::vector<GTerm> &out);
+void hphi_compile_spin_pair(int A, int s1, int s2, int B, int s3, int s4, double vre, double vim,
+                            std::vector<GTerm> &out);
+
 // diagonal term kinds (diagonalcalc.c set functions, per model)
 enum { D_INTRA = 0, D_CHEMI_H, D_CHEMI_S, D_INTER_H, D_INTER_S, D_HUND_H, D_HUND_S, D_NN, D_SS };
 

@@ -70,8 +70,11 @@ int hphi_tridiag_lowest(const double *a, const double *b, int n, int k, double *
   return k;
 }
 
-int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r) {
-  if (!e || !p || !r) return hphi_set_error("hphi_lanczos: NULL argument");
+/* The eigenvalue pass.  On success *alpha_out / *beta_out (1-based, length
+ * max_iter + 2) hold the recurrence coefficients of the r->iterations steps;
+ * the caller frees them (NULL pointers: freed here). */
+static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r, double **alpha_out,
+                        double **beta_out) {
   memset(r, 0, sizeof *r);
   uint64_t dim;
   if (hphi_engine_info(e, &dim, NULL, NULL, NULL, NULL, NULL)) return -1;
@@ -164,8 +167,151 @@ done:
   r->hv_seconds = thv;
   hphi_dev_free(v0);
   hphi_dev_free(v1);
+  if (rc == 0 && alpha_out && beta_out) {
+    *alpha_out = alpha;
+    *beta_out = beta;
+  } else {
+    free(alpha);
+    free(beta);
+  }
+  free(ev);
+  return rc;
+}
+
+int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r) {
+  if (!e || !p || !r) return hphi_set_error("hphi_lanczos: NULL argument");
+  return lanczos_core(e, p, r, NULL, NULL);
+}
+
+/* Eigenvector of the symmetric tridiagonal (a[0..n), b[0..n-1)) for the
+ * eigenvalue lam: inverse iteration, two solves of (T - lam) x = y from y = 1,
+ * with an LU factorisation by Gaussian elimination with partial pivoting (row
+ * i+1 may swap with the running row i).  Normalised, first non-zero entry > 0. */
+static void tridiag_eigvec(const double *a, const double *b, int n, double lam, double *x) {
+  if (n == 1) {
+    x[0] = 1.0;
+    return;
+  }
+  double *U = (double *)calloc(3 * (size_t)n, sizeof(double)), *L = (double *)calloc(n, sizeof(double));
+  int *sw = (int *)calloc(n, sizeof(int));
+  double *y = (double *)malloc(sizeof(double) * n);
+  double nrm_t = 0.0;
+  for (int i = 0; i < n; i++) nrm_t = fmax(nrm_t, fabs(a[i]) + (i ? fabs(b[i - 1]) : 0) + (i < n - 1 ? fabs(b[i]) : 0));
+  const double tiny = 1e-15 * (nrm_t > 0 ? nrm_t : 1.0);
+  /* running row (columns i, i+1, i+2) */
+  double c = a[0] - lam, d = b[0], f = 0.0;
+  for (int i = 0; i < n - 1; i++) {
+    const double r0 = b[i], r1 = a[i + 1] - lam, r2 = (i + 2 < n) ? b[i + 1] : 0.0;
+    if (fabs(r0) > fabs(c)) { /* row i+1 is the pivot */
+      const double m = c / r0;
+      U[3 * i] = r0, U[3 * i + 1] = r1, U[3 * i + 2] = r2;
+      L[i] = m, sw[i] = 1;
+      c = d - m * r1, d = f - m * r2, f = 0.0;
+    } else {
+      if (fabs(c) < tiny) c = tiny;
+      const double m = r0 / c;
+      U[3 * i] = c, U[3 * i + 1] = d, U[3 * i + 2] = f;
+      L[i] = m, sw[i] = 0;
+      c = r1 - m * d, d = r2 - m * f, f = 0.0;
+    }
+  }
+  if (fabs(c) < tiny) c = tiny;
+  U[3 * (n - 1)] = c;
+  for (int i = 0; i < n; i++) y[i] = 1.0;
+  for (int it = 0; it < 2; it++) {
+    /* forward: the row operations of the factorisation on y */
+    double run = y[0];
+    for (int i = 0; i < n - 1; i++) {
+      if (sw[i]) {
+        const double piv = y[i + 1];
+        y[i] = piv;
+        run = run - L[i] * piv;
+      } else {
+        y[i] = run;
+        run = y[i + 1] - L[i] * run;
+      }
+    }
+    y[n - 1] = run;
+    /* back substitution */
+    for (int i = n - 1; i >= 0; i--) {
+      double t = y[i];
+      if (i + 1 < n) t -= U[3 * i + 1] * x[i + 1];
+      if (i + 2 < n) t -= U[3 * i + 2] * x[i + 2];
+      x[i] = t / U[3 * i];
+    }
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += x[i] * x[i];
+    s = 1.0 / sqrt(s);
+    for (int i = 0; i < n; i++) y[i] = x[i] *= s;
+  }
+  int k0 = 0;
+  while (k0 < n - 1 && x[k0] == 0.0) k0++;
+  if (x[k0] < 0)
+    for (int i = 0; i < n; i++) x[i] = -x[i];
+  free(U);
+  free(L);
+  free(sw);
+  free(y);
+}
+
+int hphi_lanczos_eigenvector(hphi_engine *e, const hphi_lanczos_params *p, int k, double *vout,
+                             hphi_lanczos_result *r) {
+  if (!e || !p || !r || !vout) return hphi_set_error("hphi_lanczos_eigenvector: NULL argument");
+  if (k < 0 || k >= 16) return hphi_set_error("hphi_lanczos_eigenvector: k must be in 0..15");
+  double *alpha = NULL, *beta = NULL;
+  if (lanczos_core(e, p, r, &alpha, &beta)) return -1;
+  uint64_t dim;
+  if (hphi_engine_info(e, &dim, NULL, NULL, NULL, NULL, NULL)) return -1;
+  const int m = r->iterations > 0 ? r->iterations : 1;
+  if (k >= m) {
+    free(alpha);
+    free(beta);
+    return hphi_set_error("hphi_lanczos_eigenvector: k = %d but only %d Lanczos steps", k, m);
+  }
+  double *ev = (double *)calloc(k + 1, sizeof(double)), *c = (double *)calloc(m, sizeof(double));
+  hphi_tridiag_lowest(alpha + 1, beta + 1, m, k + 1, ev);
+  tridiag_eigvec(alpha + 1, beta + 1, m, ev[k], c);
+  r->eigenvalues[0] = ev[k];
+  double t0 = wall(), thv = 0.0, pr[2], nrm;
+  double *v0 = NULL, *v1 = NULL;
+  const size_t bytes = (size_t)dim * 16;
+  int rc = -1;
+  if (hphi_dev_alloc(bytes, (void **)&v0) || hphi_dev_alloc(bytes, (void **)&v1)) goto done;
+  /* the same start vector as the eigenvalue pass (Lanczos_EigenVector.c:65-137) */
+  if (hphi_fill_random(v1, dim, 1, p->seed) || hphi_vec_norm2(e, v1, 1, &nrm)) goto done;
+  {
+    double ab[4] = {1.0 / sqrt(nrm), 0.0, 0.0, 0.0};
+    if (hphi_vec_axpby(e, v1, v1, 1, ab)) goto done;
+  }
+  if (hphi_dev_zero(v0, bytes) || hphi_dev_zero(vout, bytes)) goto done;
+  {
+    double ab[4] = {c[0], 0.0, 0.0, 0.0}; /* v = c_1 q_1 */
+    if (hphi_vec_axpby(e, v1, vout, 1, ab)) goto done;
+  }
+  /* q_{s} = (v0 - alpha_{s-1} q_{s-1}) / beta_{s-1} in the update kernel, which
+   * also adds c_s q_s (Lanczos_EigenVector.c:139-190) */
+  for (int st = 1; st <= m; st++) {
+    if (st > 1 && hphi_vec_lanczos_update_acc(e, v0, v1, vout, alpha[st - 1], beta[st - 1], c[st - 1])) goto done;
+    if (st < m) {
+      double t = wall();
+      if (hphi_mltply_dev(e, v0, v1, pr)) goto done;
+      thv += wall() - t;
+    }
+  }
+  if (hphi_vec_norm2(e, vout, 1, &nrm)) goto done;
+  {
+    double ab[4] = {1.0 / sqrt(nrm), 0.0, 0.0, 0.0};
+    if (hphi_vec_axpby(e, vout, vout, 1, ab)) goto done;
+  }
+  rc = 0;
+done:
+  r->seconds += wall() - t0;
+  r->hv_seconds += thv;
+  hphi_dev_free(v0);
+  hphi_dev_free(v1);
   free(alpha);
   free(beta);
   free(ev);
+  free(c);
   return rc;
 }

@@ -145,6 +145,25 @@ extern "C" int hphi_vec_lanczos_beta(hphi_engine *e, const double *v0, const dou
   return finish_sum(e, g, 1, beta2);
 }
 
+// the update above plus vg += c * v1_new (eigenvector accumulation)
+__global__ __launch_bounds__(VO_TPB) void k_lanczos_update_acc(double2 *v0, double2 *v1, double2 *vg, uint64_t n,
+                                                               double alpha, double beta, double c) {
+  for (uint64_t i = (uint64_t)blockIdx.x * VO_TPB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * VO_TPB) {
+    double2 a = v0[i], b = v1[i], g = vg[i];
+    double2 t2 = make_double2((a.x - alpha * b.x) / beta, (a.y - alpha * b.y) / beta);
+    v0[i] = make_double2(-beta * b.x, -beta * b.y);
+    v1[i] = t2;
+    vg[i] = make_double2(g.x + c * t2.x, g.y + c * t2.y);
+  }
+}
+extern "C" int hphi_vec_lanczos_update_acc(hphi_engine *e, double *v0, double *v1, double *vg, double alpha,
+                                           double beta, double c) {
+  if (!e || !v0 || !v1 || !vg) return hphi_set_error("hphi_vec_lanczos_update_acc: NULL argument");
+  hipLaunchKernelGGL(k_lanczos_update_acc, dim3(vo_grid(e->dim)), dim3(VO_TPB), 0, e->stream, (double2 *)v0,
+                     (double2 *)v1, (double2 *)vg, e->dim, alpha, beta, c);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 extern "C" int hphi_vec_lanczos_update(hphi_engine *e, double *v0, double *v1, double alpha, double beta) {
   if (!e || !v0 || !v1) return hphi_set_error("hphi_vec_lanczos_update: NULL argument");
   hipLaunchKernelGGL(k_lanczos_update, dim3(vo_grid(e->dim)), dim3(VO_TPB), 0, e->stream, (double2 *)v0,

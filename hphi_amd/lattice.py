@@ -77,6 +77,10 @@ class ModelSpec:
     # ``exchange`` holds the Hubbard Exchange (i, j, J)
     interall: List[Tuple[int, int, int, int, int, int, int, int, float, float]] = field(default_factory=list)
     pairhop: List[Tuple[int, int, float]] = field(default_factory=list)
+    # Green's functions to measure: OneBodyG (i, s1, j, s2) for <c+_{i s1} c_{j s2}>,
+    # TwoBodyG (i, s1, j, s2, k, s3, l, s4) for <c+_{i s1} c_{j s2} c+_{k s3} c_{l s4}>
+    green1: List[Tuple[int, int, int, int]] = field(default_factory=list)
+    green2: List[Tuple[int, int, int, int, int, int, int, int]] = field(default_factory=list)
     calc_type: int = CALC_LANCZOS
     params: Dict[str, float] = field(default_factory=dict)
 
@@ -403,6 +407,14 @@ def write_expert(spec: ModelSpec, directory: str, calc_type: Optional[int] = Non
             txt = _header(f"N{kw}", len(lst), f"========{kw}======")
             txt += "".join(f"{i} {j} {_f(v)}\n" for i, j, v in lst)
             put(kw, fname, txt)
+    if spec.green1:
+        txt = _header("NCisAjs", len(spec.green1), "======== Green functions ======")
+        txt += "".join(" ".join(str(int(v)) for v in e) + "\n" for e in spec.green1)
+        put("OneBodyG", "greenone.def", txt)
+    if spec.green2:
+        txt = _header("NCisAjsCktAltDC", len(spec.green2), "======== Green functions for Sq AND Nq ======")
+        txt += "".join(" ".join(str(int(v)) for v in e) + "\n" for e in spec.green2)
+        put("TwoBodyG", "greentwo.def", txt)
     nl = os.path.join(directory, "namelist.def")
     with open(nl, "w") as fh:
         for kw, fname in files.items():
@@ -549,3 +561,14 @@ def processed_terms(spec: ModelSpec) -> Dict[str, np.ndarray]:
         "interall_diag_V": np.array(ia_diagV, dtype=np.float64),
         "pairhop_ij": arr2(pairhop, 2), "pairhop_J": arrv(pairhop, 2),
     }
+
+
+def processed_green(spec: ModelSpec) -> Tuple[List[Tuple[int, ...]], List[Tuple[int, ...]]]:
+    """OneBodyG / TwoBodyG entries as the reference's reader keeps them
+    (readdef.c:1441-1515): a canonical Spin one-body entry with i != j is
+    dropped with a warning; a spin two-body entry must read (i s1 i s2 k s3 k s4)."""
+    g1 = [tuple(e) for e in spec.green1 if not (spec.model == SPIN and e[0] != e[2])]
+    g2 = [tuple(e) for e in spec.green2]
+    if spec.model in (SPIN, SPIN_GC) and any(e[0] != e[2] or e[4] != e[6] for e in g2):
+        raise ValueError("spin TwoBodyG entries need i == j and k == l")
+    return g1, g2

@@ -154,6 +154,9 @@ int hphi_fill_random(double *v, uint64_t dim, int k, uint64_t seed);
  *   axpby           y_c <- a_c x_c + b_c y_c, ab = {a_0, b_0, a_1, b_1, ...} complex */
 int hphi_vec_lanczos_beta(hphi_engine *e, const double *v0, const double *v1, double alpha, double *beta2);
 int hphi_vec_lanczos_update(hphi_engine *e, double *v0, double *v1, double alpha, double beta);
+/* the update, and vg += c * v1_new (Lanczos_EigenVector.c:170-185) */
+int hphi_vec_lanczos_update_acc(hphi_engine *e, double *v0, double *v1, double *vg, double alpha, double beta,
+                                double c);
 int hphi_vec_norm2(hphi_engine *e, const double *v, int k, double *out);
 int hphi_vec_dot(hphi_engine *e, const double *x, const double *y, int k, double *out);
 int hphi_vec_axpby(hphi_engine *e, const double *x, double *y, int k, const double *ab);
@@ -183,6 +186,35 @@ typedef struct {
 } hphi_lanczos_result;
 
 int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r);
+/* Lanczos eigenvector (Lanczos_EigenVector, src/Lanczos_EigenVector.c:46-208):
+ * runs hphi_lanczos with p, takes the eigenvector c of the k-th lowest
+ * eigenvalue (k = 0, 1, ...) of the final m x m tridiagonal matrix (vec12,
+ * src/vec12.c:31-80), re-runs the recurrence from the same start vector with the
+ * stored alpha/beta and accumulates v = sum_m c_m q_m into v (device array of
+ * dim complex128), normalised to 1.  The sign of c is fixed by c_1 > 0 (the
+ * reference takes LAPACK's).  r->eigenvalues[0] is the k-th Ritz value. */
+int hphi_lanczos_eigenvector(hphi_engine *e, const hphi_lanczos_params *p, int k, double *v,
+                             hphi_lanczos_result *r);
+
+/* ---- expectation values of a normalised device vector v (dim complex128) ---- */
+/* expec_energy_flct (src/expec_energy_flct.c:40-660): energy <v|H|v>, var
+ * <Hv|Hv>; doublon, doublon2, num, num2 (Hubbard models: measured; spin:
+ * 0, 0, Nsite, Nsite^2); sz, sz2 = <Sz>, <Sz^2> with Sz = (N_up - N_down)/2
+ * (canonical Spin: the sector's Sz and its square, as the reference). */
+typedef struct {
+  double energy, var, doublon, doublon2, num, num2, sz, sz2;
+} hphi_phys;
+int hphi_expec_energy_flct(hphi_engine *e, const double *v, hphi_phys *out);
+/* One-body Green's functions <v|c+_{i s1} c_{j s2}|v> (expec_cisajs,
+ * src/expec_cisajs.c:69-650) for idx[n][4] = (i, s1, j, s2), 0-based sites;
+ * out[2n] = (re, im).  Spin models: c+_{i s1} c_{i s2} = |s1><s2| on site i (bit
+ * value = s); entries the reference sets to 0 are 0 (spin i != j; Sz-changing
+ * entries in canonical sectors). */
+int hphi_expec_cisajs(hphi_engine *e, const double *v, int n, const int *idx, double *out);
+/* Two-body Green's functions <v|c+_{i s1} c_{j s2} c+_{k s3} c_{l s4}|v>
+ * (expec_cisajscktaltdc, src/expec_cisajscktaltdc.c:87-1647) for idx[n][8];
+ * spin entries need i == j and k == l. */
+int hphi_expec_cisajscktalt(hphi_engine *e, const double *v, int n, const int *idx, double *out);
 /* Lowest `want` distinct eigenvalues of the plain-Lanczos tridiagonal
  * (alpha[0..m), beta[0..m-1)) after Cullum-Willoughby removal of spurious
  * values and merging of ghost copies; returns how many were found. */

@@ -18,6 +18,10 @@
  *                 0-based), prdct.txt
  *   mode sums   : as hv but write only checksums (sums.txt) — for large dims
  *   mode bench  : reps timed mltply calls; print one JSON line
+ *   mode green  : v0 = the synthetic vector(seed) normalised; expec_energy_flct
+ *                 (src/expec_energy_flct.c:40-160: moments on v0, then v1 = v0,
+ *                 v0 = H v1) -> phys.txt; expec_cisajs and expec_cisajscktaltdc on
+ *                 v1 write output/zvo_cisajs.dat and output/zvo_cisajscktalt.dat
  * The input directory is entered with chdir, as the reference opens its def
  * files by relative name and writes ./output/.
  */
@@ -34,6 +38,9 @@
 #include "HPhiTrans.h"
 #include "check.h"
 #include "diagonalcalc.h"
+#include "expec_cisajs.h"
+#include "expec_cisajscktaltdc.h"
+#include "expec_energy_flct.h"
 #include "global.h"
 #include "mltply.h"
 #include "readdef.h"
@@ -130,6 +137,28 @@ int main(int argc, char *argv[]) {
   for (unsigned long j = 1; j <= dim; j++) {
     v1[j] = (2.0 * unit_rng(seed, 2 * (j - 1)) - 1.0) + I * (2.0 * unit_rng(seed, 2 * (j - 1) + 1) - 1.0);
     v0[j] = 0.0;
+  }
+
+  if (strcmp(mode, "green") == 0) {
+    double n2 = 0.0;
+    for (unsigned long j = 1; j <= dim; j++) n2 += creal(conj(v1[j]) * v1[j]);
+    for (unsigned long j = 1; j <= dim; j++) {
+      v0[j] = v1[j] / sqrt(n2);
+      v1[j] = 0.0;
+    }
+    X.Bind.Def.St = 0;
+    if (expec_energy_flct(&(X.Bind)) != 0) return 6;
+    char path[4200];
+    snprintf(path, sizeof path, "%s/phys.txt", outdir);
+    FILE *f = fopen(path, "w");
+    fprintf(f, "energy %.17g\nvar %.17g\ndoublon %.17g\ndoublon2 %.17g\nnum %.17g\nnum2 %.17g\nSz %.17g\n"
+               "Sz2 %.17g\n",
+            X.Bind.Phys.energy, X.Bind.Phys.var, X.Bind.Phys.doublon, X.Bind.Phys.doublon2,
+            X.Bind.Phys.num, X.Bind.Phys.num2, X.Bind.Phys.Sz, X.Bind.Phys.Sz2);
+    fclose(f);
+    if (expec_cisajs(&(X.Bind), v1) != 0) return 6;
+    if (expec_cisajscktaltdc(&(X.Bind), v1) != 0) return 6;
+    return 0;
   }
 
   if (strcmp(mode, "bench") == 0) {

@@ -151,3 +151,76 @@ def kron_xxz_gc(nsite, bonds, Jz, Jxy, hx) -> np.ndarray:
     for i in range(nsite):
         H += -hx * op(sx, i)
     return H
+
+
+def _spin_ops(x: int, ops):
+    """(|a><b|)_site products on a spin-1/2 state, applied right to left."""
+    for site, a, b in reversed(ops):
+        if ((x >> site) & 1) != b:
+            return None
+        x = (x | (1 << site)) if a else (x & ~(1 << site))
+    return x
+
+
+def green1(spec, v, entry) -> complex:
+    """<v|c+_{i s1} c_{j s2}|v> from the definitions; for spins c+_{i s1} c_{i s2}
+    is |s1><s2| on site i and an entry with i != j is 0 (localized spins)."""
+    states = basis(spec)
+    index = {x: k for k, x in enumerate(states)}
+    i, s1, j, s2 = entry
+    acc = 0.0
+    for k, x in enumerate(states):
+        if spec.model in (L.SPIN, L.SPIN_GC):
+            if i != j:
+                return 0.0
+            y, sg = _spin_ops(x, [(i, s1, s2)]), 1
+        else:
+            y, sg = apply_ops(x, [(1, 2 * i + s1), (0, 2 * j + s2)])
+        if y is not None and sg and y in index:
+            acc += np.conj(v[index[y]]) * sg * v[k]
+    return acc
+
+
+def green2(spec, v, entry) -> complex:
+    """<v|c+_{i s1} c_{j s2} c+_{k s3} c_{l s4}|v> from the definitions (spins:
+    i == j, k == l, (|s1><s2|)_i (|s3><s4|)_k)."""
+    states = basis(spec)
+    index = {x: k for k, x in enumerate(states)}
+    i, s1, j, s2, kk, s3, l, s4 = entry
+    acc = 0.0
+    for k, x in enumerate(states):
+        if spec.model in (L.SPIN, L.SPIN_GC):
+            y, sg = _spin_ops(x, [(i, s1, s2), (kk, s3, s4)]), 1
+        else:
+            y, sg = apply_ops(x, [(1, 2 * i + s1), (0, 2 * j + s2), (1, 2 * kk + s3), (0, 2 * l + s4)])
+        if y is not None and sg and y in index:
+            acc += np.conj(v[index[y]]) * sg * v[k]
+    return acc
+
+
+def phys(spec, v, H=None) -> dict:
+    """energy <v|H|v>, var <Hv|Hv>, and the doublon / electron number / Sz
+    moments (Sz = (N_up - N_down)/2) of a normalised v; canonical Spin reports
+    its sector's Sz."""
+    H = dense(spec) if H is None else H
+    hv = H @ v
+    out = {"energy": float(np.vdot(v, hv).real), "var": float(np.vdot(hv, hv).real)}
+    w = np.abs(v) ** 2
+    states = basis(spec)
+    N = spec.nsite
+    if spec.model in (L.SPIN, L.SPIN_GC):
+        out.update(doublon=0.0, doublon2=0.0, num=float(N), num2=float(N * N))
+        if spec.model == L.SPIN:
+            sz = 0.5 * spec.two_sz
+            out.update(Sz=sz, Sz2=sz * sz)
+        else:
+            S = np.array([2 * bin(x).count("1") - N for x in states], float)
+            out.update(Sz=0.5 * np.sum(w * S), Sz2=0.25 * np.sum(w * S * S))
+    else:
+        up = np.array([sum((x >> (2 * i)) & 1 for i in range(N)) for x in states], float)
+        dn = np.array([sum((x >> (2 * i + 1)) & 1 for i in range(N)) for x in states], float)
+        D = np.array([sum((x >> (2 * i)) & (x >> (2 * i + 1)) & 1 for i in range(N)) for x in states], float)
+        out.update(doublon=np.sum(w * D), doublon2=np.sum(w * D * D), num=np.sum(w * (up + dn)),
+                   num2=np.sum(w * (up + dn) ** 2), Sz=0.5 * np.sum(w * (up - dn)),
+                   Sz2=0.25 * np.sum(w * (up - dn) ** 2))
+    return out

@@ -136,3 +136,33 @@ def hubbard_gc():
 
 def two_body_cases():
     return {c.name: c for c in (hubbard_two_body(), spin_interall(), spingc_interall(), hubbard_gc())}
+
+
+def green_cases():
+    """Models of two_body_cases() with OneBodyG / TwoBodyG entries covering each
+    branch of the reference's expec_cisajs / expec_cisajscktaltdc."""
+    sp = spin_interall()
+    sp.green1 = [(i, s, i, s) for i in range(8) for s in (0, 1)] + [(0, 0, 0, 1), (3, 1, 5, 1), (2, 1, 2, 0)]
+    sp.green2 = [(1, 0, 1, 0, 4, 0, 4, 0), (1, 0, 1, 0, 4, 1, 4, 1), (1, 0, 1, 1, 4, 1, 4, 0),
+                 (1, 1, 1, 0, 4, 0, 4, 1), (6, 0, 6, 1, 2, 1, 2, 0), (3, 0, 3, 1, 3, 1, 3, 0),
+                 (3, 1, 3, 1, 3, 1, 3, 1), (0, 1, 0, 0, 3, 1, 3, 0), (5, 1, 5, 1, 0, 0, 0, 1)]
+    gc = spingc_interall()
+    gc.green1 = [(i, a, i, b) for i in (0, 3, 7) for a in (0, 1) for b in (0, 1)] + [(0, 0, 1, 0)]
+    gc.green2 = [(1, a, 1, b, 4, c, 4, d) for a in (0, 1) for b in (0, 1) for c in (0, 1) for d in (0, 1)]
+    gc.green2 += [(2, 0, 2, 1, 2, 1, 2, 1), (2, 1, 2, 1, 2, 1, 2, 0), (6, 1, 6, 0, 2, 0, 2, 0), (5, 0, 5, 1, 5, 1, 5, 0)]
+    hb = hubbard_two_body()
+    hb.green1 = [(i, s, i, s) for i in range(6) for s in (0, 1)] + [(0, 0, 1, 0), (1, 0, 0, 0), (2, 1, 5, 1),
+                                                                   (0, 0, 0, 1), (4, 1, 3, 0)]
+    hb.green2 = [(0, 0, 0, 0, 1, 1, 1, 1), (0, 0, 1, 0, 1, 0, 0, 0), (0, 0, 1, 1, 1, 1, 0, 0),
+                 (2, 0, 3, 0, 4, 1, 5, 1), (3, 1, 3, 1, 1, 0, 2, 0), (1, 0, 4, 0, 4, 1, 4, 1),
+                 (0, 0, 2, 1, 3, 1, 1, 0), (5, 1, 5, 1, 5, 1, 5, 1), (0, 0, 1, 0, 2, 0, 3, 1)]
+    hg = hubbard_gc()
+    hg.green1 = [(i, s, i, s) for i in range(5) for s in (0, 1)] + [(0, 0, 1, 1), (2, 1, 2, 0), (1, 0, 3, 0),
+                                                                   (4, 1, 0, 0)]
+    hg.green2 = [(0, 0, 0, 0, 1, 1, 1, 1), (0, 0, 1, 1, 2, 0, 3, 0), (2, 1, 3, 0, 4, 0, 4, 1),
+                 (1, 0, 1, 1, 1, 1, 1, 0), (3, 0, 2, 0, 1, 1, 0, 0), (4, 1, 4, 1, 0, 0, 1, 0)]
+    out = {}
+    for s in (sp, gc, hb, hg):
+        s.name = "green_" + s.name
+        out[s.name] = s
+    return out

@@ -17,7 +17,12 @@ InterAll; pass --two-body) are stored under "cases_two_body": the C
 restatement in oracle/ does not cover them, so these fixtures come from the
 reference alone.
 
-usage: python tests/golden/gen_golden.py [--large | --two-body]
+Expectation values (pass --green) of the cases of green_cases() are stored
+under "cases_green": energy, variance and moments (expec_energy_flct) and the
+one- and two-body Green's functions (expec_cisajs, expec_cisajscktaltdc) of
+the normalised synthetic vector.
+
+usage: python tests/golden/gen_golden.py [--large | --two-body | --green]
 """
 
 from __future__ import annotations
@@ -37,7 +42,7 @@ sys.path.insert(0, HERE)
 
 from hphi_amd import lattice as L  # noqa: E402
 from oracle import oracle as O  # noqa: E402
-from cases import SEED, golden_cases, two_body_cases  # noqa: E402
+from cases import SEED, golden_cases, green_cases, two_body_cases  # noqa: E402
 
 FULL_MAX = 5000
 
@@ -81,6 +86,26 @@ def run_case(name, spec, work):
     return rec
 
 
+def run_green(name, spec, work):
+    """The reference's expectation values of the normalised SplitMix64 vector
+    (seed SEED): expec_energy_flct's energy, variance and moments, and the
+    OneBodyG / TwoBodyG lines of output/zvo_cisajs.dat, zvo_cisajscktalt.dat
+    (values printed with 10 decimals)."""
+    d = os.path.join(work, name)
+    L.write_expert(spec, d)
+    O.run_reference(d, "green", d + "/out", seed=SEED)
+    phys = {k: float(v) for k, v in (ln.split() for ln in open(d + "/out/phys.txt") if ln.strip())}
+
+    def lines(fname, nidx):
+        path = os.path.join(d, "output", fname)
+        if not os.path.exists(path):
+            return []
+        return [[int(x) for x in p[:nidx]] + [float(p[nidx]), float(p[nidx + 1])]
+                for p in (ln.split() for ln in open(path)) if p]
+    return {"dim": spec.dim, "seed": SEED, "phys": phys,
+            "green1": lines("zvo_cisajs.dat", 4), "green2": lines("zvo_cisajscktalt.dat", 8)}
+
+
 def run_large(n, work):
     spec = L.baseline_config(n)
     d = os.path.join(work, f"cfg{n}")
@@ -112,6 +137,10 @@ def main():
         if "--two-body" in sys.argv:
             for name, spec in two_body_cases().items():
                 gold.setdefault("cases_two_body", {})[name] = run_case(name, spec, work)
+                print(name, "done", flush=True)
+        elif "--green" in sys.argv:
+            for name, spec in green_cases().items():
+                gold.setdefault("cases_green", {})[name] = run_green(name, spec, work)
                 print(name, "done", flush=True)
         elif "--large" in sys.argv:
             for n in [int(a) for a in sys.argv[sys.argv.index("--large") + 1:] if a.isdigit()] or [4, 2, 3]:
