@@ -34,7 +34,9 @@ EXPORTED = [
     "hphi_server_create", "hphi_server_destroy", "hphi_server_submit", "hphi_server_tick",
     "hphi_server_results", "hphi_server_now", "hphi_server_stats", "hphi_lobpcg", "hphi_herm_eig",
     "hphi_lanczos_eigenvector", "hphi_vec_lanczos_update_acc", "hphi_expec_energy_flct", "hphi_expec_cisajs",
-    "hphi_expec_cisajscktalt",
+    "hphi_expec_cisajscktalt", "hphi_write_vec_file", "hphi_read_vec_file", "hphi_write_restart_vecs",
+    "hphi_read_restart_vecs", "hphi_write_tmcomponents", "hphi_read_tmcomponents", "hphi_write_energy",
+    "hphi_write_cisajs", "hphi_write_cisajscktalt", "hphi_write_tpq_files",
 ]
 
 
@@ -169,6 +171,16 @@ def lib():
         L.hphi_expec_energy_flct.argtypes = [_vp, _vp, C.POINTER(Phys)]
         L.hphi_expec_cisajs.argtypes = [_vp, _vp, C.c_int, C.POINTER(C.c_int), _f64p]
         L.hphi_expec_cisajscktalt.argtypes = [_vp, _vp, C.c_int, C.POINTER(C.c_int), _f64p]
+        L.hphi_write_vec_file.argtypes = [C.c_char_p, C.c_int, C.c_uint64, _f64p]
+        L.hphi_read_vec_file.argtypes = [C.c_char_p, C.c_uint64, _f64p, C.POINTER(C.c_int)]
+        L.hphi_write_restart_vecs.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, _f64p, _f64p]
+        L.hphi_read_restart_vecs.argtypes = [C.c_char_p, C.c_uint64, _f64p, _f64p, _u64p]
+        L.hphi_write_tmcomponents.argtypes = [C.c_char_p, C.c_int, C.c_double, _f64p, _f64p]
+        L.hphi_read_tmcomponents.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), _f64p, _f64p, _f64p]
+        L.hphi_write_energy.argtypes = [C.c_char_p, C.POINTER(Phys)]
+        L.hphi_write_cisajs.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), _f64p]
+        L.hphi_write_cisajscktalt.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int), _f64p]
+        L.hphi_write_tpq_files.argtypes = [C.c_char_p, C.c_int, C.c_int, _f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -366,9 +378,9 @@ class Engine:
 
     def tpq(self, k: int, steps: int, seed: int = 0, init_kind: int = -1, layout: int = 1,
             use_graph: bool = True, large_value: float = 0.0):
-        """Microcanonical TPQ for k samples; returns (out[steps, k, 8], TpqResult)."""
+        """Microcanonical TPQ for k samples; returns (out[steps, k, 12], TpqResult)."""
         p = TpqParams(k, steps, init_kind, layout, 1 if use_graph else 0, large_value, seed)
-        out = np.zeros((steps, k, 8))
+        out = np.zeros((steps, k, 12))
         r = TpqResult()
         check(lib().hphi_tpq(self.h, C.byref(p), _p(out, _f64p), C.byref(r)))
         return out, r
@@ -489,3 +501,73 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---- the reference's output and restart files (host-side C writers/readers) ----
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def write_vec_file(path: str, step: int, v: np.ndarray) -> None:
+    """<head>_eigenvec_<k>_rank_<r>.dat / tmpvec_set<i>_rank_<r>.dat layout."""
+    v = _c128(v)
+    check(lib().hphi_write_vec_file(path.encode(), step, v.size, v.ctypes.data_as(_f64p)))
+
+
+def read_vec_file(path: str, dim: int) -> Tuple[np.ndarray, int]:
+    v = np.zeros(dim, np.complex128)
+    st = C.c_int()
+    check(lib().hphi_read_vec_file(path.encode(), dim, v.ctypes.data_as(_f64p), C.byref(st)))
+    return v, st.value
+
+
+def write_restart_vecs(path: str, step: int, v0: np.ndarray, v1: np.ndarray) -> None:
+    v0, v1 = _c128(v0), _c128(v1)
+    check(lib().hphi_write_restart_vecs(path.encode(), step, v0.size, v0.ctypes.data_as(_f64p),
+                                        v1.ctypes.data_as(_f64p)))
+
+
+def read_restart_vecs(path: str, dim: int) -> Tuple[np.ndarray, np.ndarray, int]:
+    v0, v1 = np.zeros(dim, np.complex128), np.zeros(dim, np.complex128)
+    st = C.c_uint64()
+    check(lib().hphi_read_restart_vecs(path.encode(), dim, v0.ctypes.data_as(_f64p), v1.ctypes.data_as(_f64p),
+                                       C.byref(st)))
+    return v0, v1, st.value
+
+
+def write_tmcomponents(path: str, norm: float, alpha: np.ndarray, beta: np.ndarray) -> None:
+    a, b = np.ascontiguousarray(alpha, np.float64), np.ascontiguousarray(beta, np.float64)
+    check(lib().hphi_write_tmcomponents(path.encode(), a.size, norm, _p(a, _f64p), _p(b, _f64p)))
+
+
+def read_tmcomponents(path: str, cap: int = 100000) -> Tuple[float, np.ndarray, np.ndarray]:
+    a, b = np.zeros(cap), np.zeros(cap)
+    n, nrm = C.c_int(), C.c_double()
+    check(lib().hphi_read_tmcomponents(path.encode(), cap, C.byref(n), C.byref(nrm), _p(a, _f64p), _p(b, _f64p)))
+    return nrm.value, a[: n.value].copy(), b[: n.value].copy()
+
+
+def write_energy(path: str, phys: Dict[str, float]) -> None:
+    p = Phys(**{k: phys.get(k, 0.0) for k, _ in Phys._fields_})
+    check(lib().hphi_write_energy(path.encode(), C.byref(p)))
+
+
+def write_cisajs(path: str, idx, vals) -> None:
+    idx = np.ascontiguousarray(np.asarray(idx, np.int32).reshape(-1, 4))
+    v = np.ascontiguousarray(np.column_stack([np.real(vals), np.imag(vals)]).reshape(-1), np.float64)
+    check(lib().hphi_write_cisajs(path.encode(), idx.shape[0], idx.ctypes.data_as(C.POINTER(C.c_int)),
+                                  _p(v, _f64p)))
+
+
+def write_cisajscktalt(path: str, model: int, idx, vals) -> None:
+    idx = np.ascontiguousarray(np.asarray(idx, np.int32).reshape(-1, 8))
+    v = np.ascontiguousarray(np.column_stack([np.real(vals), np.imag(vals)]).reshape(-1), np.float64)
+    check(lib().hphi_write_cisajscktalt(path.encode(), model, idx.shape[0], idx.ctypes.data_as(C.POINTER(C.c_int)),
+                                        _p(v, _f64p)))
+
+
+def write_tpq_files(directory: str, rand_i: int, rows: np.ndarray) -> None:
+    """rows[steps, 12] of one sample (a slice out[:, c] of Engine.tpq)."""
+    r = np.ascontiguousarray(rows, np.float64)
+    check(lib().hphi_write_tpq_files(directory.encode(), rand_i, r.shape[0], _p(r, _f64p), 12))

@@ -29,7 +29,8 @@
 
 #define TQ_TPB 256
 #define TQ_GRID 1024
-#define TQ_NRED 5  // per column: |Hw|^2, obs, obs^2, |w_next|^2, particle number
+#define TQ_NRED 8  // per column: |Hw|^2, D, D^2, N, N^2, S, S^2, |w_next|^2
+#define TQ_NOUT 12  // output doubles per (step, sample)
 
 __device__ __forceinline__ uint64_t tq_splitmix64(uint64_t x) {
   uint64_t z = x + 0x9E3779B97F4A7C15ull;
@@ -82,13 +83,10 @@ __global__ __launch_bounds__(TQ_TPB) void k_tpq_colnorm(const double2 *v, uint64
   }
 }
 
-__device__ __forceinline__ double tq_obs(int kind, uint64_t x, int nsite) {
-  if (kind == 1) return (double)(2 * (int)__popcll(x) - nsite);  // SpinGC: 2 Sz(x)
-  if (kind >= 2) return (double)(int)__popcll(x & (x >> 1) & 0x5555555555555555ull);  // Hubbard: doublons
-  return 0.0;
-}
 
-// step s: var += |h|^2, obs moments of w, w_next = (l w - h/Ns) * inv_norm, |w_next|^2
+// step s: var += |h|^2, the moments of w (expec_energy_flct: obs_kind 1 SpinGC
+// S = 2 popcount - Nsite; 2 Hubbard models D doublons, N electrons,
+// S = N_up - N_down), w_next = (l w - h/Ns) * inv_norm, |w_next|^2
 __global__ __launch_bounds__(TQ_TPB) void k_tpq_fused(double2 *v1, const double2 *v0, uint64_t dim, int k,
                                                       uint64_t sj, uint64_t sc, const double *state, double lv,
                                                       double invNs, int obs_kind, int nsite,
@@ -103,14 +101,26 @@ __global__ __launch_bounds__(TQ_TPB) void k_tpq_fused(double2 *v1, const double2
     if (obs_kind) {
       const double ww = w.x * w.x + w.y * w.y;
       const uint64_t x = list1 ? list1[j] : j;
-      const double o = tq_obs(obs_kind, x, nsite);
-      r[1] += ww * o;
-      r[2] += ww * o * o;
-      if (obs_kind == 3) r[4] += ww * (double)(int)__popcll(x);  // HubbardGC: electron number
+      const uint64_t EVEN = 0x5555555555555555ull;
+      double S;
+      if (obs_kind == 1) {
+        S = (double)(2 * (int)__popcll(x) - nsite);
+      } else {
+        const double D = (double)(int)__popcll(x & (x >> 1) & EVEN);
+        const int up = (int)__popcll(x & EVEN), dn = (int)__popcll(x & (EVEN << 1));
+        const double N = (double)(up + dn);
+        S = (double)(up - dn);
+        r[1] += ww * D;
+        r[2] += ww * D * D;
+        r[3] += ww * N;
+        r[4] += ww * N * N;
+      }
+      r[5] += ww * S;
+      r[6] += ww * S * S;
     }
     const double2 wn = make_double2((lv * w.x - h.x * invNs) * sc_c, (lv * w.y - h.y * invNs) * sc_c);
     v1[idx] = wn;
-    r[3] += wn.x * wn.x + wn.y * wn.y;
+    r[7] += wn.x * wn.x + wn.y * wn.y;
   }
   __shared__ double red[TQ_TPB / 64][16][TQ_NRED];
 #pragma unroll
@@ -142,7 +152,7 @@ __global__ void k_tpq_start(const double *partial, int nblk, int k, double *stat
 
 // one block: output row of step s, next scale
 __global__ void k_tpq_finalize(const double *partial, int nblk, int k, const double2 *prdct, double *state,
-                               double lv, double Ns, double num_const, int num_measured, double *out,
+                               double lv, double Ns, int obs_kind, int nsite, double sz_const, double *out,
                                int max_steps) {
   const int c = threadIdx.x;
   const int s = (int)state[32];
@@ -155,18 +165,30 @@ __global__ void k_tpq_finalize(const double *partial, int nblk, int k, const dou
     const double E = prdct[c].x / n;
     const double var = r[0] / n;
     if (s < max_steps) {
-      double *o = out + ((uint64_t)s * k + c) * 8;
+      // row of SS_rand / Norm_rand / Flct_rand (CalcByTPQ.c:150-210,235-256) with
+      // the per-model moments of expec_energy_flct (expec_energy_flct.c:186-500)
+      double *o = out + ((uint64_t)s * k + c) * TQ_NOUT;
       o[0] = (s == 0) ? 0.0 : (2.0 * s / Ns) / (lv - E / Ns);
       o[1] = E;
       o[2] = var;
       o[3] = sqrt(n);
-      o[4] = r[1] / n;  // SpinGC: 2<Sz>; Hubbard: <doublon>
-      o[5] = r[2] / n;  // SpinGC: 4<Sz^2>; Hubbard, HubbardGC: <doublon^2>
-      o[6] = num_measured ? r[4] / n : num_const;  // HubbardGC: <N>
-      o[7] = prdct[c].y / n;
+      if (obs_kind == 2) {
+        o[4] = r[1] / n;
+        o[5] = r[2] / n;
+        o[6] = r[3] / n;
+        o[7] = r[4] / n;
+      } else {
+        o[4] = o[5] = 0.0;
+        o[6] = (double)nsite;
+        o[7] = (double)nsite * nsite;
+      }
+      o[8] = obs_kind ? 0.5 * r[5] / n : sz_const;
+      o[9] = obs_kind ? 0.25 * r[6] / n : sz_const * sz_const;
+      o[10] = prdct[c].y / n;
+      o[11] = 0.0;
     }
-    state[c] = r[3];
-    state[16 + c] = 1.0 / sqrt(r[3]);
+    state[c] = r[7];
+    state[16 + c] = 1.0 / sqrt(r[7]);
   }
   __syncthreads();
   if (c == 0) state[32] = (double)(s + 1);
@@ -212,7 +234,7 @@ struct TpqPlan {
   double *V0, *V1, *state, *part, *out;
   int k, layout, g, grid_hv;
   uint64_t sj, sc;
-  double lv, Ns, num_const;
+  double lv, Ns, sz_const;
   int obs_kind, steps;
 };
 
@@ -235,8 +257,8 @@ static int tpq_enqueue_step(hphi_engine *e, const TpqPlan &P) {
                      hphi_model_gc(e->model) ? (const uint64_t *)nullptr : (const uint64_t *)e->d_list1,
                      P.part);
   hipLaunchKernelGGL(k_tpq_finalize, dim3(1), dim3(64), 0, e->stream, (const double *)P.part, P.g, P.k,
-                     (const double2 *)(P.layout == 1 ? e->d_prdct : e->d_coef), P.state, P.lv, P.Ns, P.num_const,
-                     P.obs_kind == 3 ? 1 : 0, P.out, P.steps);
+                     (const double2 *)(P.layout == 1 ? e->d_prdct : e->d_coef), P.state, P.lv, P.Ns, P.obs_kind,
+                     e->nsite, P.sz_const, P.out, P.steps);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -262,15 +284,15 @@ extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_ho
     if (hphi_diag_maxabs(e, &m)) return -1;
     P.lv = 3.0 * m / P.Ns;  // l = 3 max_j |list_Diagonal[j]| / Ns
   }
-  P.obs_kind = e->model == HPHI_MODEL_SPINGC ? 1 : e->model == HPHI_MODEL_HUBBARD ? 2 : e->model == HPHI_MODEL_HUBBARDGC ? 3 : 0;
-  P.num_const = (e->model == HPHI_MODEL_HUBBARD) ? (double)(e->nup + e->ndown) : (double)e->nsite;
+  P.obs_kind = e->model == HPHI_MODEL_SPINGC ? 1 : hphi_model_hubbard(e->model) ? 2 : 0;
+  P.sz_const = 0.5 * (double)(e->nup - e->ndown);  // canonical Spin (expec_energy_flct.c:104-118)
   P.g = tq_grid(dim, k);
   size_t vbytes = sizeof(double2) * dim * k;
   HIPCHK(hipMalloc(&P.V0, vbytes));
   HIPCHK(hipMalloc(&P.V1, vbytes));
   HIPCHK(hipMalloc(&P.state, sizeof(double) * 64));
   HIPCHK(hipMalloc(&P.part, sizeof(double) * (size_t)TQ_GRID * k * TQ_NRED));
-  HIPCHK(hipMalloc(&P.out, sizeof(double) * (size_t)p->steps * k * 8));
+  HIPCHK(hipMalloc(&P.out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT));
   int rc = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -311,7 +333,8 @@ extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_ho
     if (hipStreamSynchronize(e->stream) != hipSuccess) { rc = hphi_set_error("hphi_tpq: stream sync failed"); break; }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
-    if (hipMemcpy(out_host, P.out, sizeof(double) * (size_t)p->steps * k * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(out_host, P.out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT, hipMemcpyDeviceToHost) !=
+        hipSuccess) {
       rc = hphi_set_error("hphi_tpq: copy of results failed");
       break;
     }

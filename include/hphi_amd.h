@@ -215,6 +215,34 @@ int hphi_expec_cisajs(hphi_engine *e, const double *v, int n, const int *idx, do
  * (expec_cisajscktaltdc, src/expec_cisajscktaltdc.c:87-1647) for idx[n][8];
  * spin entries need i == j and k == l. */
 int hphi_expec_cisajscktalt(hphi_engine *e, const double *v, int n, const int *idx, double *out);
+
+/* ---- the reference's output and restart files (host memory; no device work) ----
+ * Vectors are 0-based complex128 host arrays of dim elements; files hold the
+ * reference's 1-based arrays (element 0 written as 0, skipped on reading).
+ *   vec file      int step, unsigned long dim, complex128[dim+1]: the eigenvector
+ *                 <head>_eigenvec_<k>_rank_<r>.dat (CalcByLanczos.c:160-198,262-272)
+ *                 and the TPQ vector tmpvec_set<i>_rank_<r>.dat (CalcByTPQ.c:88-104,280-287)
+ *   restart vecs  unsigned long step, unsigned long dim, v0[dim+1], v1[dim+1]:
+ *                 <head>_recalcvec_rank_<r>.dat (Lanczos_EigenValue.c:428-489)
+ *   TMComponents  "%d \n" steps, "%.10lf \n" norm, "%.10lf %.10lf \n" alpha_i beta_i
+ *                 (OutputTMComponents / ReadTMComponents, Lanczos_EigenValue.c:624-720)
+ *   energy        Energy / Doublon / Sz lines (CalcByLanczos.c:238-252)
+ *   cisajs        " %4ld x4 %.10lf %.10lf\n" per entry (expec_cisajs.c)
+ *   cisajscktalt  " %4ld x8 %.10lf %.10lf" with the reference's per-model line ends
+ *                 (expec_cisajscktaltdc.c)
+ *   TPQ files     SS_rand<i>.dat, Norm_rand<i>.dat, Flct_rand<i>.dat in dir from hphi_tpq
+ *                 rows (12 doubles, rows[s*row_stride + q]) (CalcByTPQ.c:150-256)
+ * Readers reject a file whose dimension differs from dim. */
+int hphi_write_vec_file(const char *path, int step, uint64_t dim, const double *v);
+int hphi_read_vec_file(const char *path, uint64_t dim, double *v, int *step);
+int hphi_write_restart_vecs(const char *path, uint64_t step, uint64_t dim, const double *v0, const double *v1);
+int hphi_read_restart_vecs(const char *path, uint64_t dim, double *v0, double *v1, uint64_t *step);
+int hphi_write_tmcomponents(const char *path, int steps, double norm, const double *alpha, const double *beta);
+int hphi_read_tmcomponents(const char *path, int cap, int *steps, double *norm, double *alpha, double *beta);
+int hphi_write_energy(const char *path, const hphi_phys *p);
+int hphi_write_cisajs(const char *path, int n, const int *idx, const double *val);
+int hphi_write_cisajscktalt(const char *path, int model, int n, const int *idx, const double *val);
+int hphi_write_tpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride);
 /* Lowest `want` distinct eigenvalues of the plain-Lanczos tridiagonal
  * (alpha[0..m), beta[0..m-1)) after Cullum-Willoughby removal of spurious
  * values and merging of ghost copies; returns how many were found. */
@@ -226,10 +254,13 @@ int hphi_lanczos_converged(const double *prev, int nprev, const double *cur, int
  * when large_value <= 0.  layout 1 keeps the k vectors state-major and uses
  * the block H·v; layout 0 keeps them vector-major with one H·v per sample.
  * use_graph captures one step in a hipGraph and replays it.
- * out[(s*k + c)*8 + q], s = 0..steps-1 (the reference's step_i), q:
+ * out[(s*k + c)*12 + q], s = 0..steps-1 (the reference's step_i), q:
  *   0 inv_temp, 1 energy, 2 <H^2>, 3 norm (global_norm; step 0: global_1st_norm),
- *   4 SpinGC 2<Sz> / Hubbard, HubbardGC <doublon>, 5 SpinGC 4<Sz^2> / Hubbard,
- *   HubbardGC <doublon^2>, 6 particle number (HubbardGC: <N>), 7 Im <H> (rounding check). */
+ *   4 doublon, 5 doublon^2, 6 num, 7 num^2, 8 Sz, 9 Sz^2 (the values of
+ *   expec_energy_flct per model: measured for Hubbard models and SpinGC Sz;
+ *   Nsite, Nsite^2 and the sector's Sz for spin models), 10 Im <H> (rounding
+ *   check), 11 reserved (0).  Rows map to SS_rand / Norm_rand / Flct_rand
+ *   (hphi_write_tpq_files). */
 typedef struct {
   int k, steps, init_kind, layout, use_graph;
   double large_value;

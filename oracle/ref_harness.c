@@ -22,6 +22,9 @@
  *                 (src/expec_energy_flct.c:40-160: moments on v0, then v1 = v0,
  *                 v0 = H v1) -> phys.txt; expec_cisajs and expec_cisajscktaltdc on
  *                 v1 write output/zvo_cisajs.dat and output/zvo_cisajscktalt.dat
+ *   mode tpq    : the reference's CalcByTPQ (src/CalcByTPQ.c:52-294) as HPhiMain.c:315-321
+ *                 calls it (input written with CalcType 1); it writes output/SS_rand*.dat,
+ *                 Norm_rand*.dat, Flct_rand*.dat
  * The input directory is entered with chdir, as the reference opens its def
  * files by relative name and writes ./output/.
  */
@@ -34,6 +37,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include "CalcByTPQ.h"
 #include "Common.h"
 #include "HPhiTrans.h"
 #include "check.h"
@@ -130,6 +134,11 @@ int main(int argc, char *argv[]) {
     }
     if (dump(outdir, "list_Diagonal.bin", list_Diagonal, (dim + 1) * sizeof(double))) return 4;
     return 0;
+  }
+
+  if (strcmp(mode, "tpq") == 0) {
+    if (X.Bind.Def.iCalcType != TPQCalc) return 7;
+    return CalcByTPQ(NumAve, X.Bind.Def.Param.ExpecInterval, &X) == TRUE ? 0 : 7;
   }
 
   /* synthetic input, element j (1-based) from counter 2(j-1), 2(j-1)+1 */

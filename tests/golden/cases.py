@@ -166,3 +166,17 @@ def green_cases():
         s.name = "green_" + s.name
         out[s.name] = s
     return out
+
+
+def tpq_file_cases():
+    """Small models whose TPQ output files (SS_rand, Norm_rand, Flct_rand) the
+    reference writes for the file-format fixtures: one per observable kind."""
+    out = {}
+    for name, s in (("tpq_hubbard_chain4", L.hubbard_chain(4)), ("tpq_chain8", L.heisenberg_chain(8)),
+                    ("tpq_tri_3x2_gc", L.triangular_xxz_gc(3, 2, 1.0, 0.8, 0.1)),
+                    ("tpq_hubbard_gc", hubbard_gc())):
+        s.params["Lanczos_max"] = 12
+        s.params["NumAve"] = 2
+        s.name = name
+        out[name] = s
+    return out

@@ -22,7 +22,12 @@ under "cases_green": energy, variance and moments (expec_energy_flct) and the
 one- and two-body Green's functions (expec_cisajs, expec_cisajscktaltdc) of
 the normalised synthetic vector.
 
-usage: python tests/golden/gen_golden.py [--large | --two-body | --green]
+Reference-written output files (pass --files) go to tests/golden/files/<case>/:
+the Green's-function files of the green cases and the SS_rand / Norm_rand /
+Flct_rand files of tpq_file_cases() (the reference's own CalcByTPQ); the file
+writers are checked byte for byte against them.
+
+usage: python tests/golden/gen_golden.py [--large | --two-body | --green | --files]
 """
 
 from __future__ import annotations
@@ -42,7 +47,8 @@ sys.path.insert(0, HERE)
 
 from hphi_amd import lattice as L  # noqa: E402
 from oracle import oracle as O  # noqa: E402
-from cases import SEED, golden_cases, green_cases, two_body_cases  # noqa: E402
+from cases import SEED, golden_cases, green_cases, tpq_file_cases, two_body_cases  # noqa: E402
+import shutil  # noqa: E402
 
 FULL_MAX = 5000
 
@@ -106,6 +112,40 @@ def run_green(name, spec, work):
             "green1": lines("zvo_cisajs.dat", 4), "green2": lines("zvo_cisajscktalt.dat", 8)}
 
 
+FILES = os.path.join(HERE, "files")
+
+
+def keep_files(name, src_dir, names):
+    """Copy reference-written output files into tests/golden/files/<name>/."""
+    dst = os.path.join(FILES, name)
+    os.makedirs(dst, exist_ok=True)
+    kept = []
+    for f in names:
+        if os.path.exists(os.path.join(src_dir, f)):
+            shutil.copyfile(os.path.join(src_dir, f), os.path.join(dst, f))
+            kept.append(f)
+    return kept
+
+
+def run_files(work):
+    """Reference-written output files: the Green's-function files of the
+    green cases and the TPQ files of tpq_file_cases() (CalcByTPQ)."""
+    rec = {}
+    for name, spec in green_cases().items():
+        d = os.path.join(work, "f_" + name)
+        L.write_expert(spec, d)
+        O.run_reference(d, "green", d + "/out", seed=SEED)
+        rec[name] = {"model": spec.model,
+                     "files": keep_files(name, d + "/output", ["zvo_cisajs.dat", "zvo_cisajscktalt.dat"])}
+    for name, spec in tpq_file_cases().items():
+        d = os.path.join(work, "f_" + name)
+        L.write_expert(spec, d, calc_type=L.CALC_TPQ)
+        O.run_reference(d, "tpq", d + "/out")
+        names = [f"{k}_rand{i}.dat" for i in range(int(spec.params["NumAve"])) for k in ("SS", "Norm", "Flct")]
+        rec[name] = {"model": spec.model, "files": keep_files(name, d + "/output", names)}
+    return rec
+
+
 def run_large(n, work):
     spec = L.baseline_config(n)
     d = os.path.join(work, f"cfg{n}")
@@ -138,6 +178,9 @@ def main():
             for name, spec in two_body_cases().items():
                 gold.setdefault("cases_two_body", {})[name] = run_case(name, spec, work)
                 print(name, "done", flush=True)
+        elif "--files" in sys.argv:
+            gold["files"] = run_files(work)
+            print("files done", flush=True)
         elif "--green" in sys.argv:
             for name, spec in green_cases().items():
                 gold.setdefault("cases_green", {})[name] = run_green(name, spec, work)

@@ -261,11 +261,19 @@ def test_tpq_matches_restated_reference_step(A, oracle_lib, name, k, layout, gra
     hv = lambda v: orc.hv(v)[0]
     for c in range(k):
         ref = tpq_restated(hv, oracle_lib.fill_vector(3 + c, orc.dim), steps, lv, spec.nsite, sz)
-        got = out[:, c, :6]
+        got = out[:, c]
         assert np.allclose(got[:, :4], ref[:, :4], rtol=1e-10, atol=1e-12), (got[:3], ref[:3])
-        if sz is not None:
+        if spec.model == L.SPIN_GC:  # Sz = <2Sz>/2, Sz^2 = <(2Sz)^2>/4
+            assert np.allclose(got[:, 8], 0.5 * ref[:, 4], rtol=1e-10, atol=1e-12)
+            assert np.allclose(got[:, 9], 0.25 * ref[:, 5], rtol=1e-10, atol=1e-12)
+            assert np.all(got[:, 6] == spec.nsite) and np.all(got[:, 4] == 0)
+        elif spec.model == L.HUBBARD:  # doublon moments; fixed electron number
             assert np.allclose(got[:, 4:6], ref[:, 4:6], rtol=1e-10, atol=1e-12)
-        assert np.all(np.abs(out[:, c, 7]) <= 1e-10 * np.abs(out[:, c, 1]) + 1e-12)  # <H> real
+            assert np.allclose(got[:, 6], spec.ncond, rtol=1e-12)
+            assert np.allclose(got[:, 8], 0.5 * spec.two_sz, rtol=1e-12, atol=1e-12)
+        else:  # canonical Spin: constants of the sector
+            assert np.all(got[:, 6] == spec.nsite) and np.all(got[:, 8] == 0.5 * spec.two_sz)
+        assert np.all(np.abs(got[:, 10]) <= 1e-10 * np.abs(got[:, 1]) + 1e-12)  # <H> real
     eng.close()
 
 
