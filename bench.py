@@ -189,6 +189,30 @@ def _pmc_means(directory: str, counter: str):
     return {k: v[0] / v[1] for k, v in sums.items() if v[1]}
 
 
+def summarize_stats(csv_path: str, out: str, title: str, cmd: str, kernel: str, balg: float):
+    """Short text summary of a rocprofv3 --stats kernel_stats.csv: per kernel
+    calls and average / min / max duration; the kernel named `kernel` also gets
+    its algorithmic rate balg / average."""
+    import csv
+    rows = list(csv.DictReader(open(csv_path)))
+    lines = [f"# rocprofv3 kernel statistics: {title}", f"# command: {cmd}"]
+    for r in rows:
+        if kernel and kernel in r["Name"]:
+            avg = float(r["AverageNs"]) * 1e-9
+            lines.append(f"# {r['Name']}: average {avg * 1e3:.3f} ms -> {balg / avg / 1e12:.3f} TB/s algorithmic "
+                         f"({balg:.6e} B per launch) = {100 * balg / avg / (HBM_PEAK_GBS * 1e9):.1f} % of 8 TB/s")
+            break
+    lines.append("")
+    lines.append(f"{'kernel':<72} {'calls':>6} {'avg_ms':>10} {'min_ms':>10} {'max_ms':>10} {'pct':>6}")
+    for r in rows:
+        lines.append(f"{r['Name'][:72]:<72} {int(r['Calls']):>6} {float(r['AverageNs']) * 1e-6:>10.3f} "
+                     f"{float(r['MinNs']) * 1e-6:>10.3f} {float(r['MaxNs']) * 1e-6:>10.3f} "
+                     f"{float(r['Percentage']):>6.2f}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines[:4]))
+
+
 def reduce_pmc(fetch_dir: str, write_dir: str, out: str, cmd: str):
     """HBM bytes per H·v launch from a FETCH_SIZE and a WRITE_SIZE run of
     --pmc-probe, each scaled by what the counter reads for the calibration
@@ -337,6 +361,11 @@ def main():
                     help="write profiles/traffic_hv_j1j2_8x4.txt from two rocprofv3 --pmc runs of --pmc-probe")
     ap.add_argument("--pmc-cmd", type=str, default="", help="command line recorded in the traffic file")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--summarize-stats", nargs=2, metavar=("KERNEL_STATS_CSV", "OUT"),
+                    help="write a text summary of a rocprofv3 --stats run (with --stats-cmd, --stats-title)")
+    ap.add_argument("--stats-cmd", type=str, default="")
+    ap.add_argument("--stats-title", type=str, default="plain H·v on the J1-J2 8x4 Sz=0 sector (dim 601,080,390)")
+    ap.add_argument("--stats-kernel", type=str, default="k_hv_canon")
     ap.add_argument("--serve", action="store_true", help="served-stream benchmark instead of plain H·v")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rates", type=str, default="0", help="arrival rates /s, comma separated; 0 = all at once")
@@ -349,6 +378,10 @@ def main():
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.summarize_stats:
+        summarize_stats(args.summarize_stats[0], args.summarize_stats[1], args.stats_title, args.stats_cmd,
+                        args.stats_kernel, b_alg(601080390, 39710085120))
+        return
     if args.reduce_pmc:
         reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1], os.path.join(ROOT, "profiles", "traffic_hv_j1j2_8x4.txt"),
                    args.pmc_cmd)

@@ -39,7 +39,7 @@ int hphi_autotune(hphi_engine *e, int k, double *v0, const double *v1, int reps,
   if (!e || !v0 || !v1) return hphi_set_error("hphi_autotune: NULL argument");
   if (reps < 1) reps = 2;
   cand_t cands[96];
-  int n = 0, c2 = 0, blk = 0, lds_ok = 0;
+  int n = 0, c2 = 0, blk = 0, lds_ok = 0, crank_ok = 0;
   memset(cands, 0, sizeof cands);
   if (k == 1 && hphi_engine_get_tune(e, "use_c2", &c2) == 0 && c2) {
     /* is list_2_1 allowed in LDS for this model? */
@@ -47,13 +47,16 @@ int hphi_autotune(hphi_engine *e, int k, double *v0, const double *v1, int reps,
     hphi_engine_get_tune(e, "lds21", &saved);
     lds_ok = hphi_engine_tune(e, "lds21", 1) == 0;
     hphi_engine_tune(e, "lds21", saved);
+    hphi_engine_get_tune(e, "c2_crank", &saved);
+    crank_ok = hphi_engine_tune(e, "c2_crank", 1) == 0;
+    hphi_engine_tune(e, "c2_crank", saved);
     for (int xcd = 1; xcd >= 0; xcd--) {
       int tpbs[3] = {256, 512, 1024};
       if (lds_ok) {
         for (int a = 1; a < 3; a++) {
           cand_t *c = &cands[n++];
           add(c, "use_c2", 1), add(c, "xcd_map", xcd), add(c, "tpb", tpbs[a]);
-          add(c, "wgs_per_cu", 1), add(c, "lds21", 1);
+          add(c, "wgs_per_cu", 1), add(c, "lds21", 1), add(c, "c2_crank", 0);
         }
       }
       for (int a = 0; a < 3; a++)
@@ -61,8 +64,16 @@ int hphi_autotune(hphi_engine *e, int k, double *v0, const double *v1, int reps,
           if (tpbs[a] * w <= 4096) {
             cand_t *c = &cands[n++];
             add(c, "use_c2", 1), add(c, "xcd_map", xcd), add(c, "tpb", tpbs[a]);
-            add(c, "wgs_per_cu", w), add(c, "lds21", 0);
+            add(c, "wgs_per_cu", w), add(c, "lds21", 0), add(c, "c2_crank", 0);
           }
+      if (crank_ok) { /* spin rank tables: large grids of small workgroups */
+        const int ws[4] = {14, 21, 40, 48};
+        for (int a = 0; a < 4; a++) {
+          cand_t *c = &cands[n++];
+          add(c, "use_c2", 1), add(c, "xcd_map", xcd), add(c, "tpb", 256);
+          add(c, "wgs_per_cu", ws[a]), add(c, "c2_crank", 1);
+        }
+      }
     }
   }
   if (k > 1 && hphi_engine_get_tune(e, "use_blk", &blk) == 0 && blk) {

@@ -381,7 +381,7 @@ static void classify_and_upload_terms(hphi_engine *e) {
     z.kind = P_EQ;
     z.tmask = 0;
     z.tval = 1;  // never active
-    while (v.size() % HV_UNROLL) v.push_back(z);
+    while (v.size() % HV_TERM_PAD) v.push_back(z);
   };
   pad(low);
   pad(high);
@@ -752,10 +752,26 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
     (void)hipMemcpy(e->d_dterms, e->h_dterms.data(), e->h_dterms.size() * sizeof(DTerm), hipMemcpyHostToDevice);
   e->bytes += tb + db;
   hphi_canon_config(e);
-  if (e->c2_lds21) {  // list_2_1 in LDS: one 1024-thread workgroup per CU
+  if (e->c2_crank_ok) {
+    std::vector<uint16_t> t(CRANK_N);
+    hphi_crank_tables(t.data());
+    if (hipMalloc(&e->d_crank, sizeof(uint16_t) * CRANK_N) != hipSuccess ||
+        hipMemcpy(e->d_crank, t.data(), sizeof(uint16_t) * CRANK_N, hipMemcpyHostToDevice) != hipSuccess) {
+      hphi_engine_destroy(e);
+      return hphi_set_error("hipMalloc(rank tables) failed");
+    }
+    e->bytes += sizeof(uint16_t) * CRANK_N;
+  }
+  if (e->c2_crank_ok) {  // spin: 5 KB rank tables, many resident workgroups
+    e->c2_crank = 1;
+    e->c2_tpb = 256;
+    e->c2_wgs_per_cu = 48;  // measured on the 8x4 J1-J2 sector: 107.3 ms vs 138.9 ms with the LDS table;
+    e->xcd_map = 1;         // grids of 40-64 workgroups per CU all land within 1 %
+  } else if (e->c2_lds21) {  // list_2_1 in LDS: one 1024-thread workgroup per CU
     e->c2_tpb = 1024;
     e->c2_wgs_per_cu = 1;
-    e->xcd_map = 0;     // measured best for this shape (8x4 J1-J2, 4x4 Hubbard)
+    e->c2_unroll = 16;  // 16 gathers in flight per lane: 27.3 ms vs 28.7 ms (4x4 Hubbard)
+    e->xcd_map = 0;
   } else {
     e->c2_tpb = 512;
     e->c2_wgs_per_cu = 8;
@@ -774,6 +790,7 @@ extern "C" void hphi_engine_destroy(hphi_engine *e) {
   (void)hipFree(e->d_list1);
   (void)hipFree(e->d_l21);
   (void)hipFree(e->d_l22);
+  (void)hipFree(e->d_crank);
   (void)hipFree(e->d_jb);
   (void)hipFree(e->d_diag);
   (void)hipFree(e->d_terms);
@@ -822,6 +839,12 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
     } else {
       e->c2_lds21 = 0;
     }
+  } else if (k == "c2_crank") {
+    if (value && !e->c2_crank_ok) return hphi_set_error("rank tables need a canonical Spin model with nlow <= 16");
+    e->c2_crank = value ? 1 : 0;
+  } else if (k == "c2_unroll") {
+    if (value != 8 && value != 16) return hphi_set_error("c2_unroll must be 8 or 16");
+    e->c2_unroll = value;
   } else if (k == "xcd_map") {
     e->xcd_map = value ? 1 : 0;
   } else if (k == "use_c2") {
@@ -855,6 +878,8 @@ extern "C" int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int 
   if (k == "tpb") *value = e->c2_tpb;
   else if (k == "wgs_per_cu") *value = e->c2_wgs_per_cu;
   else if (k == "lds21") *value = e->c2_lds21;
+  else if (k == "c2_crank") *value = e->c2_crank;
+  else if (k == "c2_unroll") *value = e->c2_unroll;
   else if (k == "xcd_map") *value = e->xcd_map;
   else if (k == "use_c2") *value = e->use_c2 && e->c2_ok;
   else if (k == "grid_per_cu") *value = e->grid_per_cu;
@@ -988,6 +1013,7 @@ HvArgs hphi_hv_args(hphi_engine *e, int k, double *v0, const double *v1) {
   A.v1 = v1;
   A.partial = (double2 *)e->d_partial;
   A.xcd_map = e->xcd_map;
+  A.crank = e->d_crank;
   A.size21 = e->size21;
   (void)k;
   return A;

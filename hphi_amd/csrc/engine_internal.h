@@ -9,7 +9,9 @@
 #include "hphi_amd.h"
 
 #define HV_TPB 256        // threads per workgroup of the H·v kernels
-#define HV_UNROLL 8       // terms per unrolled gather batch (term groups are padded to it)
+#define HV_UNROLL 8       // terms per unrolled gather batch of the generic kernels
+#define HV_TERM_PAD 16    // term groups are padded with never-active terms to a multiple of
+                          // this, so every batch size in use (8, 16) reads whole groups
 #define HV_MAX_GRID 65536 // upper bound of H·v workgroups (sizes the partial-sum buffer)
 
 // predicate kinds of a gather term, evaluated on the TARGET state x
@@ -88,7 +90,14 @@ struct HvArgs {
   int xcd_map;
   uint64_t size21;
   int overwrite;  // 1: v0 = H v1 (v0 not read); 0: v0 += H v1 (reference semantics)
+  // canonical Spin: list_2_1[ia] = rank of ia among the values of its popcount
+  // + 2, from two byte tables (hphi_crank_tables)
+  const uint16_t *crank;
 };
+
+// byte tables of the combinatorial rank (spin, nlow <= 16): rank(ia) =
+// t[ia & 255] + t[256 + 9 * ((ia >> 8) & 255) + popcount(ia & 255)]
+#define CRANK_N (256 + 256 * 9)
 
 struct hphi_engine {
   int model = 0, nsite = 0, nup = 0, ndown = 0, nbits = 0, nlow = 0;
@@ -112,6 +121,8 @@ struct hphi_engine {
   // canonical single-vector kernel (hv_canon.hip) configuration
   int c2_ok = 0, c2_realc = 1, c2_lds21 = 0, c2_max_lds = 0;
   int c2_tpb = 1024, c2_wgs_per_cu = 1, use_c2 = 1;
+  int c2_crank = 0, c2_crank_ok = 0, c2_unroll = 8;
+  uint16_t *d_crank = nullptr;
   // block kernel (hv_block.hip): columns per lane for k = 2, 4, 8, 16 (0: default),
   // workgroup size (0: by table size), workgroups per CU (0: by workgroup size)
   int bk_cpl[4] = {0, 0, 0, 0};
@@ -126,6 +137,7 @@ int hphi_hv_launch_ex(hphi_engine *e, int k, double *v0, const double *v1, int o
 int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct);
 int hphi_hv_grid(const hphi_engine *e, int k);
 int hphi_canon_config(hphi_engine *e);
+void hphi_crank_tables(uint16_t *t);
 int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e);
 int hphi_ensure_binom_table();
 SzParams hphi_sz_params(const hphi_engine *e);

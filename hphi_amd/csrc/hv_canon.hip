@@ -22,7 +22,18 @@
     if (e_ != hipSuccess) return hphi_set_error("%s failed: %s", #x, hipGetErrorString(e_)); \
   } while (0)
 
-#define C2_UNROLL 8
+// rank lookup of list_2_1 (rank-in-class + 2): RK 0 the global table, 1 the
+// table staged in LDS as uint16, 2 (canonical Spin) the combinatorial rank
+// from two byte tables in LDS: the k-th lowest set bit at position p of ia
+// contributes C(p, k), split into the low byte and the high byte given the
+// low byte's popcount
+template <int RK>
+__device__ __forceinline__ int64_t r21(const HvArgs &A, const uint16_t *s21, uint32_t ia) {
+  if (RK == 2)
+    return (int64_t)s21[ia & 255u] + (int64_t)s21[256u + 9u * ((ia >> 8) & 255u) + (uint32_t)__popc(ia & 255u)] + 2;
+  if (RK == 1) return (int64_t)s21[ia];
+  return (int64_t)A.l21[ia];
+}
 
 __device__ __forceinline__ bool pred32(int kind, uint32_t tmask, uint32_t tval, uint32_t x) {
   const uint32_t ym = x & tmask;
@@ -41,12 +52,15 @@ __device__ __forceinline__ void cmac(double &ar, double &ai, double cr, double c
   }
 }
 
-template <int TPB, bool REALC, bool LDS21>
+template <int TPB, bool REALC, int RK, int UNR>
 __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
   extern __shared__ uint16_t s21[];
   const uint32_t tid = threadIdx.x;
-  if (LDS21) {
+  if (RK == 1) {
     for (uint32_t i = tid; i < (uint32_t)A.size21; i += TPB) s21[i] = (uint16_t)A.l21[i];
+    __syncthreads();
+  } else if (RK == 2) {
+    for (uint32_t i = tid; i < (uint32_t)CRANK_N; i += TPB) s21[i] = A.crank[i];
     __syncthreads();
   }
   const double2 *__restrict__ v1 = (const double2 *)A.v1;
@@ -80,31 +94,31 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
     const double2 a = v1[jj];
     const double d = A.diag[jj];
     double ar = d * a.x, ai = d * a.y;
-    const int64_t b21 = LDS21 ? (int64_t)s21[ia] : (int64_t)A.l21[ia];
+    const int64_t b21 = r21<RK>(A, s21, ia);
 
     // ---- low-half terms: target in the same block, per-lane rank lookup
-    for (int t = 0; t < A.n_low; t += C2_UNROLL) {
-      double2 g[C2_UNROLL];
-      int64_t l[C2_UNROLL];
-      bool act[C2_UNROLL];
+    for (int t = 0; t < A.n_low; t += UNR) {
+      double2 g[UNR];
+      int64_t l[UNR];
+      bool act[UNR];
       // phase 1: all rank lookups of the batch (LDS, or L2 issued back to back)
 #pragma unroll
-      for (int u = 0; u < C2_UNROLL; u++) {
+      for (int u = 0; u < UNR; u++) {
         const GTerm T = load_term(tl + t + u);
         const uint32_t y = x ^ (uint32_t)T.flip;
         act[u] = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
-        l[u] = LDS21 ? (int64_t)s21[y & irght] : (int64_t)A.l21[y & irght];
+        l[u] = r21<RK>(A, s21, y & irght);
       }
       // phase 2: all gathers of the batch, masked to the active lanes
 #pragma unroll
-      for (int u = 0; u < C2_UNROLL; u++) {
+      for (int u = 0; u < UNR; u++) {
         const uint64_t idx = (uint64_t)((int64_t)j + l[u] - b21);
         const bool a2 = act[u] & (l[u] != 0) & (idx < dim);
         g[u] = make_double2(0.0, 0.0);
         if (a2) g[u] = v1[idx];
       }
 #pragma unroll
-      for (int u = 0; u < C2_UNROLL; u++) {
+      for (int u = 0; u < UNR; u++) {
         const GTerm T = load_term(tl + t + u);
         double2 gg = g[u];
         if ((uint32_t)T.sgn != 0u && (__popc(x & (uint32_t)T.sgn) & 1)) {
@@ -122,11 +136,11 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
       // wave-uniform block: predicate, lookups and sign of high terms are scalar
       const uint32_t xh = ib0 << nlow;
       const int64_t l22_0 = (int64_t)A.l22[ib0];
-      for (int t = 0; t < A.n_high; t += C2_UNROLL) {
-        double2 g[C2_UNROLL];
-        bool au[C2_UNROLL];
+      for (int t = 0; t < A.n_high; t += UNR) {
+        double2 g[UNR];
+        bool au[UNR];
 #pragma unroll
-        for (int u = 0; u < C2_UNROLL; u++) {
+        for (int u = 0; u < UNR; u++) {
           const GTerm T = load_term(th + t + u);
           au[u] = pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, xh);
           g[u] = make_double2(0.0, 0.0);
@@ -137,7 +151,7 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
           }
         }
 #pragma unroll
-        for (int u = 0; u < C2_UNROLL; u++) {
+        for (int u = 0; u < UNR; u++) {
           const GTerm T = load_term(th + t + u);
           if (au[u]) {
             double s = ((uint32_t)T.sgn != 0u && (__popc(xh & (uint32_t)T.sgn) & 1)) ? -1.0 : 1.0;
@@ -145,27 +159,27 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
           }
         }
       }
-      for (int t = 0; t < A.n_mixed; t += C2_UNROLL) {
-        double2 g[C2_UNROLL];
-        int64_t la[C2_UNROLL], lb[C2_UNROLL];
-        bool act[C2_UNROLL];
+      for (int t = 0; t < A.n_mixed; t += UNR) {
+        double2 g[UNR];
+        int64_t la[UNR], lb[UNR];
+        bool act[UNR];
 #pragma unroll
-        for (int u = 0; u < C2_UNROLL; u++) {
+        for (int u = 0; u < UNR; u++) {
           const GTerm T = load_term(tm + t + u);
           const uint32_t y = x ^ (uint32_t)T.flip;
           act[u] = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
           lb[u] = (int64_t)A.l22[(xh ^ (uint32_t)T.flip) >> nlow];  // uniform
-          la[u] = LDS21 ? (int64_t)s21[y & irght] : (int64_t)A.l21[y & irght];
+          la[u] = r21<RK>(A, s21, y & irght);
         }
 #pragma unroll
-        for (int u = 0; u < C2_UNROLL; u++) {
+        for (int u = 0; u < UNR; u++) {
           const uint64_t idx = (uint64_t)(la[u] + lb[u] - 3);
           const bool a2 = act[u] & (la[u] != 0) & (lb[u] != 0) & (idx < dim);
           g[u] = make_double2(0.0, 0.0);
           if (a2) g[u] = v1[idx];
         }
 #pragma unroll
-        for (int u = 0; u < C2_UNROLL; u++) {
+        for (int u = 0; u < UNR; u++) {
           const GTerm T = load_term(tm + t + u);
           double2 gg = g[u];
           if ((uint32_t)T.sgn != 0u && (__popc(x & (uint32_t)T.sgn) & 1)) {
@@ -178,17 +192,17 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
     } else {
       // lanes span several blocks: per-lane list_2_2 lookups
       const int64_t b22 = (int64_t)A.l22[ib];
-      for (int t = 0; t < A.n_high + A.n_mixed; t += C2_UNROLL) {
-        double2 g[C2_UNROLL];
+      for (int t = 0; t < A.n_high + A.n_mixed; t += UNR) {
+        double2 g[UNR];
 #pragma unroll
-        for (int u = 0; u < C2_UNROLL; u++) {
+        for (int u = 0; u < UNR; u++) {
           const GTerm T = load_term(th + t + u);
           const uint32_t y = x ^ (uint32_t)T.flip;
           bool act = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
           int64_t la = b21, lb = 0;
           if (act) {
             lb = (int64_t)A.l22[y >> nlow];
-            if ((y & irght) != ia) la = LDS21 ? (int64_t)s21[y & irght] : (int64_t)A.l21[y & irght];
+            if ((y & irght) != ia) la = r21<RK>(A, s21, y & irght);
           }
           const uint64_t idx = (uint64_t)(la + lb - 3);
           act = act & (la != 0) & (lb != 0) & (idx < dim);
@@ -196,7 +210,7 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
           if (act) g[u] = v1[idx];
         }
 #pragma unroll
-        for (int u = 0; u < C2_UNROLL; u++) {
+        for (int u = 0; u < UNR; u++) {
           const GTerm T = load_term(th + t + u);
           double2 gg = g[u];
           if ((uint32_t)T.sgn != 0u && (__popc(x & (uint32_t)T.sgn) & 1)) {
@@ -254,29 +268,67 @@ int hphi_canon_config(hphi_engine *e) {
   bool small_rank = e->nlow <= 16;
   e->c2_lds21 = small_rank && need + 1024 <= (size_t)maxlds;
   e->c2_ok = e->nbits <= 32 && !hphi_model_gc(e->model);
+  e->c2_crank_ok = e->c2_ok && e->model == HPHI_MODEL_SPIN && e->nlow <= 16;
   return 0;
 }
 
-template <int TPB, bool REALC, bool LDS21>
+// t[b] (b < 256): sum over the k-th set bit of b (k = 1, 2, ...) at position p
+// of C(p, k); t[256 + 9 h + c]: the same for the bits of h at positions 8 + q
+// with k counted from c + 1 (c = set bits below position 8).  For a canonical
+// spin sector list_2_1[ia] - 2 is the rank of ia among the values of its
+// popcount in ascending order, which is this sum (combinatorial number system).
+void hphi_crank_tables(uint16_t *t) {
+  uint32_t C[17][17] = {};
+  for (int n = 0; n <= 16; n++) {
+    C[n][0] = 1;
+    for (int k = 1; k <= n; k++) C[n][k] = C[n - 1][k - 1] + (k <= n - 1 ? C[n - 1][k] : 0);
+  }
+  for (int b = 0; b < 256; b++) {
+    uint32_t r = 0;
+    int k = 0;
+    for (int p = 0; p < 8; p++)
+      if ((b >> p) & 1) r += C[p][++k];
+    t[b] = (uint16_t)r;
+  }
+  for (int h = 0; h < 256; h++)
+    for (int c = 0; c <= 8; c++) {
+      uint32_t r = 0;
+      int k = c;
+      for (int q = 0; q < 8; q++)
+        if ((h >> q) & 1) r += C[8 + q][++k];
+      t[256 + 9 * h + c] = (uint16_t)r;
+    }
+}
+
+template <int TPB, bool REALC, int RK, int UNR>
 static int launch_one(hphi_engine *e, const HvArgs &A, int g) {
-  size_t lds = LDS21 ? 2 * (size_t)e->size21 : 0;
-  if (LDS21 && lds > 65536)
-    (void)hipFuncSetAttribute((const void *)k_hv_canon<TPB, REALC, LDS21>,
+  const size_t lds = RK == 1 ? 2 * (size_t)e->size21 : RK == 2 ? 2 * (size_t)CRANK_N : 0;
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void *)k_hv_canon<TPB, REALC, RK, UNR>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_hv_canon<TPB, REALC, LDS21>), dim3(g), dim3(TPB), lds, e->stream, A);
+  hipLaunchKernelGGL((k_hv_canon<TPB, REALC, RK, UNR>), dim3(g), dim3(TPB), lds, e->stream, A);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
+template <int TPB, bool REALC>
+static int launch_rk(hphi_engine *e, const HvArgs &A, int g) {
+  const int rk = e->c2_crank ? 2 : (e->c2_lds21 ? 1 : 0);
+  if (e->c2_unroll == 16) {
+    if (rk == 2) return launch_one<TPB, REALC, 2, 16>(e, A, g);
+    if (rk == 1) return launch_one<TPB, REALC, 1, 16>(e, A, g);
+    return launch_one<TPB, REALC, 0, 16>(e, A, g);
+  }
+  if (rk == 2) return launch_one<TPB, REALC, 2, 8>(e, A, g);
+  if (rk == 1) return launch_one<TPB, REALC, 1, 8>(e, A, g);
+  return launch_one<TPB, REALC, 0, 8>(e, A, g);
+}
+
 int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g) {
   const int tpb = e->c2_tpb;
-#define SEL(TPB)                                                                      \
-  if (tpb == TPB) {                                                                   \
-    if (e->c2_realc) return e->c2_lds21 ? launch_one<TPB, true, true>(e, A, g)        \
-                                        : launch_one<TPB, true, false>(e, A, g);      \
-    return e->c2_lds21 ? launch_one<TPB, false, true>(e, A, g)                        \
-                       : launch_one<TPB, false, false>(e, A, g);                      \
-  }
+  if (e->c2_crank && !A.crank) return hphi_set_error("rank tables are not built");
+#define SEL(TPB)                                                                   \
+  if (tpb == TPB) return e->c2_realc ? launch_rk<TPB, true>(e, A, g) : launch_rk<TPB, false>(e, A, g);
   SEL(256)
   SEL(512)
   SEL(1024)

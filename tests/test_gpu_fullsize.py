@@ -47,14 +47,16 @@ def _checksums(h):
     return s1, s2, s3, l1, l1w
 
 
-@pytest.mark.parametrize("n", [4, 2, 3])
-def test_full_size_hv_matches_reference_checksums(A, n):
+@pytest.mark.parametrize("n,tuning", [(4, {}), (2, {}), (3, {}), (3, {"c2_crank": 1, "tpb": 256, "wgs_per_cu": 8})],
+                         ids=["cfg4", "cfg2", "cfg3", "cfg3-rank-tables"])
+def test_full_size_hv_matches_reference_checksums(A, n, tuning):
     rec = GOLD.get(f"cfg{n}")
     if rec is None:
         pytest.skip(f"no reference checksums for workload {n}")
     spec = L.baseline_config(n)
-    print(f"[gpu] full-size H·v workload {n} dim {spec.dim}", flush=True)
+    print(f"[gpu] full-size H·v workload {n} dim {spec.dim} {tuning}", flush=True)
     eng = A.Engine.from_spec(spec)
+    eng.tune(**tuning)
     dim = eng.dim
     assert dim == rec["dim"]
     v1 = A.DeviceArray(dim)

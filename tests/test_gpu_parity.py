@@ -215,7 +215,11 @@ def test_lanczos_known_answers(A, oracle_lib, name, spec, ev, tol):
 
 TUNINGS = [dict(use_c2=0), dict(use_c2=1, lds21=0, tpb=256, wgs_per_cu=4),
            dict(use_c2=1, lds21=1, tpb=1024, wgs_per_cu=1, xcd_map=1),
-           dict(use_c2=1, lds21=1, tpb=512, wgs_per_cu=1, xcd_map=0)]
+           dict(use_c2=1, lds21=1, tpb=512, wgs_per_cu=1, xcd_map=0),
+           dict(use_c2=1, lds21=1, c2_unroll=16, tpb=1024, wgs_per_cu=1),
+           dict(use_c2=1, lds21=0, c2_unroll=16, tpb=256, wgs_per_cu=4),
+           dict(use_c2=1, c2_crank=1, tpb=256, wgs_per_cu=8),           # spin only: rank tables
+           dict(use_c2=1, c2_crank=1, c2_unroll=16, tpb=512, wgs_per_cu=2, xcd_map=1)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))

@@ -84,7 +84,9 @@ def test_two_body_tables_diag_hv_vs_reference(A, oracle_lib, tmp_path, name, sou
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("tuning", [dict(use_c2=0), dict(use_c2=1, lds21=1, tpb=256, wgs_per_cu=4),
-                                    dict(use_c2=1, lds21=0, tpb=512, wgs_per_cu=2)],
+                                    dict(use_c2=1, lds21=0, tpb=512, wgs_per_cu=2),
+                                    dict(use_c2=1, lds21=0, c2_unroll=16, tpb=256, wgs_per_cu=4),
+                                    dict(use_c2=1, c2_crank=1, tpb=256, wgs_per_cu=8)],
                          ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
 @pytest.mark.parametrize("name", CASES)
 def test_two_body_every_kernel(A, oracle_lib, tuning, name):
@@ -93,8 +95,10 @@ def test_two_body_every_kernel(A, oracle_lib, tuning, name):
     spec = two_body_cases()[name]
     rec = _rec(name)
     print(f"[gpu] two-body kernels {name} {tuning}", flush=True)
+    if tuning.get("c2_crank") and spec.model != L.SPIN:
+        pytest.skip("rank tables are for canonical Spin sectors")
     eng = A.Engine.from_spec(spec)
-    if spec.model != L.SPIN_GC:  # the canonical kernels' knobs
+    if spec.model not in (L.SPIN_GC, L.HUBBARD_GC):  # the canonical kernels' knobs
         eng.tune(**tuning)
     v = oracle_lib.fill_vector(SEED, eng.dim)
     hv, _ = eng.hv(v)
