@@ -99,6 +99,12 @@ struct HvArgs {
 // t[ia & 255] + t[256 + 9 * ((ia >> 8) & 255) + popcount(ia & 255)]
 #define CRANK_N (256 + 256 * 9)
 
+// list_2_1[ia] (rank + 2) of a canonical spin sector from the byte tables t
+// staged in LDS (hphi_crank_tables)
+__device__ __forceinline__ int64_t crank_l21(const uint16_t *t, uint32_t ia) {
+  return (int64_t)t[ia & 255u] + (int64_t)t[256u + 9u * ((ia >> 8) & 255u) + (uint32_t)__popc(ia & 255u)] + 2;
+}
+
 struct hphi_engine {
   int model = 0, nsite = 0, nup = 0, ndown = 0, nbits = 0, nlow = 0;
   uint64_t dim = 0, sdim = 0, size21 = 0, size22 = 0, irght = 0, ilft = 0, ihfbit = 0;

@@ -64,7 +64,14 @@ __device__ __forceinline__ void bk_load(const double2 *__restrict__ p, bool act,
   }
 }
 
-template <int TPB, int K, int CPL, bool CANON>
+// rank lookup: the uint16 list_2_1 staged in LDS, or (CRANK, canonical Spin)
+// the byte tables of the combinatorial rank
+template <bool CRANK>
+__device__ __forceinline__ int64_t blk_l21(const uint16_t *s21, uint32_t ia) {
+  return CRANK ? crank_l21(s21, ia) : (int64_t)s21[ia];
+}
+
+template <int TPB, int K, int CPL, bool CANON, bool CRANK>
 __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc) {
   constexpr int LPS = K / CPL;                // lanes per state
   constexpr int SPT = TPB / LPS;              // states per tile
@@ -72,7 +79,10 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
   static_assert(K % CPL == 0 && LPS <= 64, "bad block shape");
   extern __shared__ uint16_t s21[];
   const uint32_t tid = threadIdx.x;
-  if (CANON) {
+  if (CANON && CRANK) {
+    for (uint32_t i = tid; i < (uint32_t)CRANK_N; i += TPB) s21[i] = A.crank[i];
+    __syncthreads();
+  } else if (CANON) {
     for (uint32_t i = tid; i < (uint32_t)A.size21; i += TPB) s21[i] = (uint16_t)A.l21[i];
     __syncthreads();
   }
@@ -122,7 +132,7 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
     }
     if (CANON) {
       const uint32_t ia = x & irght, ib = x >> nlow;
-      const int64_t b21 = (int64_t)s21[ia];
+      const int64_t b21 = blk_l21<CRANK>(s21, ia);
       // ---- low-half terms: same block, rank lookup in LDS
       for (int t = 0; t < A.n_low; t += U) {
         uint64_t idx[U];
@@ -131,7 +141,7 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
         for (int u = 0; u < U; u++) {
           const GTerm T = load_term(tl + t + u);
           const uint32_t y = x ^ (uint32_t)T.flip;
-          const int64_t l = (int64_t)s21[y & irght];
+          const int64_t l = blk_l21<CRANK>(s21, y & irght);
           idx[u] = (uint64_t)((int64_t)j + l - b21);
           act[u] = valid & bk_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x) & (l != 0) & (idx[u] < dim);
         }
@@ -181,7 +191,7 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
             const GTerm T = load_term(tm + t + u);
             const uint32_t y = x ^ (uint32_t)T.flip;
             const int64_t lb = (int64_t)A.l22[(xh ^ (uint32_t)T.flip) >> nlow];  // uniform
-            const int64_t la = (int64_t)s21[y & irght];
+            const int64_t la = blk_l21<CRANK>(s21, y & irght);
             idx[u] = (uint64_t)(la + lb - 3);
             act[u] = valid & bk_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x) & (la != 0) & (lb != 0) &
                      (idx[u] < dim);
@@ -209,7 +219,7 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
             int64_t la = b21, lb = 0;
             if (a) {
               lb = (int64_t)A.l22[y >> nlow];
-              if ((y & irght) != ia) la = (int64_t)s21[y & irght];
+              if ((y & irght) != ia) la = blk_l21<CRANK>(s21, y & irght);
             }
             idx[u] = (uint64_t)(la + lb - 3);
             act[u] = a & (la != 0) & (lb != 0) & (idx[u] < dim);
@@ -294,22 +304,26 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
 
 // ---------------------------------------------------------------- host ----
 
-template <int TPB, int K, int CPL, bool CANON>
+template <int TPB, int K, int CPL, bool CANON, bool CRANK>
 static int bk_launch(hphi_engine *e, const HvArgs &A, int g) {
-  const size_t lds = CANON ? 2 * (size_t)e->size21 : 0;
+  const size_t lds = CANON ? (CRANK ? 2 * (size_t)CRANK_N : 2 * (size_t)e->size21) : 0;
   if (lds > 65536)
-    (void)hipFuncSetAttribute((const void *)k_hv_blk<TPB, K, CPL, CANON>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-  hipLaunchKernelGGL((k_hv_blk<TPB, K, CPL, CANON>), dim3(g), dim3(TPB), lds, e->stream, A, e->c2_realc);
+    (void)hipFuncSetAttribute((const void *)k_hv_blk<TPB, K, CPL, CANON, CRANK>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_hv_blk<TPB, K, CPL, CANON, CRANK>), dim3(g), dim3(TPB), lds, e->stream, A, e->c2_realc);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 template <int K, int CPL>
 static int bk_launch_tpb(hphi_engine *e, const HvArgs &A, int g, int tpb, bool canon) {
-  if (!canon) return bk_launch<256, K, CPL, false>(e, A, g);
-  if (tpb == 1024) return bk_launch<1024, K, CPL, true>(e, A, g);
-  return bk_launch<256, K, CPL, true>(e, A, g);
+  if (!canon) return bk_launch<256, K, CPL, false, false>(e, A, g);
+  if (e->c2_crank) {
+    if (!A.crank) return hphi_set_error("rank tables are not built");
+    return bk_launch<256, K, CPL, true, true>(e, A, g);
+  }
+  if (tpb == 1024) return bk_launch<1024, K, CPL, true, false>(e, A, g);
+  return bk_launch<256, K, CPL, true, false>(e, A, g);
 }
 
 // columns per lane actually instantiated for each K
@@ -321,7 +335,7 @@ int hphi_blk_cpl_ok(int k, int cpl) {
 int hphi_blk_ok(const hphi_engine *e) {
   if (e->nbits > 32) return 0;
   if (hphi_model_gc(e->model)) return 1;
-  return e->c2_lds21;
+  return e->c2_lds21 || e->c2_crank;
 }
 
 // default launch shape for the block kernel
@@ -337,11 +351,11 @@ void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_pe
   *cpl = c;
   const bool canon = !hphi_model_gc(e->model);
   int t = e->bk_tpb;
-  if (t != 256 && t != 1024) t = (canon && 2 * e->size21 > 32768) ? 1024 : 256;
-  if (!canon) t = 256;
+  if (t != 256 && t != 1024) t = (canon && !e->c2_crank && 2 * e->size21 > 32768) ? 1024 : 256;
+  if (!canon || e->c2_crank) t = 256;  // the rank tables leave room for many 256-thread workgroups
   *tpb = t;
   int w = e->bk_wgs_per_cu;
-  if (w <= 0) w = (t == 1024) ? 1 : 8;
+  if (w <= 0) w = (t == 1024) ? 1 : (canon && e->c2_crank ? 48 : 8);  // 8x4 J1-J2, k=4: 110.4 ms/column
   *wgs_per_cu = w;
 }
 

@@ -29,8 +29,7 @@
 // low byte's popcount
 template <int RK>
 __device__ __forceinline__ int64_t r21(const HvArgs &A, const uint16_t *s21, uint32_t ia) {
-  if (RK == 2)
-    return (int64_t)s21[ia & 255u] + (int64_t)s21[256u + 9u * ((ia >> 8) & 255u) + (uint32_t)__popc(ia & 255u)] + 2;
+  if (RK == 2) return crank_l21(s21, ia);
   if (RK == 1) return (int64_t)s21[ia];
   return (int64_t)A.l21[ia];
 }
