@@ -36,7 +36,7 @@ EXPORTED = [
     "hphi_lanczos_eigenvector", "hphi_vec_lanczos_update_acc", "hphi_expec_energy_flct", "hphi_expec_cisajs",
     "hphi_expec_cisajscktalt", "hphi_write_vec_file", "hphi_read_vec_file", "hphi_write_restart_vecs",
     "hphi_read_restart_vecs", "hphi_write_tmcomponents", "hphi_read_tmcomponents", "hphi_write_energy",
-    "hphi_write_cisajs", "hphi_write_cisajscktalt", "hphi_write_tpq_files",
+    "hphi_write_cisajs", "hphi_write_cisajscktalt", "hphi_write_tpq_files", "hphi_lanczos_restart",
 ]
 
 
@@ -181,6 +181,8 @@ def lib():
         L.hphi_write_cisajs.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), _f64p]
         L.hphi_write_cisajscktalt.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int), _f64p]
         L.hphi_write_tpq_files.argtypes = [C.c_char_p, C.c_int, C.c_int, _f64p, C.c_int]
+        L.hphi_lanczos_restart.argtypes = [_vp, C.POINTER(LanczosParams), C.c_char_p, C.c_char_p, C.c_char_p,
+                                           C.c_char_p, C.POINTER(LanczosResult)]
         _lib = L
     return _lib
 
@@ -398,6 +400,20 @@ class Engine:
         p = LanczosParams(max_iter, n_eigen, eps, target, seed, check_every, mode)
         r = LanczosResult()
         check(lib().hphi_lanczos(self.h, C.byref(p), C.byref(r)))
+        return r
+
+    def lanczos_restart(self, restart_in: Optional[Tuple[str, str]] = None,
+                        restart_out: Optional[Tuple[str, str]] = None, n_eigen: int = 1, max_iter: int = 2000,
+                        eps: float = 1e-14, target: int = 2, seed: int = 0, check_every: int = 1,
+                        mode: int = 0) -> LanczosResult:
+        """Lanczos continued from / writing the reference's restart files:
+        restart_in / restart_out = (TMComponents path, recalcvec path)."""
+        p = LanczosParams(max_iter, n_eigen, eps, target, seed, check_every, mode)
+        r = LanczosResult()
+        enc = lambda x: x.encode() if x else None
+        ti, vi = restart_in or (None, None)
+        to, vo = restart_out or (None, None)
+        check(lib().hphi_lanczos_restart(self.h, C.byref(p), enc(ti), enc(vi), enc(to), enc(vo), C.byref(r)))
         return r
 
     def lanczos_eigenvector(self, k: int = 0, n_eigen: int = 1, max_iter: int = 2000, eps: float = 1e-14,

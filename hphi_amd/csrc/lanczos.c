@@ -70,16 +70,30 @@ int hphi_tridiag_lowest(const double *a, const double *b, int n, int k, double *
   return k;
 }
 
+/* State of a run continued from, or handed to, the reference's restart files
+ * (Lanczos_EigenValue.c:78-100): n steps done, alpha_1..alpha_n and beta_1..beta_n
+ * (0-based arrays), and the host vectors v0, v1 after step n's H·v. */
+typedef struct {
+  int n;
+  double *alpha, *beta;
+  double *v0, *v1;
+  double norm;
+} lz_state;
+
 /* The eigenvalue pass.  On success *alpha_out / *beta_out (1-based, length
  * max_iter + 2) hold the recurrence coefficients of the r->iterations steps;
- * the caller frees them (NULL pointers: freed here). */
+ * the caller frees them (NULL pointers: freed here).  With in->n > 0 the run
+ * continues from that state for up to max_iter further steps; out (if not
+ * NULL) receives the final state (arrays allocated here, freed by the caller). */
 static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r, double **alpha_out,
-                        double **beta_out) {
+                        double **beta_out, const lz_state *in, lz_state *out) {
   memset(r, 0, sizeof *r);
   uint64_t dim;
   if (hphi_engine_info(e, &dim, NULL, NULL, NULL, NULL, NULL)) return -1;
-  int maxit = p->max_iter > 0 ? p->max_iter : 2000;
+  const int n0 = (in && in->n > 0) ? in->n : 0;
+  int maxit = (p->max_iter > 0 ? p->max_iter : 2000) + n0; /* Lanczos_EigenValue.c:93 */
   if ((uint64_t)maxit > dim) maxit = (int)dim; /* Lanczos_EigenValue.c:126-133 */
+  if (n0 && n0 >= maxit) return hphi_set_error("restart: %d steps done, nothing left to run", n0);
   int target = p->target;
   if (target < p->n_eigen) target = p->n_eigen; /* readdef.c:969 */
   int nwant = p->n_eigen > 16 ? 16 : (p->n_eigen < 1 ? 1 : p->n_eigen);
@@ -96,11 +110,23 @@ static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lancz
   double *alpha = (double *)calloc(maxit + 2, sizeof(double));
   double *beta = (double *)calloc(maxit + 2, sizeof(double));
   double *ev = (double *)calloc(maxit + 2, sizeof(double));
-  int rc = -1, stp = 1, nprev = 0;
-  double pr[2], nrm, b2, ebefor = 0.0, prevv[16];
+  int rc = -1, stp = 1, nprev = 0, last = 0, stp0 = 2;
+  double pr[2], nrm, b2, ebefor = 0.0, prevv[16], norm0 = 0.0;
 
+  if (n0) { /* continue: the recurrence state of step n0 */
+    for (int i = 0; i < n0; i++) {
+      alpha[i + 1] = in->alpha[i];
+      beta[i + 1] = in->beta[i];
+    }
+    if (hphi_memcpy_h2d(v0, in->v0, bytes) || hphi_memcpy_h2d(v1, in->v1, bytes)) goto done;
+    norm0 = in->norm;
+    stp0 = n0 + 1;
+    last = n0;
+    goto loop;
+  }
   /* initial vector (SetInitialVector): random, normalised */
   if (hphi_fill_random(v1, dim, 1, p->seed) || hphi_vec_norm2(e, v1, 1, &nrm)) goto done;
+  norm0 = sqrt(nrm);
   {
     double ab[4] = {0.0, 0.0, 1.0 / sqrt(nrm), 0.0}; /* v1 <- 0*v1 + (1/|v1|) v1 */
     if (hphi_vec_axpby(e, v1, v1, 1, ab)) goto done;
@@ -114,6 +140,7 @@ static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lancz
   alpha[1] = pr[0];
   if (hphi_vec_lanczos_beta(e, v0, v1, alpha[1], &b2)) goto done;
   beta[1] = sqrt(b2);
+  last = 1;
   if (dim == 1) {
     r->eigenvalues[0] = alpha[1];
     r->n_found = 1;
@@ -122,7 +149,8 @@ static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lancz
     rc = 0;
     goto done;
   }
-  for (stp = 2; stp <= maxit; stp++) {
+loop:
+  for (stp = stp0; stp <= maxit; stp++) {
     if (hphi_vec_lanczos_update(e, v0, v1, alpha[stp - 1], beta[stp - 1])) goto done;
     double t = wall();
     if (hphi_mltply_dev(e, v0, v1, pr)) goto done;
@@ -130,6 +158,7 @@ static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lancz
     alpha[stp] = pr[0];
     if (hphi_vec_lanczos_beta(e, v0, v1, alpha[stp], &b2)) goto done;
     beta[stp] = sqrt(b2);
+    last = stp;
     if (stp % every != 0 && stp != maxit && beta[stp] >= 1e-14) continue;
     if (p->mode == 1) {
       double cur[16];
@@ -162,6 +191,23 @@ static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lancz
     }
   }
   rc = 0;
+  if (out) { /* the state of the last completed step */
+    out->n = last;
+    out->norm = norm0;
+    out->alpha = (double *)malloc(sizeof(double) * (last + 1));
+    out->beta = (double *)malloc(sizeof(double) * (last + 1));
+    out->v0 = (double *)malloc(bytes);
+    out->v1 = (double *)malloc(bytes);
+    if (!out->alpha || !out->beta || !out->v0 || !out->v1) {
+      rc = hphi_set_error("restart: host allocation failed");
+    } else {
+      for (int i = 0; i < last; i++) {
+        out->alpha[i] = alpha[i + 1];
+        out->beta[i] = beta[i + 1];
+      }
+      if (hphi_memcpy_d2h(out->v0, v0, bytes) || hphi_memcpy_d2h(out->v1, v1, bytes)) rc = -1;
+    }
+  }
 done:
   r->seconds = wall() - t0;
   r->hv_seconds = thv;
@@ -180,7 +226,57 @@ done:
 
 int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r) {
   if (!e || !p || !r) return hphi_set_error("hphi_lanczos: NULL argument");
-  return lanczos_core(e, p, r, NULL, NULL);
+  return lanczos_core(e, p, r, NULL, NULL, NULL, NULL);
+}
+
+static void lz_free(lz_state *s) {
+  free(s->alpha);
+  free(s->beta);
+  free(s->v0);
+  free(s->v1);
+  memset(s, 0, sizeof *s);
+}
+
+int hphi_lanczos_restart(hphi_engine *e, const hphi_lanczos_params *p, const char *tm_in, const char *vec_in,
+                         const char *tm_out, const char *vec_out, hphi_lanczos_result *r) {
+  if (!e || !p || !r) return hphi_set_error("hphi_lanczos_restart: NULL argument");
+  if ((tm_in == NULL) != (vec_in == NULL) || (tm_out == NULL) != (vec_out == NULL))
+    return hphi_set_error("hphi_lanczos_restart: the components and vector files come in pairs");
+  uint64_t dim;
+  if (hphi_engine_info(e, &dim, NULL, NULL, NULL, NULL, NULL)) return -1;
+  lz_state in = {0}, out = {0};
+  int rc = -1;
+  if (tm_in) { /* ReadInitialVector + ReadTMComponents (Lanczos_EigenValue.c:78-91) */
+    const int cap = 1 << 20;
+    uint64_t nvec = 0;
+    in.alpha = (double *)malloc(sizeof(double) * cap);
+    in.beta = (double *)malloc(sizeof(double) * cap);
+    in.v0 = (double *)malloc((size_t)dim * 16);
+    in.v1 = (double *)malloc((size_t)dim * 16);
+    if (!in.alpha || !in.beta || !in.v0 || !in.v1) {
+      hphi_set_error("restart: host allocation failed");
+      goto end;
+    }
+    if (hphi_read_tmcomponents(tm_in, cap, &in.n, &in.norm, in.alpha, in.beta) ||
+        hphi_read_restart_vecs(vec_in, dim, in.v0, in.v1, &nvec))
+      goto end;
+    if (nvec != (uint64_t)in.n) {
+      hphi_set_error("restart: %s holds step %llu but %s has %d components", vec_in, (unsigned long long)nvec,
+                     tm_in, in.n);
+      goto end;
+    }
+  }
+  if (lanczos_core(e, p, r, NULL, NULL, &in, tm_out ? &out : NULL)) goto end;
+  if (tm_out) { /* OutputTMComponents + OutputLanczosVector (Lanczos_EigenValue.c:282-287,456-489) */
+    if (hphi_write_tmcomponents(tm_out, out.n, out.norm, out.alpha, out.beta) ||
+        hphi_write_restart_vecs(vec_out, (uint64_t)out.n, dim, out.v0, out.v1))
+      goto end;
+  }
+  rc = 0;
+end:
+  lz_free(&in);
+  lz_free(&out);
+  return rc;
 }
 
 /* Eigenvector of the symmetric tridiagonal (a[0..n), b[0..n-1)) for the
@@ -259,7 +355,7 @@ int hphi_lanczos_eigenvector(hphi_engine *e, const hphi_lanczos_params *p, int k
   if (!e || !p || !r || !vout) return hphi_set_error("hphi_lanczos_eigenvector: NULL argument");
   if (k < 0 || k >= 16) return hphi_set_error("hphi_lanczos_eigenvector: k must be in 0..15");
   double *alpha = NULL, *beta = NULL;
-  if (lanczos_core(e, p, r, &alpha, &beta)) return -1;
+  if (lanczos_core(e, p, r, &alpha, &beta, NULL, NULL)) return -1;
   uint64_t dim;
   if (hphi_engine_info(e, &dim, NULL, NULL, NULL, NULL, NULL)) return -1;
   const int m = r->iterations > 0 ? r->iterations : 1;

@@ -186,6 +186,14 @@ typedef struct {
 } hphi_lanczos_result;
 
 int hphi_lanczos(hphi_engine *e, const hphi_lanczos_params *p, hphi_lanczos_result *r);
+/* Lanczos with the reference's restart files (ReStart, Lanczos_EigenValue.c:78-100,
+ * 282-287, 428-489): with tm_in / vec_in (a TMComponents file and a recalcvec file
+ * of the same step n) the recurrence continues from step n for up to max_iter
+ * further steps; with tm_out / vec_out the components and vectors of the last
+ * step are written on return.  Either pair may be NULL.  The continued run uses
+ * the alpha, beta read from the file (10 decimals), as the reference does. */
+int hphi_lanczos_restart(hphi_engine *e, const hphi_lanczos_params *p, const char *tm_in, const char *vec_in,
+                         const char *tm_out, const char *vec_out, hphi_lanczos_result *r);
 /* Lanczos eigenvector (Lanczos_EigenVector, src/Lanczos_EigenVector.c:46-208):
  * runs hphi_lanczos with p, takes the eigenvector c of the k-th lowest
  * eigenvalue (k = 0, 1, ...) of the final m x m tridiagonal matrix (vec12,

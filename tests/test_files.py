@@ -163,3 +163,30 @@ def test_device_outputs_to_reference_files(A, tmp_path, oracle_lib):
     assert np.allclose(vals[:6], want, atol=1e-13)
     dv.free()
     eng.close()
+
+
+@pytest.mark.gpu
+def test_lanczos_restart_continues_the_recurrence(A, tmp_path):
+    """30 steps written to TMComponents + recalcvec, continued for 40 more,
+    equal a straight 70-step run: the first 30 components identically, the rest
+    and the eigenvalues to the precision of the 10-decimal alpha, beta the
+    continued run starts from (as the reference's restart)."""
+    print("[gpu] lanczos restart", flush=True)
+    spec = L.heisenberg_chain(20, 1.0, 0.3)  # dim 184,756: 70 steps stop short of machine-precision convergence
+    eng = A.Engine.from_spec(spec)
+    f = lambda n: (str(tmp_path / f"tm{n}.dat"), str(tmp_path / f"vec{n}.dat"))
+    kw = dict(n_eigen=1, eps=1e-300, target=0, seed=4)
+    eng.lanczos_restart(restart_out=f(1), max_iter=30, **kw)
+    rb = eng.lanczos_restart(restart_in=f(1), restart_out=f(2), max_iter=40, **kw)
+    rc = eng.lanczos_restart(restart_out=f(3), max_iter=70, **kw)
+    _, a2, b2 = capi.read_tmcomponents(f(2)[0])
+    _, a3, b3 = capi.read_tmcomponents(f(3)[0])
+    assert len(a2) == len(a3) == 70 and rb.iterations == rc.iterations == 70
+    assert np.array_equal(a2[:30], a3[:30]) and np.array_equal(b2[:30], b3[:30])
+    assert np.allclose(a2, a3, rtol=0, atol=1e-8) and np.allclose(b2, b3, rtol=0, atol=1e-8)
+    assert abs(rb.eigenvalues[0] - rc.eigenvalues[0]) <= 1e-9
+    _, _, st = capi.read_restart_vecs(f(2)[1], eng.dim)
+    assert st == 70
+    with pytest.raises(A.HphiError, match="holds step"):  # mismatched pair
+        eng.lanczos_restart(restart_in=(f(1)[0], f(2)[1]), max_iter=5, **kw)
+    eng.close()
