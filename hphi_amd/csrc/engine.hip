@@ -753,20 +753,25 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
   e->bytes += tb + db;
   hphi_canon_config(e);
   if (e->c2_crank_ok) {
-    std::vector<uint16_t> t(CRANK_N);
-    hphi_crank_tables(t.data());
-    if (hipMalloc(&e->d_crank, sizeof(uint16_t) * CRANK_N) != hipSuccess ||
-        hipMemcpy(e->d_crank, t.data(), sizeof(uint16_t) * CRANK_N, hipMemcpyHostToDevice) != hipSuccess) {
+    const bool hub = e->model == HPHI_MODEL_HUBBARD;
+    const size_t n = hub ? HRANK_N : CRANK_N;
+    std::vector<uint16_t> t(n);
+    if (hub) hphi_hrank_tables(t.data());
+    else hphi_crank_tables(t.data());
+    if (hipMalloc(&e->d_crank, sizeof(uint16_t) * n) != hipSuccess ||
+        hipMemcpy(e->d_crank, t.data(), sizeof(uint16_t) * n, hipMemcpyHostToDevice) != hipSuccess) {
       hphi_engine_destroy(e);
       return hphi_set_error("hipMalloc(rank tables) failed");
     }
-    e->bytes += sizeof(uint16_t) * CRANK_N;
+    e->bytes += sizeof(uint16_t) * n;
   }
-  if (e->c2_crank_ok) {  // spin: 5 KB rank tables, many resident workgroups
+  if (e->c2_crank_ok) {  // rank tables (5 / 13 KB of LDS): many resident workgroups
     e->c2_crank = 1;
     e->c2_tpb = 256;
-    e->c2_wgs_per_cu = 48;  // measured on the 8x4 J1-J2 sector: 107.3 ms vs 138.9 ms with the LDS table;
-    e->xcd_map = 1;         // grids of 40-64 workgroups per CU all land within 1 %
+    e->c2_wgs_per_cu = 48;  // 8x4 J1-J2: 107.3 ms vs 138.9 ms with the LDS table, grids of 40-64 workgroups
+                            // per CU within 1 %; 4x4 Hubbard: 20.1 ms vs 27.3 ms
+    e->c2_unroll = 8;
+    e->xcd_map = e->model == HPHI_MODEL_SPIN ? 1 : 0;
   } else if (e->c2_lds21) {  // list_2_1 in LDS: one 1024-thread workgroup per CU
     e->c2_tpb = 1024;
     e->c2_wgs_per_cu = 1;
@@ -840,7 +845,8 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
       e->c2_lds21 = 0;
     }
   } else if (k == "c2_crank") {
-    if (value && !e->c2_crank_ok) return hphi_set_error("rank tables need a canonical Spin model with nlow <= 16");
+    if (value && !e->c2_crank_ok)
+      return hphi_set_error("rank tables need a canonical Spin or Hubbard model with nlow <= 16");
     e->c2_crank = value ? 1 : 0;
   } else if (k == "c2_unroll") {
     if (value != 8 && value != 16) return hphi_set_error("c2_unroll must be 8 or 16");

@@ -105,6 +105,16 @@ __device__ __forceinline__ int64_t crank_l21(const uint16_t *t, uint32_t ia) {
   return (int64_t)t[ia & 255u] + (int64_t)t[256u + 9u * ((ia >> 8) & 255u) + (uint32_t)__popc(ia & 255u)] + 2;
 }
 
+// canonical Hubbard (up orbitals even bits, down odd): the rank of ia among the
+// values with its up and down counts, t[ia & 255] + t[256 + 25 h + 5 u + d]
+// with u, d the up / down counts of the low byte (hphi_hrank_tables)
+#define HRANK_N (256 + 256 * 25)
+__device__ __forceinline__ int64_t hrank_l21(const uint16_t *t, uint32_t ia) {
+  return (int64_t)t[ia & 255u] +
+         (int64_t)t[256u + 25u * ((ia >> 8) & 255u) + 5u * (uint32_t)__popc(ia & 0x55u) + (uint32_t)__popc(ia & 0xAAu)] +
+         2;
+}
+
 struct hphi_engine {
   int model = 0, nsite = 0, nup = 0, ndown = 0, nbits = 0, nlow = 0;
   uint64_t dim = 0, sdim = 0, size21 = 0, size22 = 0, irght = 0, ilft = 0, ihfbit = 0;
@@ -144,6 +154,7 @@ int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct);
 int hphi_hv_grid(const hphi_engine *e, int k);
 int hphi_canon_config(hphi_engine *e);
 void hphi_crank_tables(uint16_t *t);
+void hphi_hrank_tables(uint16_t *t);
 int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e);
 int hphi_ensure_binom_table();
 SzParams hphi_sz_params(const hphi_engine *e);

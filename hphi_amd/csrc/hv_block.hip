@@ -66,12 +66,13 @@ __device__ __forceinline__ void bk_load(const double2 *__restrict__ p, bool act,
 
 // rank lookup: the uint16 list_2_1 staged in LDS, or (CRANK, canonical Spin)
 // the byte tables of the combinatorial rank
-template <bool CRANK>
+template <int CRANK>
 __device__ __forceinline__ int64_t blk_l21(const uint16_t *s21, uint32_t ia) {
-  return CRANK ? crank_l21(s21, ia) : (int64_t)s21[ia];
+  return CRANK == 2 ? hrank_l21(s21, ia) : CRANK == 1 ? crank_l21(s21, ia) : (int64_t)s21[ia];
 }
 
-template <int TPB, int K, int CPL, bool CANON, bool CRANK>
+// CRANK: 0 the uint16 list_2_1 in LDS, 1 spin rank tables, 2 Hubbard rank tables
+template <int TPB, int K, int CPL, bool CANON, int CRANK>
 __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc) {
   constexpr int LPS = K / CPL;                // lanes per state
   constexpr int SPT = TPB / LPS;              // states per tile
@@ -80,7 +81,8 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
   extern __shared__ uint16_t s21[];
   const uint32_t tid = threadIdx.x;
   if (CANON && CRANK) {
-    for (uint32_t i = tid; i < (uint32_t)CRANK_N; i += TPB) s21[i] = A.crank[i];
+    const uint32_t n = CRANK == 2 ? (uint32_t)HRANK_N : (uint32_t)CRANK_N;
+    for (uint32_t i = tid; i < n; i += TPB) s21[i] = A.crank[i];
     __syncthreads();
   } else if (CANON) {
     for (uint32_t i = tid; i < (uint32_t)A.size21; i += TPB) s21[i] = (uint16_t)A.l21[i];
@@ -304,9 +306,9 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
 
 // ---------------------------------------------------------------- host ----
 
-template <int TPB, int K, int CPL, bool CANON, bool CRANK>
+template <int TPB, int K, int CPL, bool CANON, int CRANK>
 static int bk_launch(hphi_engine *e, const HvArgs &A, int g) {
-  const size_t lds = CANON ? (CRANK ? 2 * (size_t)CRANK_N : 2 * (size_t)e->size21) : 0;
+  const size_t lds = CANON ? (CRANK == 2 ? 2 * (size_t)HRANK_N : CRANK == 1 ? 2 * (size_t)CRANK_N : 2 * (size_t)e->size21) : 0;
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void *)k_hv_blk<TPB, K, CPL, CANON, CRANK>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -317,13 +319,14 @@ static int bk_launch(hphi_engine *e, const HvArgs &A, int g) {
 
 template <int K, int CPL>
 static int bk_launch_tpb(hphi_engine *e, const HvArgs &A, int g, int tpb, bool canon) {
-  if (!canon) return bk_launch<256, K, CPL, false, false>(e, A, g);
+  if (!canon) return bk_launch<256, K, CPL, false, 0>(e, A, g);
   if (e->c2_crank) {
     if (!A.crank) return hphi_set_error("rank tables are not built");
-    return bk_launch<256, K, CPL, true, true>(e, A, g);
+    if (e->model == HPHI_MODEL_HUBBARD) return bk_launch<256, K, CPL, true, 2>(e, A, g);
+    return bk_launch<256, K, CPL, true, 1>(e, A, g);
   }
-  if (tpb == 1024) return bk_launch<1024, K, CPL, true, false>(e, A, g);
-  return bk_launch<256, K, CPL, true, false>(e, A, g);
+  if (tpb == 1024) return bk_launch<1024, K, CPL, true, 0>(e, A, g);
+  return bk_launch<256, K, CPL, true, 0>(e, A, g);
 }
 
 // columns per lane actually instantiated for each K

@@ -26,9 +26,12 @@
 // table staged in LDS as uint16, 2 (canonical Spin) the combinatorial rank
 // from two byte tables in LDS: the k-th lowest set bit at position p of ia
 // contributes C(p, k), split into the low byte and the high byte given the
-// low byte's popcount
+// low byte's popcount; 3 (canonical Hubbard) the same with one binomial per
+// spin species, the high byte's table indexed by the low byte's up and down
+// counts
 template <int RK>
 __device__ __forceinline__ int64_t r21(const HvArgs &A, const uint16_t *s21, uint32_t ia) {
+  if (RK == 3) return hrank_l21(s21, ia);
   if (RK == 2) return crank_l21(s21, ia);
   if (RK == 1) return (int64_t)s21[ia];
   return (int64_t)A.l21[ia];
@@ -58,8 +61,9 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
   if (RK == 1) {
     for (uint32_t i = tid; i < (uint32_t)A.size21; i += TPB) s21[i] = (uint16_t)A.l21[i];
     __syncthreads();
-  } else if (RK == 2) {
-    for (uint32_t i = tid; i < (uint32_t)CRANK_N; i += TPB) s21[i] = A.crank[i];
+  } else if (RK >= 2) {
+    const uint32_t n = RK == 3 ? (uint32_t)HRANK_N : (uint32_t)CRANK_N;
+    for (uint32_t i = tid; i < n; i += TPB) s21[i] = A.crank[i];
     __syncthreads();
   }
   const double2 *__restrict__ v1 = (const double2 *)A.v1;
@@ -267,7 +271,7 @@ int hphi_canon_config(hphi_engine *e) {
   bool small_rank = e->nlow <= 16;
   e->c2_lds21 = small_rank && need + 1024 <= (size_t)maxlds;
   e->c2_ok = e->nbits <= 32 && !hphi_model_gc(e->model);
-  e->c2_crank_ok = e->c2_ok && e->model == HPHI_MODEL_SPIN && e->nlow <= 16;
+  e->c2_crank_ok = e->c2_ok && (e->model == HPHI_MODEL_SPIN || e->model == HPHI_MODEL_HUBBARD) && e->nlow <= 16;
   return 0;
 }
 
@@ -299,9 +303,39 @@ void hphi_crank_tables(uint16_t *t) {
     }
 }
 
+// Hubbard: list_2_1[ia] - 2 = sum over the set bits p of ia of
+// C(ceil(p/2), up bits at positions <= p) C(floor(p/2), down bits at positions
+// <= p) (the class rank the sz kernel builds, C = 0 outside 0..n).  t[b]: the
+// bits of the low byte; t[256 + 25 h + 5 u + d]: the bits of the high byte h
+// given u up and d down bits in the low byte.
+void hphi_hrank_tables(uint16_t *t) {
+  auto C = [](int n, int k) -> uint32_t {
+    if (k < 0 || k > n) return 0;
+    uint32_t r = 1;
+    for (int i = 1; i <= k; i++) r = r * (uint32_t)(n - k + i) / (uint32_t)i;
+    return r;
+  };
+  auto part = [&](uint32_t bits, int p0, int u0, int d0) -> uint32_t {
+    // bits of one byte at positions p0..p0+7, counts below p0 given
+    uint32_t r = 0;
+    int u = u0, d = d0;
+    for (int q = 0; q < 8; q++) {
+      if (!((bits >> q) & 1)) continue;
+      const int p = p0 + q;
+      if ((p & 1) == 0) u++; else d++;
+      r += C((p + 1) / 2, u) * C(p / 2, d);
+    }
+    return r;
+  };
+  for (int b = 0; b < 256; b++) t[b] = (uint16_t)part((uint32_t)b, 0, 0, 0);
+  for (int h = 0; h < 256; h++)
+    for (int u = 0; u <= 4; u++)
+      for (int d = 0; d <= 4; d++) t[256 + 25 * h + 5 * u + d] = (uint16_t)part((uint32_t)h, 8, u, d);
+}
+
 template <int TPB, bool REALC, int RK, int UNR>
 static int launch_one(hphi_engine *e, const HvArgs &A, int g) {
-  const size_t lds = RK == 1 ? 2 * (size_t)e->size21 : RK == 2 ? 2 * (size_t)CRANK_N : 0;
+  const size_t lds = RK == 1 ? 2 * (size_t)e->size21 : RK == 2 ? 2 * (size_t)CRANK_N : RK == 3 ? 2 * (size_t)HRANK_N : 0;
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void *)k_hv_canon<TPB, REALC, RK, UNR>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -312,12 +346,14 @@ static int launch_one(hphi_engine *e, const HvArgs &A, int g) {
 
 template <int TPB, bool REALC>
 static int launch_rk(hphi_engine *e, const HvArgs &A, int g) {
-  const int rk = e->c2_crank ? 2 : (e->c2_lds21 ? 1 : 0);
+  const int rk = e->c2_crank ? (e->model == HPHI_MODEL_HUBBARD ? 3 : 2) : (e->c2_lds21 ? 1 : 0);
   if (e->c2_unroll == 16) {
+    if (rk == 3) return launch_one<TPB, REALC, 3, 16>(e, A, g);
     if (rk == 2) return launch_one<TPB, REALC, 2, 16>(e, A, g);
     if (rk == 1) return launch_one<TPB, REALC, 1, 16>(e, A, g);
     return launch_one<TPB, REALC, 0, 16>(e, A, g);
   }
+  if (rk == 3) return launch_one<TPB, REALC, 3, 8>(e, A, g);
   if (rk == 2) return launch_one<TPB, REALC, 2, 8>(e, A, g);
   if (rk == 1) return launch_one<TPB, REALC, 1, 8>(e, A, g);
   return launch_one<TPB, REALC, 0, 8>(e, A, g);

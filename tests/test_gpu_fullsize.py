@@ -47,8 +47,10 @@ def _checksums(h):
     return s1, s2, s3, l1, l1w
 
 
-@pytest.mark.parametrize("n,tuning", [(4, {}), (2, {}), (3, {}), (3, {"c2_crank": 1, "tpb": 256, "wgs_per_cu": 8})],
-                         ids=["cfg4", "cfg2", "cfg3", "cfg3-rank-tables"])
+@pytest.mark.parametrize("n,tuning", [(4, {}), (2, {}), (3, {}), (3, {"c2_crank": 1, "tpb": 256, "wgs_per_cu": 8}),
+                                      (3, {"c2_crank": 0, "lds21": 1, "tpb": 1024, "wgs_per_cu": 1}),
+                                      (2, {"c2_crank": 1, "tpb": 256, "wgs_per_cu": 48})],
+                         ids=["cfg4", "cfg2", "cfg3", "cfg3-rank-tables", "cfg3-lds-table", "cfg2-rank-tables"])
 def test_full_size_hv_matches_reference_checksums(A, n, tuning):
     rec = GOLD.get(f"cfg{n}")
     if rec is None:

@@ -95,8 +95,8 @@ def test_two_body_every_kernel(A, oracle_lib, tuning, name):
     spec = two_body_cases()[name]
     rec = _rec(name)
     print(f"[gpu] two-body kernels {name} {tuning}", flush=True)
-    if tuning.get("c2_crank") and spec.model != L.SPIN:
-        pytest.skip("rank tables are for canonical Spin sectors")
+    if tuning.get("c2_crank") and spec.model not in (L.SPIN, L.HUBBARD):
+        pytest.skip("rank tables are for canonical sectors")
     eng = A.Engine.from_spec(spec)
     if spec.model not in (L.SPIN_GC, L.HUBBARD_GC):  # the canonical kernels' knobs
         eng.tune(**tuning)
