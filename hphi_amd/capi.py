@@ -37,6 +37,7 @@ EXPORTED = [
     "hphi_expec_cisajscktalt", "hphi_write_vec_file", "hphi_read_vec_file", "hphi_write_restart_vecs",
     "hphi_read_restart_vecs", "hphi_write_tmcomponents", "hphi_read_tmcomponents", "hphi_write_energy",
     "hphi_write_cisajs", "hphi_write_cisajscktalt", "hphi_write_tpq_files", "hphi_lanczos_restart",
+    "hphi_ctpq", "hphi_write_ctpq_files",
 ]
 
 
@@ -80,6 +81,11 @@ class LanczosResult(C.Structure):
 class TpqParams(C.Structure):
     _fields_ = [("k", C.c_int), ("steps", C.c_int), ("init_kind", C.c_int), ("layout", C.c_int),
                 ("use_graph", C.c_int), ("large_value", C.c_double), ("seed", C.c_uint64)]
+
+
+class CtpqParams(C.Structure):
+    _fields_ = [("k", C.c_int), ("steps", C.c_int), ("init_kind", C.c_int), ("expand_coef", C.c_int),
+                ("large_value", C.c_double), ("seed", C.c_uint64)]
 
 
 class TpqResult(C.Structure):
@@ -154,6 +160,7 @@ def lib():
         L.hphi_autotune.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_char_p, C.c_size_t]
         L.hphi_tpq.argtypes = [_vp, C.POINTER(TpqParams), _f64p, C.POINTER(TpqResult)]
         L.hphi_diag_maxabs.argtypes = [_vp, _f64p]
+        L.hphi_ctpq.argtypes = [_vp, C.POINTER(CtpqParams), _f64p, C.POINTER(TpqResult)]
         L.hphi_lobpcg.argtypes = [_vp, C.POINTER(LobpcgParams), _f64p, C.POINTER(LobpcgResult)]
         L.hphi_herm_eig.argtypes = [C.c_int, _f64p, _f64p, _f64p]
         L.hphi_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(_vp)]
@@ -181,6 +188,7 @@ def lib():
         L.hphi_write_cisajs.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), _f64p]
         L.hphi_write_cisajscktalt.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int), _f64p]
         L.hphi_write_tpq_files.argtypes = [C.c_char_p, C.c_int, C.c_int, _f64p, C.c_int]
+        L.hphi_write_ctpq_files.argtypes = [C.c_char_p, C.c_int, C.c_int, _f64p, C.c_int]
         L.hphi_lanczos_restart.argtypes = [_vp, C.POINTER(LanczosParams), C.c_char_p, C.c_char_p, C.c_char_p,
                                            C.c_char_p, C.POINTER(LanczosResult)]
         _lib = L
@@ -387,6 +395,15 @@ class Engine:
         check(lib().hphi_tpq(self.h, C.byref(p), _p(out, _f64p), C.byref(r)))
         return out, r
 
+    def ctpq(self, k: int, steps: int, seed: int = 0, init_kind: int = -1, expand_coef: int = 0,
+             large_value: float = 0.0):
+        """Canonical TPQ for k samples; returns (out[steps, k, 12], TpqResult)."""
+        p = CtpqParams(k, steps, init_kind, expand_coef, large_value, seed)
+        out = np.zeros((steps, k, 12))
+        r = TpqResult()
+        check(lib().hphi_ctpq(self.h, C.byref(p), _p(out, _f64p), C.byref(r)))
+        return out, r
+
     def diag_maxabs(self) -> float:
         m = C.c_double()
         check(lib().hphi_diag_maxabs(self.h, C.byref(m)))
@@ -587,3 +604,9 @@ def write_tpq_files(directory: str, rand_i: int, rows: np.ndarray) -> None:
     """rows[steps, 12] of one sample (a slice out[:, c] of Engine.tpq)."""
     r = np.ascontiguousarray(rows, np.float64)
     check(lib().hphi_write_tpq_files(directory.encode(), rand_i, r.shape[0], _p(r, _f64p), 12))
+
+
+def write_ctpq_files(directory: str, rand_i: int, rows: np.ndarray) -> None:
+    """rows[steps, 12] of one canonical TPQ sample (a slice out[:, c] of Engine.ctpq)."""
+    r = np.ascontiguousarray(rows, np.float64)
+    check(lib().hphi_write_ctpq_files(directory.encode(), rand_i, r.shape[0], _p(r, _f64p), 12))

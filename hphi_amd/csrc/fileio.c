@@ -189,7 +189,9 @@ int hphi_write_cisajscktalt(const char *path, int model, int n, const int *idx, 
   return fclose(f) == 0 ? 0 : hphi_set_error("write failed: %s", path);
 }
 
-int hphi_write_tpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride) {
+/* flct_from: first step with a Flct_rand line (microcanonical 1, CalcByTPQ.c:150-170;
+ * canonical 0, CalcByCanonicalTPQ.c:193-200) */
+static int write_tpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride, int flct_from) {
   if (!dir || (steps > 0 && !rows)) return hphi_set_error("hphi_write_tpq_files: NULL argument");
   char p_ss[4096], p_norm[4096], p_flct[4096];
   snprintf(p_ss, sizeof p_ss, "%s/SS_rand%d.dat", dir, rand_i);
@@ -216,10 +218,18 @@ int hphi_write_tpq_files(const char *dir, int rand_i, int steps, const double *r
     const double *o = rows + (size_t)s * row_stride;
     fprintf(fs, "%.16lf  %.16lf %.16lf %.16lf %.16lf %d\n", o[0], o[1], o[2], o[4], o[6], s);
     fprintf(fn, "%.16lf %.16lf %.16lf %d\n", o[0], s == 0 ? norm1 : o[3], norm1, s);
-    if (s > 0) /* no fluctuation line at step 0 (CalcByTPQ.c:150-170) */
+    if (s >= flct_from)
       fprintf(ff, "%.16lf %.16lf %.16lf %.16lf %.16lf %.16lf %.16lf %d\n", o[0], o[6], o[7], o[4], o[5], o[8], o[9],
               s);
   }
   const int e1 = fclose(fs), e2 = fclose(fn), e3 = fclose(ff);
   return (e1 || e2 || e3) ? hphi_set_error("write failed in %s", dir) : 0;
+}
+
+int hphi_write_tpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride) {
+  return write_tpq_files(dir, rand_i, steps, rows, row_stride, 1);
+}
+
+int hphi_write_ctpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride) {
+  return write_tpq_files(dir, rand_i, steps, rows, row_stride, 0);
 }

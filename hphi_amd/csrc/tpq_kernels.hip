@@ -355,3 +355,227 @@ extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_ho
   (void)hipFree(P.out);
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// Canonical TPQ (CalcByCanonicalTPQ, src/CalcByCanonicalTPQ.c:52-300): each
+// step applies exp(-dtau H / 2) to the normalised psi by its Taylor series to
+// order ExpandCoef (MultiplyForCanonicalTPQ, src/Multiply.c:152-205), reusing H psi
+// of the previous measurement, normalises, and measures (expec_energy_flct):
+// ExpandCoef block H·v per step.  Vectors are state-major [dim][k].
+// ---------------------------------------------------------------------------
+
+// A = P + t Hp
+__global__ __launch_bounds__(256) void k_ct_acc0(double2 *A, const double2 *P, const double2 *Hp, uint64_t n, double t) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double2 p = P[i], h = Hp[i];
+    A[i] = make_double2(p.x + t * h.x, p.y + t * h.y);
+  }
+}
+
+// A += t C
+__global__ __launch_bounds__(256) void k_ct_acc(double2 *A, const double2 *Cv, uint64_t n, double t) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double2 a = A[i], c = Cv[i];
+    A[i] = make_double2(a.x + t * c.x, a.y + t * c.y);
+  }
+}
+
+// norms of the k columns from k_tpq_colnorm partials: state[c] = |A_c|,
+// state[16 + c] = 1 / |A_c|
+__global__ void k_ct_norms(const double *partial, int nblk, int k, double *state) {
+  const int c = threadIdx.x;
+  if (c < k) {
+    double n = 0.0;
+    for (int b = 0; b < nblk; b++) n += partial[(uint64_t)b * k + c];
+    state[c] = sqrt(n);
+    state[16 + c] = 1.0 / sqrt(n);
+  }
+}
+
+// P[j][c] = A[j][c] / |A_c|
+__global__ __launch_bounds__(256) void k_ct_scale(double2 *P, const double2 *A, uint64_t dim, int k,
+                                                  const double *state) {
+  const uint64_t n = dim * (uint64_t)k;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double s = state[16 + (int)(i % (uint64_t)k)];
+    const double2 a = A[i];
+    P[i] = make_double2(a.x * s, a.y * s);
+  }
+}
+
+// per column: |h|^2 and the moments of |p|^2 (kinds as k_tpq_fused)
+__global__ __launch_bounds__(TQ_TPB) void k_ct_moments(const double2 *P, const double2 *Hp, uint64_t dim, int k,
+                                                       int obs_kind, int nsite, const uint64_t *list1,
+                                                       double *partial) {
+  const int c = threadIdx.x % k, jl = threadIdx.x / k, spb = TQ_TPB / k;
+  double r[TQ_NRED] = {};
+  const uint64_t EVEN = 0x5555555555555555ull;
+  for (uint64_t j = (uint64_t)blockIdx.x * spb + jl; j < dim; j += (uint64_t)gridDim.x * spb) {
+    const double2 w = P[j * k + c], h = Hp[j * k + c];
+    r[0] += h.x * h.x + h.y * h.y;
+    if (obs_kind) {
+      const double ww = w.x * w.x + w.y * w.y;
+      const uint64_t x = list1 ? list1[j] : j;
+      double S;
+      if (obs_kind == 1) {
+        S = (double)(2 * (int)__popcll(x) - nsite);
+      } else {
+        const double D = (double)(int)__popcll(x & (x >> 1) & EVEN);
+        const int up = (int)__popcll(x & EVEN), dn = (int)__popcll(x & (EVEN << 1));
+        const double N = (double)(up + dn);
+        S = (double)(up - dn);
+        r[1] += ww * D;
+        r[2] += ww * D * D;
+        r[3] += ww * N;
+        r[4] += ww * N * N;
+      }
+      r[5] += ww * S;
+      r[6] += ww * S * S;
+    }
+  }
+  __shared__ double red[TQ_TPB / 64][16][TQ_NRED];
+#pragma unroll
+  for (int q = 0; q < TQ_NRED; q++) {
+    double s = r[q];
+    for (int off = 32; off >= k; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) < k) red[threadIdx.x >> 6][threadIdx.x & 63][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < k * TQ_NRED) {
+    const int cc = threadIdx.x / TQ_NRED, q = threadIdx.x % TQ_NRED;
+    double t = 0.0;
+    for (int w = 0; w < TQ_TPB / 64; w++) t += red[w][cc][q];
+    partial[((uint64_t)blockIdx.x * k + cc) * TQ_NRED + q] = t;
+  }
+}
+
+// row s of sample c (the mTPQ row layout): inv_temp (s additions of dtau), E, <H^2>, the
+// norm before normalisation (step 0: the start vector's), the moments
+__global__ void k_ct_finalize(const double *partial, int nblk, int k, const double2 *prdct, const double *state,
+                              int s, double dtau, int obs_kind, int nsite, double sz_const, double *out) {
+  const int c = threadIdx.x;
+  if (c >= k) return;
+  double r[TQ_NRED] = {};
+  for (int b = 0; b < nblk; b++)
+    for (int q = 0; q < TQ_NRED; q++) r[q] += partial[((uint64_t)b * k + c) * TQ_NRED + q];
+  double *o = out + ((uint64_t)s * k + c) * TQ_NOUT;
+  double it = 0.0;  // inv_temp += delta_tau once per step (CalcByCanonicalTPQ.c:227)
+  for (int i = 0; i < s; i++) it += dtau;
+  o[0] = it;
+  o[1] = prdct[c].x;
+  o[2] = r[0];
+  o[3] = state[c];
+  if (obs_kind == 2) {
+    o[4] = r[1];
+    o[5] = r[2];
+    o[6] = r[3];
+    o[7] = r[4];
+  } else {
+    o[4] = o[5] = 0.0;
+    o[6] = (double)nsite;
+    o[7] = (double)nsite * nsite;
+  }
+  o[8] = obs_kind ? 0.5 * r[5] : sz_const;
+  o[9] = obs_kind ? 0.25 * r[6] : sz_const * sz_const;
+  o[10] = prdct[c].y;
+  o[11] = 0.0;
+}
+
+extern "C" int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out_host, hphi_tpq_result *r) {
+  if (!e || !p || !out_host) return hphi_set_error("hphi_ctpq: NULL argument");
+  const int k = p->k;
+  if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return hphi_set_error("hphi_ctpq: k must be 1, 2, 4, 8 or 16");
+  if (p->steps < 1) return hphi_set_error("hphi_ctpq: steps must be positive");
+  const int nc = p->expand_coef > 0 ? p->expand_coef : 10;  // CalcByCanonicalTPQ.c:71-76
+  if (hphi_build_tables(e) || hphi_build_diag(e)) return -1;
+  const uint64_t dim = e->dim, n = dim * (uint64_t)k;
+  double lv = p->large_value;
+  if (lv <= 0.0) {
+    double m = 0.0;
+    if (hphi_diag_maxabs(e, &m)) return -1;
+    lv = 3.0 * m / (double)e->nsite;
+  }
+  const double dtau = 1.0 / lv;  // CalcByCanonicalTPQ.c:143
+  const int obs_kind = e->model == HPHI_MODEL_SPINGC ? 1 : hphi_model_hubbard(e->model) ? 2 : 0;
+  const double sz_const = 0.5 * (double)(e->nup - e->ndown);
+  const uint64_t *list1 = hphi_model_gc(e->model) ? (const uint64_t *)nullptr : (const uint64_t *)e->d_list1;
+  const int g = tq_grid(dim, k);
+  const unsigned gs = (unsigned)std::min<uint64_t>((n + 255) / 256, 65536);
+  const size_t vbytes = sizeof(double2) * n;
+  double2 *P = nullptr, *Hp = nullptr, *A = nullptr, *Cv = nullptr;
+  double *state = nullptr, *part = nullptr, *out = nullptr;
+  int rc = 0;
+  hipEvent_t ev0, ev1;
+  (void)hipEventCreate(&ev0);
+  (void)hipEventCreate(&ev1);
+  do {
+    if (hipMalloc(&P, vbytes) != hipSuccess || hipMalloc(&Hp, vbytes) != hipSuccess ||
+        hipMalloc(&A, vbytes) != hipSuccess || hipMalloc(&Cv, vbytes) != hipSuccess ||
+        hipMalloc(&state, sizeof(double) * 64) != hipSuccess ||
+        hipMalloc(&part, sizeof(double) * (size_t)TQ_GRID * k * TQ_NRED) != hipSuccess ||
+        hipMalloc(&out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT) != hipSuccess) {
+      rc = hphi_set_error("hphi_ctpq: device allocation failed");
+      break;
+    }
+    if (hipEventRecord(ev0, e->stream) != hipSuccess) { rc = hphi_set_error("event record failed"); break; }
+    // step 0: start vector (MakeIniVec), normalised, measured (CalcByCanonicalTPQ.c:140-175)
+    hipLaunchKernelGGL(k_tpq_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, A, dim, k,
+                       (uint64_t)k, (uint64_t)1, p->seed, p->init_kind);
+    for (int s = 0; s < p->steps && !rc; s++) {
+      if (s > 0) {  // A = sum_{m <= nc} (-dtau/2)^m / m! H^m P, with H P = Hp (MultiplyForCanonicalTPQ)
+        double t = -0.5 * dtau;
+        hipLaunchKernelGGL(k_ct_acc0, dim3(gs), dim3(256), 0, e->stream, A, (const double2 *)P,
+                           (const double2 *)Hp, n, t);
+        double2 *cur = Hp, *nxt = Cv;
+        for (int m = 2; m <= nc && !rc; m++) {
+          t *= (-0.5 * dtau) / (double)m;
+          if (hphi_hv_launch_ex(e, k, (double *)nxt, (const double *)cur, 1)) { rc = -1; break; }
+          hipLaunchKernelGGL(k_ct_acc, dim3(gs), dim3(256), 0, e->stream, A, (const double2 *)nxt, n, t);
+          std::swap(cur, nxt);
+        }
+        if (rc) break;
+      }
+      // normalise A into P (global_norm; step 0: global_1st_norm)
+      hipLaunchKernelGGL(k_tpq_colnorm, dim3(g), dim3(TQ_TPB), 0, e->stream, (const double2 *)A, dim, k, (uint64_t)k,
+                         (uint64_t)1, part);
+      hipLaunchKernelGGL(k_ct_norms, dim3(1), dim3(64), 0, e->stream, (const double *)part, g, k, state);
+      hipLaunchKernelGGL(k_ct_scale, dim3(gs), dim3(256), 0, e->stream, P, (const double2 *)A, dim, k,
+                         (const double *)state);
+      // measure: Hp = H P (its prdct is E), <H^2> and the moments (expec_energy_flct)
+      if (hphi_hv_launch_ex(e, k, (double *)Hp, (const double *)P, 1)) { rc = -1; break; }
+      hipLaunchKernelGGL(k_ct_moments, dim3(g), dim3(TQ_TPB), 0, e->stream, (const double2 *)P, (const double2 *)Hp,
+                         dim, k, obs_kind, e->nsite, list1, part);
+      hipLaunchKernelGGL(k_ct_finalize, dim3(1), dim3(64), 0, e->stream, (const double *)part, g, k,
+                         (const double2 *)e->d_prdct, (const double *)state, s, dtau, obs_kind, e->nsite, sz_const,
+                         out);
+      if (hipGetLastError() != hipSuccess) { rc = hphi_set_error("hphi_ctpq: launch failed"); break; }
+    }
+    if (rc) break;
+    if (hipEventRecord(ev1, e->stream) != hipSuccess || hipStreamSynchronize(e->stream) != hipSuccess) {
+      rc = hphi_set_error("hphi_ctpq: stream sync failed");
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    if (hipMemcpy(out_host, out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT, hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      rc = hphi_set_error("hphi_ctpq: copy of results failed");
+      break;
+    }
+    if (r) {
+      r->seconds = ms * 1e-3;
+      r->large_value = lv;
+      r->steps = p->steps;
+    }
+  } while (0);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  (void)hipFree(P);
+  (void)hipFree(Hp);
+  (void)hipFree(A);
+  (void)hipFree(Cv);
+  (void)hipFree(state);
+  (void)hipFree(part);
+  (void)hipFree(out);
+  return rc;
+}

@@ -52,6 +52,7 @@ CALC_LANCZOS = 0
 CALC_TPQ = 1
 CALC_FULLDIAG = 2
 CALC_CG = 3
+CALC_CTPQ = 5
 
 _MODEL_NAMES = {HUBBARD: "Hubbard", SPIN: "Spin", HUBBARD_GC: "HubbardGC", SPIN_GC: "SpinGC"}
 
@@ -378,6 +379,8 @@ def write_expert(spec: ModelSpec, directory: str, calc_type: Optional[int] = Non
     for k in ("Lanczos_max", "initial_iv", "exct", "LanczosEps", "LanczosTarget",
               "LargeValue", "NumAve", "ExpecInterval"):
         lines.append(f"{k} {_f(spec.params[k]) if k == 'LargeValue' else int(spec.params[k])}")
+    if "ExpandCoef" in spec.params:  # canonical TPQ's Taylor order (0: the default 10)
+        lines.append(f"ExpandCoef {int(spec.params['ExpandCoef'])}")
     put("ModPara", "modpara.def", "\n".join(lines) + "\n")
 
     loc = 1 if spec.model in (SPIN, SPIN_GC) else 0

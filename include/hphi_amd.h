@@ -251,6 +251,9 @@ int hphi_write_energy(const char *path, const hphi_phys *p);
 int hphi_write_cisajs(const char *path, int n, const int *idx, const double *val);
 int hphi_write_cisajscktalt(const char *path, int model, int n, const int *idx, const double *val);
 int hphi_write_tpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride);
+/* The same files for canonical TPQ rows (hphi_ctpq), which also write a
+ * Flct_rand line at step 0 (CalcByCanonicalTPQ.c:160-200). */
+int hphi_write_ctpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride);
 /* Lowest `want` distinct eigenvalues of the plain-Lanczos tridiagonal
  * (alpha[0..m), beta[0..m-1)) after Cullum-Willoughby removal of spurious
  * values and merging of ghost copies; returns how many were found. */
@@ -279,6 +282,20 @@ typedef struct {
   int steps;
 } hphi_tpq_result;
 int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out, hphi_tpq_result *r);
+/* Canonical TPQ (CalcByCanonicalTPQ, src/CalcByCanonicalTPQ.c:52-300) for k
+ * samples at once, state-major with the block H·v.  Each step s >= 1 applies
+ * exp(-dtau H/2) to the normalised psi by its Taylor series to order
+ * expand_coef (MultiplyForCanonicalTPQ, src/Multiply.c:152-205; 0 means the
+ * reference's default 10), normalises and measures; dtau = 1/l with l as in
+ * hphi_tpq.  Rows as hphi_tpq with q0 = inv_temp = s dtau and q3 = the norm
+ * before normalisation (global_norm; step 0: global_1st_norm).  Costs
+ * expand_coef block H·v per step. */
+typedef struct {
+  int k, steps, init_kind, expand_coef;
+  double large_value;
+  uint64_t seed;
+} hphi_ctpq_params;
+int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out, hphi_tpq_result *r);
 /* max_j |list_Diagonal[j]| */
 int hphi_diag_maxabs(hphi_engine *e, double *out);
 

@@ -25,6 +25,12 @@
  *   mode tpq    : the reference's CalcByTPQ (src/CalcByTPQ.c:52-294) as HPhiMain.c:315-321
  *                 calls it (input written with CalcType 1); it writes output/SS_rand*.dat,
  *                 Norm_rand*.dat, Flct_rand*.dat
+ *   mode ctpq   : the same for CalcByCanonicalTPQ (src/CalcByCanonicalTPQ.c:49-300,
+ *                 HPhiMain.c:323-329; CalcType 5)
+ *                 Both TPQ modes first write each sample's start vector as the
+ *                 reference makes it in this process (MakeIniVec(rand_i), which
+ *                 depends on the thread count): ini_rand<i>.bin = global_1st_norm
+ *                 (double) then v1[1..dim] (complex128, normalised)
  * The input directory is entered with chdir, as the reference opens its def
  * files by relative name and writes ./output/.
  */
@@ -37,7 +43,9 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include "CalcByCanonicalTPQ.h"
 #include "CalcByTPQ.h"
+#include "MakeIniVec.h"
 #include "Common.h"
 #include "HPhiTrans.h"
 #include "check.h"
@@ -136,8 +144,23 @@ int main(int argc, char *argv[]) {
     return 0;
   }
 
-  if (strcmp(mode, "tpq") == 0) {
-    if (X.Bind.Def.iCalcType != TPQCalc) return 7;
+  if (strcmp(mode, "tpq") == 0 || strcmp(mode, "ctpq") == 0) {
+    const int canonical = strcmp(mode, "ctpq") == 0;
+    if (X.Bind.Def.iCalcType != (canonical ? cTPQ : TPQCalc)) return 7;
+    for (int i = 0; i < NumAve; i++) {
+      if (MakeIniVec(i, &(X.Bind)) != 0) return 7;
+      char name[64];
+      snprintf(name, sizeof name, "ini_rand%d.bin", i);
+      char path[4200];
+      snprintf(path, sizeof path, "%s/%s", outdir, name);
+      FILE *f = fopen(path, "wb");
+      if (!f) return 4;
+      size_t w = fwrite(&global_1st_norm, sizeof(double), 1, f);
+      w += fwrite(v1 + 1, sizeof(double complex), dim, f);
+      fclose(f);
+      if (w != dim + 1) return 4;
+    }
+    if (canonical) return CalcByCanonicalTPQ(NumAve, X.Bind.Def.Param.ExpecInterval, &X) == TRUE ? 0 : 7;
     return CalcByTPQ(NumAve, X.Bind.Def.Param.ExpecInterval, &X) == TRUE ? 0 : 7;
   }
 

@@ -180,3 +180,19 @@ def tpq_file_cases():
         s.name = name
         out[name] = s
     return out
+
+
+def ctpq_file_cases():
+    """Small models whose canonical TPQ output files the reference writes: one
+    per observable kind, the default Taylor order (10) and a lower one (4)."""
+    out = {}
+    for name, s, coef in (("ctpq_hubbard_chain4", L.hubbard_chain(4), 0), ("ctpq_chain8", L.heisenberg_chain(8), 4),
+                          ("ctpq_tri_3x2_gc", L.triangular_xxz_gc(3, 2, 1.0, 0.8, 0.1), 0),
+                          ("ctpq_hubbard_gc", hubbard_gc(), 0)):
+        s.params["Lanczos_max"] = 12
+        s.params["NumAve"] = 2
+        s.params["LargeValue"] = 4.0
+        s.params["ExpandCoef"] = coef
+        s.name = name
+        out[name] = s
+    return out

@@ -24,8 +24,11 @@ the normalised synthetic vector.
 
 Reference-written output files (pass --files) go to tests/golden/files/<case>/:
 the Green's-function files of the green cases and the SS_rand / Norm_rand /
-Flct_rand files of tpq_file_cases() (the reference's own CalcByTPQ); the file
-writers are checked byte for byte against them.
+Flct_rand files of tpq_file_cases() (the reference's own CalcByTPQ) and of
+ctpq_file_cases() (CalcByCanonicalTPQ), with each sample's start vector as the
+reference made it (ini_rand<i>.bin: global_1st_norm, then the normalised v1);
+the file writers are checked byte for byte against them and the TPQ
+restatements against their numbers.
 
 usage: python tests/golden/gen_golden.py [--large | --two-body | --green | --files]
 """
@@ -47,7 +50,7 @@ sys.path.insert(0, HERE)
 
 from hphi_amd import lattice as L  # noqa: E402
 from oracle import oracle as O  # noqa: E402
-from cases import SEED, golden_cases, green_cases, tpq_file_cases, two_body_cases  # noqa: E402
+from cases import SEED, ctpq_file_cases, golden_cases, green_cases, tpq_file_cases, two_body_cases  # noqa: E402,E501
 import shutil  # noqa: E402
 
 FULL_MAX = 5000
@@ -129,7 +132,8 @@ def keep_files(name, src_dir, names):
 
 def run_files(work):
     """Reference-written output files: the Green's-function files of the
-    green cases and the TPQ files of tpq_file_cases() (CalcByTPQ)."""
+    green cases and the TPQ files of tpq_file_cases() (CalcByTPQ) and
+    ctpq_file_cases() (CalcByCanonicalTPQ)."""
     rec = {}
     for name, spec in green_cases().items():
         d = os.path.join(work, "f_" + name)
@@ -137,12 +141,18 @@ def run_files(work):
         O.run_reference(d, "green", d + "/out", seed=SEED)
         rec[name] = {"model": spec.model,
                      "files": keep_files(name, d + "/output", ["zvo_cisajs.dat", "zvo_cisajscktalt.dat"])}
-    for name, spec in tpq_file_cases().items():
-        d = os.path.join(work, "f_" + name)
-        L.write_expert(spec, d, calc_type=L.CALC_TPQ)
-        O.run_reference(d, "tpq", d + "/out")
-        names = [f"{k}_rand{i}.dat" for i in range(int(spec.params["NumAve"])) for k in ("SS", "Norm", "Flct")]
-        rec[name] = {"model": spec.model, "files": keep_files(name, d + "/output", names)}
+    for cases, ct, mode in ((tpq_file_cases(), L.CALC_TPQ, "tpq"), (ctpq_file_cases(), L.CALC_CTPQ, "ctpq")):
+        for name, spec in cases.items():
+            d = os.path.join(work, "f_" + name)
+            L.write_expert(spec, d, calc_type=ct)
+            O.run_reference(d, mode, d + "/out")
+            nave = int(spec.params["NumAve"])
+            names = [f"{k}_rand{i}.dat" for i in range(nave) for k in ("SS", "Norm", "Flct")]
+            kept = keep_files(name, d + "/output", names)
+            kept += keep_files(name, d + "/out", [f"ini_rand{i}.bin" for i in range(nave)])
+            rec[name] = {"model": spec.model, "files": kept, "dim": spec.dim, "nsite": spec.nsite,
+                         "large_value": float(spec.params["LargeValue"]),
+                         "expand_coef": int(spec.params.get("ExpandCoef", 0)) or 10}
     return rec
 
 

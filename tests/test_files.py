@@ -61,13 +61,16 @@ def _tpq_rows(d, i):
     return rows
 
 
-@pytest.mark.parametrize("name", sorted(k for k in GOLD if k.startswith("tpq_")))
+@pytest.mark.parametrize("name", sorted(k for k in GOLD if k.startswith(("tpq_", "ctpq_"))))
 def test_tpq_files_byte_identical(tmp_path, name):
+    """SS_rand / Norm_rand / Flct_rand as the reference's CalcByTPQ (tpq_*) and
+    CalcByCanonicalTPQ (ctpq_*, with a Flct line at step 0) wrote them."""
     d = os.path.join(FILES, name)
     nsamp = sum(1 for f in GOLD[name]["files"] if f.startswith("SS_rand"))
     assert nsamp >= 1
+    write = capi.write_ctpq_files if name.startswith("ctpq_") else capi.write_tpq_files
     for i in range(nsamp):
-        capi.write_tpq_files(str(tmp_path), i, _tpq_rows(d, i))
+        write(str(tmp_path), i, _tpq_rows(d, i))
         for k in ("SS", "Norm", "Flct"):
             f = f"{k}_rand{i}.dat"
             assert _read(str(tmp_path / f)) == _read(os.path.join(d, f)), f

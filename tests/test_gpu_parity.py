@@ -281,6 +281,54 @@ def test_tpq_matches_restated_reference_step(A, oracle_lib, name, k, layout, gra
     eng.close()
 
 
+def _state_moments(spec, orc):
+    """Per-state doublon count D, electron number N and 2 Sz of the oracle's basis."""
+    E = 0x5555555555555555
+    xs = [int(x) for x in orc.list_1[1:]] if spec.model in (L.SPIN, L.HUBBARD) else list(range(orc.dim))
+    if spec.model in (L.SPIN, L.SPIN_GC):
+        D = np.zeros(len(xs))
+        N = np.full(len(xs), float(spec.nsite))
+        S = np.array([2 * bin(x).count("1") - spec.nsite for x in xs], float)
+    else:
+        D = np.array([bin(x & (x >> 1) & E).count("1") for x in xs], float)
+        up = np.array([bin(x & E).count("1") for x in xs], float)
+        dn = np.array([bin(x & (E << 1)).count("1") for x in xs], float)
+        N, S = up + dn, up - dn
+    return D, N, S
+
+
+@pytest.mark.parametrize("name,k,coef", [("tri_4x3_gc", 4, 0), ("hubbard_4x2", 2, 0), ("j1j2_4x4", 1, 4),
+                                         ("chain16", 16, 10), ("hubbard_chain6_complex", 2, 6)])
+def test_ctpq_matches_restated_reference_step(A, oracle_lib, name, k, coef):
+    """Canonical TPQ rows (inv_temp, E, <H^2>, norm, moments) equal the
+    reference's step (CalcByCanonicalTPQ, MultiplyForCanonicalTPQ) restated in
+    numpy on the oracle's H·v, from the same start vectors."""
+    from tests.tpq_restated import ctpq_restated
+    print(f"[gpu] ctpq {name} k={k} coef={coef}", flush=True)
+    spec = golden_cases()[name]
+    eng = A.Engine.from_spec(spec)
+    steps = 15
+    out, res = eng.ctpq(k, steps, seed=3, init_kind=0, expand_coef=coef)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    lv = 3.0 * np.max(np.abs(orc.diag[1:])) / spec.nsite
+    assert abs(res.large_value - lv) <= 1e-15 * lv
+    D, N, S = _state_moments(spec, orc)
+
+    def obs(v):
+        w = np.abs(v) ** 2
+        return (w @ D, w @ D ** 2, w @ N, w @ N ** 2, 0.5 * (w @ S), 0.25 * (w @ S ** 2))
+    hv = lambda v: orc.hv(v)[0]  # noqa: E731
+    for c in range(k):
+        ref = ctpq_restated(hv, oracle_lib.fill_vector(3 + c, orc.dim), steps, 1.0 / lv, coef or 10, obs)
+        got = out[:, c]
+        assert np.allclose(got[:, :4], ref[:, :4], rtol=1e-10, atol=1e-12), (got[:3, :4], ref[:3, :4])
+        assert np.allclose(got[:, 4:10], ref[:, 4:10], rtol=1e-10, atol=1e-12), (got[:3, 4:10], ref[:3, 4:10])
+        assert np.all(np.abs(got[:, 10]) <= 1e-10 * np.abs(got[:, 1]) + 1e-12)
+    # the energy falls as beta grows (imaginary-time evolution from beta = 0)
+    assert np.all(out[-1, :, 1] < out[0, :, 1])
+    eng.close()
+
+
 def test_tpq_gaussian_start_is_normalised(A):
     spec = golden_cases()["tri_4x3_gc"]
     eng = A.Engine.from_spec(spec)
