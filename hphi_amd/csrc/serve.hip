@@ -107,18 +107,29 @@ __device__ __forceinline__ int find_slab(CU64 prefix, int n, uint64_t tile) {
   return lo;
 }
 
+// XCD-contiguous chunk order.  Workgroups are dealt to the 8 XCDs round-robin
+// by blockIdx, so workgroup b of a grid of G (a multiple of 8) takes logical
+// chunk (b % 8) G/8 + b / 8: each XCD sweeps one contiguous range of chunks,
+// and the gathers of neighbouring states meet in that XCD's own L2.  Logical
+// chunks at or past the launch's total (the rounding) have no work.
+__device__ __forceinline__ uint64_t sl_chunk() {
+  return (uint64_t)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+}
+
 // pass 1: w = r - alpha_{m-1} s_{m-1} q over R per running column, |w|^2 per
 // chunk of SL_TPW tiles.  Lane (state, column) = (base + t / SL_K, t % SL_K), so
 // every load and store of a wave covers one contiguous range of rows.
 __global__ __launch_bounds__(SV_TPB) void k_sl_resid(const SlabDev *slabs, const uint64_t *prefix, int ns,
                                                      const double *js, double *partial) {
-  const int sb = find_slab((CU64)prefix, ns, blockIdx.x);
+  const uint64_t chunk = sl_chunk();
+  if (chunk >= ((CU64)prefix)[ns]) return;
+  const int sb = find_slab((CU64)prefix, ns, chunk);
   CSlab S = (CSlab)slabs + sb;
   const int c = threadIdx.x % SL_K;
   const bool run = (S->mask >> c) & 1;
   const int sl = S->slot[c];
   const double f = js[sl * JS_STRIDE + 0] * js[sl * JS_STRIDE + 1];
-  const uint64_t e0 = (blockIdx.x - ((CU64)prefix)[sb]) * SL_TPW * SV_TPB * SL_K + threadIdx.x;
+  const uint64_t e0 = (chunk - ((CU64)prefix)[sb]) * SL_TPW * SV_TPB * SL_K + threadIdx.x;
   const uint64_t n = S->dim * SL_K, cend = e0 - threadIdx.x + (uint64_t)SL_TPW * SV_TPB * SL_K;
   const uint64_t eend = n < cend ? n : cend;
   double2 *__restrict__ R = S->R;
@@ -143,7 +154,7 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_resid(const SlabDev *slabs, const
   if (threadIdx.x < SL_K) {
     double t = 0.0;
     for (int w = 0; w < SV_TPB / 64; w++) t += red[w][threadIdx.x];
-    partial[(uint64_t)blockIdx.x * SL_K + threadIdx.x] = t;
+    partial[chunk * SL_K + threadIdx.x] = t;
   }
 }
 
@@ -330,7 +341,9 @@ __device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t 
 __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const uint64_t *prefix, int ns,
                                                   const double *js, double2 *partial) {
   extern __shared__ uint16_t s21[];
-  const int sb = find_slab((CU64)prefix, ns, blockIdx.x);
+  const uint64_t chunk = sl_chunk();
+  if (chunk >= ((CU64)prefix)[ns]) return;
+  const int sb = find_slab((CU64)prefix, ns, chunk);
   CSlab S = (CSlab)slabs + sb;
   const uint32_t n21 = (uint32_t)S->size21;
   for (uint32_t i = threadIdx.x; i < n21; i += SV_TPB) s21[i] = (uint16_t)S->l21[i];
@@ -338,7 +351,7 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const ui
   double pre[SL_K], pim[SL_K];
 #pragma unroll
   for (int c = 0; c < SL_K; c++) pre[c] = pim[c] = 0.0;
-  const uint64_t t0 = (blockIdx.x - ((CU64)prefix)[sb]) * SL_TPW;
+  const uint64_t t0 = (chunk - ((CU64)prefix)[sb]) * SL_TPW;
   const uint64_t dim = S->dim;
   for (int t = 0; t < SL_TPW; t++) {
     const uint64_t j = (t0 + t) * SV_TPB + threadIdx.x;
@@ -364,7 +377,7 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const ui
       s.x += red[w][threadIdx.x].x;
       s.y += red[w][threadIdx.x].y;
     }
-    partial[(uint64_t)blockIdx.x * SL_K + threadIdx.x] = s;
+    partial[chunk * SL_K + threadIdx.x] = s;
   }
 }
 
@@ -979,7 +992,7 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
       const SlabDev *dsl = s->d_slabs + s_sd0[st];
       const ColDev *dcl = s->d_cols + s_cd0[st];
       const uint64_t *dpf = s->d_prefix + s_pf0[st];
-      const unsigned T = (unsigned)prefix[s_pf0[st] + ns];
+      const unsigned T = (unsigned)((prefix[s_pf0[st] + ns] + 7) / 8 * 8);  // grid of sl_chunk()
       double *out = s->d_out + (size_t)st * 2 * n;
       hipLaunchKernelGGL(k_sl_resid, dim3(T), dim3(SV_TPB), 0, s->stream, dsl, dpf, ns, (const double *)s->d_js,
                          s->d_part);
