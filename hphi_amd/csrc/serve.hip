@@ -71,7 +71,7 @@
 struct SlabDev {
   uint64_t dim;
   const uint64_t *list1;
-  const uint32_t *l21;
+  const uint4 *l21p;  // list_2_1 as uint16, zero-padded to a multiple of 8 entries
   const uint64_t *l22;
   const GTerm *terms;   // term structure of the slab (couplings in coef)
   const double2 *coef;  // [n_terms][SL_K]
@@ -340,13 +340,14 @@ __device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t 
 // for the running columns; <w_c|H w_c> per chunk
 __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const uint64_t *prefix, int ns,
                                                   const double *js, double2 *partial) {
-  extern __shared__ uint16_t s21[];
+  extern __shared__ uint4 s21v[];
+  const uint16_t *s21 = (const uint16_t *)s21v;
   const uint64_t chunk = sl_chunk();
   if (chunk >= ((CU64)prefix)[ns]) return;
   const int sb = find_slab((CU64)prefix, ns, chunk);
   CSlab S = (CSlab)slabs + sb;
-  const uint32_t n21 = (uint32_t)S->size21;
-  for (uint32_t i = threadIdx.x; i < n21; i += SV_TPB) s21[i] = (uint16_t)S->l21[i];
+  const uint32_t n8 = (uint32_t)((S->size21 + 7) / 8);  // 16-byte rows of the packed table
+  for (uint32_t i = threadIdx.x; i < n8; i += SV_TPB) s21v[i] = S->l21p[i];
   __syncthreads();
   double pre[SL_K], pim[SL_K];
 #pragma unroll
@@ -595,6 +596,34 @@ struct Job {
   hphi_job_result res{};
 };
 
+// The engine pads each term group (low, high, mixed) to the plain kernels'
+// batch; the slab kernel reads SL_U terms per batch, so its groups keep only
+// the padding that batch needs.  A padding term never matches (P_EQ with an
+// empty mask and a nonzero value); no compiled term has that form.
+void slab_repad(hphi_engine &h) {
+  const int n[3] = {h.n_low, h.n_high, h.n_mixed};
+  std::vector<GTerm> out;
+  int m[3], at = 0;
+  for (int g = 0; g < 3; g++) {
+    const size_t start = out.size();
+    for (int t = at; t < at + n[g]; t++) {
+      const GTerm &x = h.h_terms_dev[t];
+      if (!(x.kind == P_EQ && x.tmask == 0 && x.tval != 0)) out.push_back(x);
+    }
+    GTerm z{};
+    z.kind = P_EQ;
+    z.tmask = 0;
+    z.tval = 1;
+    while ((out.size() - start) % SL_U) out.push_back(z);
+    m[g] = (int)(out.size() - start);
+    at += n[g];
+  }
+  h.h_terms_dev.swap(out);
+  h.n_low = m[0];
+  h.n_high = m[1];
+  h.n_mixed = m[2];
+}
+
 // Jobs share a slab when basis and term structure agree (couplings may differ).
 bool same_structure(const hphi_engine &a, const hphi_engine &b) {
   if (a.model != b.model || a.nsite != b.nsite || a.nup != b.nup || a.ndown != b.ndown || a.dim != b.dim ||
@@ -614,6 +643,7 @@ struct Slab {
   size_t off = 0, bytes = 0;
   uint64_t *list1 = nullptr, *l22 = nullptr, *jb = nullptr;
   uint32_t *l21 = nullptr;
+  uint16_t *l21p = nullptr;  // list_2_1 packed for the LDS copy of every workgroup
   GTerm *terms = nullptr;
   double2 *coef = nullptr;
   double *diag = nullptr;
@@ -631,7 +661,7 @@ struct Slab {
 
 size_t slab_footprint(const hphi_engine &h, int max_iter) {
   const size_t nt = std::max<size_t>(h.h_terms_dev.size(), 1), nd = std::max<size_t>(h.h_dterms.size(), 1);
-  return al256(8 * h.dim) + al256(4 * h.size21) + al256(8 * h.size22) + al256(8 * (h.sdim + 1)) +
+  return al256(8 * h.dim) + al256(4 * h.size21) + al256(16 * ((h.size21 + 7) / 8)) + al256(8 * h.size22) + al256(8 * (h.sdim + 1)) +
          al256(sizeof(GTerm) * nt) + al256(16 * SL_K * nt) + al256(8 * SL_K * h.dim) + 2 * al256(16 * SL_K * h.dim) +
          al256(sizeof(DTerm) * SL_K * nd) + al256(8 * SL_K * 2 * (size_t)(max_iter + 2));
 }
@@ -748,6 +778,7 @@ extern "C" int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint
     delete j;
     return hphi_set_error("the service runs states of at most 32 bits");
   }
+  slab_repad(j->h);
   for (const GTerm &t : j->h.h_terms_dev)
     if (t.cim != 0.0) j->realc = 0;
   const size_t fp = slab_footprint(j->h, s->p.max_iter);
@@ -772,6 +803,12 @@ static void *stage(hphi_server *s, const void *src, size_t n) {
   return p;
 }
 
+// list_2_1 as uint16 (rank + 2 < 2^16 for <= 16 low bits), zero-padded to n entries
+__global__ void k_pack21(const uint32_t *l21, uint64_t size21, uint64_t n, uint16_t *out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = i < size21 ? (uint16_t)l21[i] : (uint16_t)0;
+}
+
 // a new slab with the job's structure: tables on the device, terms uploaded;
 // returns 1 when the staging buffer is full (retry next tick)
 static int slab_create(hphi_server *s, const Job *j, size_t off, size_t bytes, Slab **out) {
@@ -786,6 +823,7 @@ static int slab_create(hphi_server *s, const Job *j, size_t off, size_t bytes, S
   char *p = s->d_arena + off;
   b->list1 = (uint64_t *)p, p += al256(8 * h.dim);
   b->l21 = (uint32_t *)p, p += al256(4 * h.size21);
+  b->l21p = (uint16_t *)p, p += al256(16 * ((h.size21 + 7) / 8));
   b->l22 = (uint64_t *)p, p += al256(8 * h.size22);
   b->jb = (uint64_t *)p, p += al256(8 * (h.sdim + 1));
   b->terms = (GTerm *)p, p += al256(sizeof(GTerm) * nt);
@@ -804,6 +842,12 @@ static int slab_create(hphi_server *s, const Job *j, size_t off, size_t bytes, S
   }
   const SzParams P = hphi_sz_params(&h);
   if (!rc && hphi_launch_tables(P, h.sdim, h.size21, b->jb, b->l21, b->l22, b->list1, s->stream)) rc = -1;
+  if (!rc) {
+    const uint64_t n = (h.size21 + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_pack21, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, b->l21, h.size21, n,
+                       b->l21p);
+    if (hipGetLastError() != hipSuccess) rc = hphi_set_error("slab table packing failed");
+  }
   if (rc) {
     delete b;
     return -1;
@@ -949,7 +993,7 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
         if (!d.mask) continue;
         d.dim = h.dim;
         d.list1 = b->list1;
-        d.l21 = b->l21;
+        d.l21p = (const uint4 *)b->l21p;
         d.l22 = b->l22;
         d.terms = b->terms;
         d.coef = b->coef;
@@ -979,7 +1023,7 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
       tiles = std::max<uint64_t>(tiles, prefix.back());
     }
     if (tiles > s->max_tiles) return hphi_set_error("too many tiles in flight");
-    const size_t lds = 2 * (size_t)max21;  // list_2_1 as uint16 (rank + 2 < 2^16 for <= 16 low bits)
+    const size_t lds = 16 * (size_t)((max21 + 7) / 8);  // list_2_1 as uint16 (rank + 2 < 2^16 for <= 16 low bits)
     if (lds > 65536)
       (void)hipFuncSetAttribute((const void *)k_sl_hv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     HIPCHK(hipMemcpyAsync(s->d_slabs, sd.data(), sizeof(SlabDev) * sd.size(), hipMemcpyHostToDevice, s->stream));
