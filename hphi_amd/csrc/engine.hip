@@ -383,6 +383,9 @@ static void classify_and_upload_terms(hphi_engine *e) {
     z.tval = 1;  // never active
     while (v.size() % HV_TERM_PAD) v.push_back(z);
   };
+  e->r_low = (int)low.size();
+  e->r_high = (int)high.size();
+  e->r_mixed = (int)mixed.size();
   pad(low);
   pad(high);
   pad(mixed);
@@ -1013,6 +1016,9 @@ HvArgs hphi_hv_args(hphi_engine *e, int k, double *v0, const double *v1) {
   A.n_low = e->n_low;
   A.n_high = e->n_high;
   A.n_mixed = e->n_mixed;
+  A.e_low = e->n_low;
+  A.e_high = e->n_high;
+  A.e_mixed = e->n_mixed;
   A.irght = e->irght;
   A.nlow = e->nlow;
   A.v0 = v0;

@@ -81,7 +81,8 @@ struct HvArgs {
   const uint64_t *l22;
   const double *diag;
   const GTerm *terms;
-  int n_low, n_high, n_mixed;
+  int n_low, n_high, n_mixed;  // group sizes (padded): the groups' offsets
+  int e_low, e_high, e_mixed;  // loop extents: terms up to the launch's batch (<= the group sizes)
   int nlow;
   uint64_t irght;
   double *v0;
@@ -131,6 +132,7 @@ struct hphi_engine {
   std::vector<GTerm> h_terms, h_terms_dev;
   std::vector<DTerm> h_dterms;
   int n_low = 0, n_high = 0, n_mixed = 0;
+  int r_low = 0, r_high = 0, r_mixed = 0;  // compiled terms per group, before padding
   void *d_partial = nullptr, *d_prdct = nullptr, *h_prdct = nullptr, *d_coef = nullptr;
   double *d_stage0 = nullptr, *d_stage1 = nullptr;
   uint64_t bytes = 0;

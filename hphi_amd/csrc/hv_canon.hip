@@ -100,7 +100,7 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
     const int64_t b21 = r21<RK>(A, s21, ia);
 
     // ---- low-half terms: target in the same block, per-lane rank lookup
-    for (int t = 0; t < A.n_low; t += UNR) {
+    for (int t = 0; t < A.e_low; t += UNR) {
       double2 g[UNR];
       int64_t l[UNR];
       bool act[UNR];
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
       // wave-uniform block: predicate, lookups and sign of high terms are scalar
       const uint32_t xh = ib0 << nlow;
       const int64_t l22_0 = (int64_t)A.l22[ib0];
-      for (int t = 0; t < A.n_high; t += UNR) {
+      for (int t = 0; t < A.e_high; t += UNR) {
         double2 g[UNR];
         bool au[UNR];
 #pragma unroll
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
           }
         }
       }
-      for (int t = 0; t < A.n_mixed; t += UNR) {
+      for (int t = 0; t < A.e_mixed; t += UNR) {
         double2 g[UNR];
         int64_t la[UNR], lb[UNR];
         bool act[UNR];
@@ -195,32 +195,36 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
     } else {
       // lanes span several blocks: per-lane list_2_2 lookups
       const int64_t b22 = (int64_t)A.l22[ib];
-      for (int t = 0; t < A.n_high + A.n_mixed; t += UNR) {
-        double2 g[UNR];
+      for (int seg = 0; seg < 2; seg++) {  // the high group, then the mixed group
+        const CTerm tg = seg ? tm : th;
+        const int ne = seg ? A.e_mixed : A.e_high;
+        for (int t = 0; t < ne; t += UNR) {
+          double2 g[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; u++) {
-          const GTerm T = load_term(th + t + u);
-          const uint32_t y = x ^ (uint32_t)T.flip;
-          bool act = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
-          int64_t la = b21, lb = 0;
-          if (act) {
-            lb = (int64_t)A.l22[y >> nlow];
-            if ((y & irght) != ia) la = r21<RK>(A, s21, y & irght);
+          for (int u = 0; u < UNR; u++) {
+            const GTerm T = load_term(tg + t + u);
+            const uint32_t y = x ^ (uint32_t)T.flip;
+            bool act = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
+            int64_t la = b21, lb = 0;
+            if (act) {
+              lb = (int64_t)A.l22[y >> nlow];
+              if ((y & irght) != ia) la = r21<RK>(A, s21, y & irght);
+            }
+            const uint64_t idx = (uint64_t)(la + lb - 3);
+            act = act & (la != 0) & (lb != 0) & (idx < dim);
+            g[u] = make_double2(0.0, 0.0);
+            if (act) g[u] = v1[idx];
           }
-          const uint64_t idx = (uint64_t)(la + lb - 3);
-          act = act & (la != 0) & (lb != 0) & (idx < dim);
-          g[u] = make_double2(0.0, 0.0);
-          if (act) g[u] = v1[idx];
-        }
 #pragma unroll
-        for (int u = 0; u < UNR; u++) {
-          const GTerm T = load_term(th + t + u);
-          double2 gg = g[u];
-          if ((uint32_t)T.sgn != 0u && (__popc(x & (uint32_t)T.sgn) & 1)) {
-            gg.x = -gg.x;
-            gg.y = -gg.y;
+          for (int u = 0; u < UNR; u++) {
+            const GTerm T = load_term(tg + t + u);
+            double2 gg = g[u];
+            if ((uint32_t)T.sgn != 0u && (__popc(x & (uint32_t)T.sgn) & 1)) {
+              gg.x = -gg.x;
+              gg.y = -gg.y;
+            }
+            cmac<REALC>(ar, ai, T.cre, T.cim, gg);
           }
-          cmac<REALC>(ar, ai, T.cre, T.cim, gg);
         }
       }
       (void)b22;
@@ -339,7 +343,14 @@ static int launch_one(hphi_engine *e, const HvArgs &A, int g) {
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void *)k_hv_canon<TPB, REALC, RK, UNR>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_hv_canon<TPB, REALC, RK, UNR>), dim3(g), dim3(TPB), lds, e->stream, A);
+  // each group's loop stops at its last batch holding a compiled term (the
+  // groups are padded to HV_TERM_PAD, a multiple of UNR)
+  static_assert(HV_TERM_PAD % UNR == 0, "a batch must not cross a term group");
+  HvArgs B = A;
+  B.e_low = std::min(A.n_low, (e->r_low + UNR - 1) / UNR * UNR);
+  B.e_high = std::min(A.n_high, (e->r_high + UNR - 1) / UNR * UNR);
+  B.e_mixed = std::min(A.n_mixed, (e->r_mixed + UNR - 1) / UNR * UNR);
+  hipLaunchKernelGGL((k_hv_canon<TPB, REALC, RK, UNR>), dim3(g), dim3(TPB), lds, e->stream, B);
   HIPCHK(hipGetLastError());
   return 0;
 }
