@@ -37,7 +37,7 @@ EXPORTED = [
     "hphi_expec_cisajscktalt", "hphi_write_vec_file", "hphi_read_vec_file", "hphi_write_restart_vecs",
     "hphi_read_restart_vecs", "hphi_write_tmcomponents", "hphi_read_tmcomponents", "hphi_write_energy",
     "hphi_write_cisajs", "hphi_write_cisajscktalt", "hphi_write_tpq_files", "hphi_lanczos_restart",
-    "hphi_ctpq", "hphi_write_ctpq_files",
+    "hphi_ctpq", "hphi_write_ctpq_files", "hphi_mltply_dev_scaled", "hphi_vec_lanczos_resid",
 ]
 
 

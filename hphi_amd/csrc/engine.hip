@@ -1016,6 +1016,8 @@ HvArgs hphi_hv_args(hphi_engine *e, int k, double *v0, const double *v1) {
   A.n_low = e->n_low;
   A.n_high = e->n_high;
   A.n_mixed = e->n_mixed;
+  A.c0 = 1.0;
+  A.c1 = 1.0;
   A.e_low = e->n_low;
   A.e_high = e->n_high;
   A.e_mixed = e->n_mixed;
@@ -1055,9 +1057,17 @@ int hphi_hv_launch(hphi_engine *e, int k, double *v0, const double *v1) {
   return hphi_hv_launch_ex(e, k, v0, v1, 0);
 }
 
+static int hv_launch_args(hphi_engine *e, int k, HvArgs &A);
+
 int hphi_hv_launch_ex(hphi_engine *e, int k, double *v0, const double *v1, int overwrite) {
   HvArgs A = hphi_hv_args(e, k, v0, v1);
   A.overwrite = overwrite;
+  return hv_launch_args(e, k, A);
+}
+
+// launch of prepared arguments: the canonical single-vector kernel, the block
+// kernel, or the generic kernel; then the prdct partials are reduced
+static int hv_launch_args(hphi_engine *e, int k, HvArgs &A) {
   if (k == 1 && e->c2_ok && e->use_c2) {
     int g = canon_grid(e);
     if (hphi_hv_launch_canon(e, A, g)) return -1;
@@ -1111,6 +1121,18 @@ extern "C" int hphi_mltply_dev(hphi_engine *e, double *v0, const double *v1, dou
   if (!e || !v0 || !v1) return hphi_set_error("hphi_mltply_dev: NULL argument");
   if (ensure_ready(e)) return -1;
   if (hphi_hv_launch(e, 1, v0, v1)) return -1;
+  return hphi_fetch_prdct(e, 1, prdct);
+}
+
+extern "C" int hphi_mltply_dev_scaled(hphi_engine *e, double *v0, const double *v1, double c0, double c1,
+                                      int overwrite, double *prdct) {
+  if (!e || !v0 || !v1) return hphi_set_error("hphi_mltply_dev_scaled: NULL argument");
+  if (ensure_ready(e)) return -1;
+  HvArgs A = hphi_hv_args(e, 1, v0, v1);
+  A.overwrite = overwrite ? 1 : 0;
+  A.c0 = c0;
+  A.c1 = c1;
+  if (hv_launch_args(e, 1, A)) return -1;
   return hphi_fetch_prdct(e, 1, prdct);
 }
 

@@ -90,7 +90,8 @@ struct HvArgs {
   double2 *partial;
   int xcd_map;
   uint64_t size21;
-  int overwrite;  // 1: v0 = H v1 (v0 not read); 0: v0 += H v1 (reference semantics)
+  int overwrite;  // 1: v0 = c1 H v1 (v0 not read); 0: v0 = c0 v0 + c1 H v1 (reference: c0 = c1 = 1)
+  double c0, c1;  // scale factors of the output (prdct stays <v1|H v1>)
   // canonical Spin: list_2_1[ia] = rank of ia among the values of its popcount
   // + 2, from two byte tables (hphi_crank_tables)
   const uint16_t *crank;

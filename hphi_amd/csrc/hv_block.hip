@@ -264,11 +264,11 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
       const double2 *own = v1 + j * K + c0;
 #pragma unroll
       for (int c = 0; c < CPL; c++) {
-        double2 o = make_double2(ar[c], ai[c]);
+        double2 o = make_double2(A.c1 * ar[c], A.c1 * ai[c]);
         if (!A.overwrite) {
           const double2 p = out[c];
-          o.x += p.x;
-          o.y += p.y;
+          o.x += A.c0 * p.x;
+          o.y += A.c0 * p.y;
         }
         out[c] = o;
         const double2 a = own[c];

@@ -231,11 +231,11 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
     }
 
     if (valid) {
-      double2 o = make_double2(ar, ai);
+      double2 o = make_double2(A.c1 * ar, A.c1 * ai);
       if (!A.overwrite) {
         const double2 p = v0[j];
-        o.x += p.x;
-        o.y += p.y;
+        o.x += A.c0 * p.x;
+        o.y += A.c0 * p.y;
       }
       v0[j] = o;
       pre += a.x * ar + a.y * ai;

@@ -162,11 +162,11 @@ __device__ __forceinline__ void hv_tile(const HvArgs &A, uint64_t tile, double &
     double2 a;
     double ar, ai;
     hv_row<K, CANON>(A, j, vec, a, ar, ai);
-    double2 o = make_double2(ar, ai);
+    double2 o = make_double2(A.c1 * ar, A.c1 * ai);
     if (!A.overwrite) {
       const double2 p = v0[j * K + vec];
-      o.x += p.x;
-      o.y += p.y;
+      o.x += A.c0 * p.x;
+      o.y += A.c0 * p.y;
     }
     v0[j * K + vec] = o;
     // conj(a) * acc

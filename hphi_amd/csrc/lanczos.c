@@ -101,44 +101,50 @@ static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lancz
   int every = p->check_every > 0 ? p->check_every : 1;
   double t0 = wall(), thv = 0.0;
 
-  double *v0 = NULL, *v1 = NULL;
+  /* The recurrence of Lanczos_EigenValue.c:100-177 with the normalisation
+   * deferred: w_m = q_m / s_m is kept unscaled (s_m = 1/beta_{m-1}), so a step
+   * is one H·v that also forms r_m = s_m H w_m - beta_{m-1} s_{m-1} w_{m-1}
+   * (hphi_mltply_dev_scaled), and one residual pass w_{m+1} = r_m -
+   * alpha_m s_m w_m with |w_{m+1}| = beta_m (hphi_vec_lanczos_resid), instead
+   * of the reference's H·v, beta pass and update pass.  alpha_m = s_m^2 <w_m|H w_m>.
+   * Buffers: ra holds r_m, then w_{m+1}; wb holds w_m. */
+  double *ra = NULL, *wb = NULL;
   size_t bytes = (size_t)dim * 16;
-  if (hphi_dev_alloc(bytes, (void **)&v0) || hphi_dev_alloc(bytes, (void **)&v1)) {
-    if (v0) hphi_dev_free(v0);
+  if (hphi_dev_alloc(bytes, (void **)&ra) || hphi_dev_alloc(bytes, (void **)&wb)) {
+    if (ra) hphi_dev_free(ra);
     return -1;
   }
   double *alpha = (double *)calloc(maxit + 2, sizeof(double));
   double *beta = (double *)calloc(maxit + 2, sizeof(double));
   double *ev = (double *)calloc(maxit + 2, sizeof(double));
   int rc = -1, stp = 1, nprev = 0, last = 0, stp0 = 2;
-  double pr[2], nrm, b2, ebefor = 0.0, prevv[16], norm0 = 0.0;
+  double pr[2], nrm, b2, ebefor = 0.0, prevv[16], norm0 = 0.0, s_prev = 1.0, s_cur = 1.0;
 
-  if (n0) { /* continue: the recurrence state of step n0 */
+  if (n0) { /* continue: the reference's state of step n0, v0 = r_n0 and v1 = q_n0 */
     for (int i = 0; i < n0; i++) {
       alpha[i + 1] = in->alpha[i];
       beta[i + 1] = in->beta[i];
     }
-    if (hphi_memcpy_h2d(v0, in->v0, bytes) || hphi_memcpy_h2d(v1, in->v1, bytes)) goto done;
+    if (hphi_memcpy_h2d(ra, in->v0, bytes) || hphi_memcpy_h2d(wb, in->v1, bytes)) goto done;
     norm0 = in->norm;
+    /* w_{n0+1} = r - alpha q (s = 1); beta_n0 is the file's, as in the reference */
+    if (hphi_vec_lanczos_resid(e, ra, wb, NULL, alpha[n0], 0.0, &b2)) goto done;
+    s_cur = 1.0;
     stp0 = n0 + 1;
     last = n0;
     goto loop;
   }
-  /* initial vector (SetInitialVector): random, normalised */
-  if (hphi_fill_random(v1, dim, 1, p->seed) || hphi_vec_norm2(e, v1, 1, &nrm)) goto done;
+  /* initial vector (SetInitialVector): random; q_1 = s_1 w_1 */
+  if (hphi_fill_random(wb, dim, 1, p->seed) || hphi_vec_norm2(e, wb, 1, &nrm)) goto done;
   norm0 = sqrt(nrm);
-  {
-    double ab[4] = {0.0, 0.0, 1.0 / sqrt(nrm), 0.0}; /* v1 <- 0*v1 + (1/|v1|) v1 */
-    if (hphi_vec_axpby(e, v1, v1, 1, ab)) goto done;
-  }
-  if (hphi_dev_zero(v0, bytes)) goto done;
+  s_cur = 1.0 / sqrt(nrm);
   {
     double t = wall();
-    if (hphi_mltply_dev(e, v0, v1, pr)) goto done;
+    if (hphi_mltply_dev_scaled(e, ra, wb, 0.0, s_cur, 1, pr)) goto done; /* r_1 = H q_1 */
     thv += wall() - t;
   }
-  alpha[1] = pr[0];
-  if (hphi_vec_lanczos_beta(e, v0, v1, alpha[1], &b2)) goto done;
+  alpha[1] = s_cur * s_cur * pr[0];
+  if (hphi_vec_lanczos_resid(e, ra, wb, NULL, alpha[1] * s_cur, 0.0, &b2)) goto done;
   beta[1] = sqrt(b2);
   last = 1;
   if (dim == 1) {
@@ -151,12 +157,19 @@ static int lanczos_core(hphi_engine *e, const hphi_lanczos_params *p, hphi_lancz
   }
 loop:
   for (stp = stp0; stp <= maxit; stp++) {
-    if (hphi_vec_lanczos_update(e, v0, v1, alpha[stp - 1], beta[stp - 1])) goto done;
-    double t = wall();
-    if (hphi_mltply_dev(e, v0, v1, pr)) goto done;
-    thv += wall() - t;
-    alpha[stp] = pr[0];
-    if (hphi_vec_lanczos_beta(e, v0, v1, alpha[stp], &b2)) goto done;
+    s_prev = s_cur;
+    s_cur = 1.0 / beta[stp - 1];
+    {
+      double t = wall();
+      /* wb <- -beta_{m-1} s_{m-1} w_{m-1} + s_m H w_m = r_m; then ra, wb swap roles */
+      if (hphi_mltply_dev_scaled(e, wb, ra, -beta[stp - 1] * s_prev, s_cur, 0, pr)) goto done;
+      thv += wall() - t;
+      double *tmp = ra;
+      ra = wb;
+      wb = tmp;
+    }
+    alpha[stp] = s_cur * s_cur * pr[0];
+    if (hphi_vec_lanczos_resid(e, ra, wb, NULL, alpha[stp] * s_cur, 0.0, &b2)) goto done;
     beta[stp] = sqrt(b2);
     last = stp;
     if (stp % every != 0 && stp != maxit && beta[stp] >= 1e-14) continue;
@@ -191,7 +204,7 @@ loop:
     }
   }
   rc = 0;
-  if (out) { /* the state of the last completed step */
+  if (out) { /* the reference's state of the last completed step: v0 = r, v1 = q */
     out->n = last;
     out->norm = norm0;
     out->alpha = (double *)malloc(sizeof(double) * (last + 1));
@@ -205,14 +218,18 @@ loop:
         out->alpha[i] = alpha[i + 1];
         out->beta[i] = beta[i + 1];
       }
-      if (hphi_memcpy_d2h(out->v0, v0, bytes) || hphi_memcpy_d2h(out->v1, v1, bytes)) rc = -1;
+      /* r_last = w_{last+1} + alpha_last s_last w_last, q_last = s_last w_last */
+      double ar[4] = {alpha[last] * s_cur, 0.0, 1.0, 0.0}, aq[4] = {0.0, 0.0, s_cur, 0.0};
+      if (hphi_vec_axpby(e, wb, ra, 1, ar) || hphi_vec_axpby(e, wb, wb, 1, aq) ||
+          hphi_memcpy_d2h(out->v0, ra, bytes) || hphi_memcpy_d2h(out->v1, wb, bytes))
+        rc = -1;
     }
   }
 done:
   r->seconds = wall() - t0;
   r->hv_seconds = thv;
-  hphi_dev_free(v0);
-  hphi_dev_free(v1);
+  hphi_dev_free(ra);
+  hphi_dev_free(wb);
   if (rc == 0 && alpha_out && beta_out) {
     *alpha_out = alpha;
     *beta_out = beta;
@@ -368,30 +385,41 @@ int hphi_lanczos_eigenvector(hphi_engine *e, const hphi_lanczos_params *p, int k
   hphi_tridiag_lowest(alpha + 1, beta + 1, m, k + 1, ev);
   tridiag_eigvec(alpha + 1, beta + 1, m, ev[k], c);
   r->eigenvalues[0] = ev[k];
-  double t0 = wall(), thv = 0.0, pr[2], nrm;
-  double *v0 = NULL, *v1 = NULL;
+  double t0 = wall(), thv = 0.0, pr[2], nrm, b2;
+  double *ra = NULL, *wb = NULL;
   const size_t bytes = (size_t)dim * 16;
   int rc = -1;
-  if (hphi_dev_alloc(bytes, (void **)&v0) || hphi_dev_alloc(bytes, (void **)&v1)) goto done;
-  /* the same start vector as the eigenvalue pass (Lanczos_EigenVector.c:65-137) */
-  if (hphi_fill_random(v1, dim, 1, p->seed) || hphi_vec_norm2(e, v1, 1, &nrm)) goto done;
+  if (hphi_dev_alloc(bytes, (void **)&ra) || hphi_dev_alloc(bytes, (void **)&wb)) goto done;
+  /* The eigenvalue pass's operations again, in the same order on the same start
+   * vector (Lanczos_EigenVector.c:65-190), so every w_s is bitwise the same; the
+   * residual pass of step s+1 also adds c_s q_s = c_s s_s w_s to v. */
+  if (hphi_fill_random(wb, dim, 1, p->seed) || hphi_vec_norm2(e, wb, 1, &nrm)) goto done;
+  if (hphi_dev_zero(vout, bytes)) goto done;
   {
-    double ab[4] = {1.0 / sqrt(nrm), 0.0, 0.0, 0.0};
-    if (hphi_vec_axpby(e, v1, v1, 1, ab)) goto done;
-  }
-  if (hphi_dev_zero(v0, bytes) || hphi_dev_zero(vout, bytes)) goto done;
-  {
-    double ab[4] = {c[0], 0.0, 0.0, 0.0}; /* v = c_1 q_1 */
-    if (hphi_vec_axpby(e, v1, vout, 1, ab)) goto done;
-  }
-  /* q_{s} = (v0 - alpha_{s-1} q_{s-1}) / beta_{s-1} in the update kernel, which
-   * also adds c_s q_s (Lanczos_EigenVector.c:139-190) */
-  for (int st = 1; st <= m; st++) {
-    if (st > 1 && hphi_vec_lanczos_update_acc(e, v0, v1, vout, alpha[st - 1], beta[st - 1], c[st - 1])) goto done;
-    if (st < m) {
+    double s_cur = 1.0 / sqrt(nrm), s_prev;
+    if (m == 1) {
+      double ab[4] = {c[0] * s_cur, 0.0, 0.0, 0.0}; /* v = c_1 q_1 */
+      if (hphi_vec_axpby(e, wb, vout, 1, ab)) goto done;
+    } else {
       double t = wall();
-      if (hphi_mltply_dev(e, v0, v1, pr)) goto done;
+      if (hphi_mltply_dev_scaled(e, ra, wb, 0.0, s_cur, 1, pr)) goto done;
       thv += wall() - t;
+    }
+    for (int st = 2; st <= m; st++) {
+      if (hphi_vec_lanczos_resid(e, ra, wb, vout, alpha[st - 1] * s_cur, c[st - 2] * s_cur, &b2)) goto done;
+      s_prev = s_cur;
+      s_cur = 1.0 / beta[st - 1];
+      if (st < m) {
+        double t = wall();
+        if (hphi_mltply_dev_scaled(e, wb, ra, -beta[st - 1] * s_prev, s_cur, 0, pr)) goto done;
+        thv += wall() - t;
+        double *tmp = ra;
+        ra = wb;
+        wb = tmp;
+      } else {
+        double ab[4] = {c[m - 1] * s_cur, 0.0, 1.0, 0.0}; /* v += c_m q_m, q_m = s_m w_m (in ra) */
+        if (hphi_vec_axpby(e, ra, vout, 1, ab)) goto done;
+      }
     }
   }
   if (hphi_vec_norm2(e, vout, 1, &nrm)) goto done;
@@ -403,8 +431,8 @@ int hphi_lanczos_eigenvector(hphi_engine *e, const hphi_lanczos_params *p, int k
 done:
   r->seconds += wall() - t0;
   r->hv_seconds += thv;
-  hphi_dev_free(v0);
-  hphi_dev_free(v1);
+  hphi_dev_free(ra);
+  hphi_dev_free(wb);
   free(alpha);
   free(beta);
   free(ev);

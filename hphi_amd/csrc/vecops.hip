@@ -44,6 +44,27 @@ __global__ __launch_bounds__(VO_TPB) void k_lanczos_beta(const double2 *v0, cons
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Lanczos residual with the normalisation deferred: r <- r - f w (f = alpha_m
+// s_m, w the unnormalised Lanczos vector of step m), |r|^2 per block; with vg
+// also vg += cg w (the eigenvector sum).  One pass of 48 bytes per state.
+__global__ __launch_bounds__(VO_TPB) void k_lz_resid(double2 *r, const double2 *w, double2 *vg, uint64_t n, double f,
+                                                     double cg, double *partial) {
+  __shared__ double sh[VO_TPB / 64];
+  double s = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * VO_TPB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * VO_TPB) {
+    const double2 a = r[i], b = w[i];
+    const double2 t = make_double2(a.x - f * b.x, a.y - f * b.y);
+    r[i] = t;
+    s += t.x * t.x + t.y * t.y;
+    if (vg) {
+      const double2 g = vg[i];
+      vg[i] = make_double2(g.x + cg * b.x, g.y + cg * b.y);
+    }
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // temp1 = v1; temp2 = (v0 - alpha v1)/beta; v0 = -beta temp1; v1 = temp2
 // (Lanczos_EigenValue.c:154-160)
 __global__ __launch_bounds__(VO_TPB) void k_lanczos_update(double2 *v0, double2 *v1, uint64_t n, double alpha,
@@ -134,6 +155,15 @@ static int finish_sum(hphi_engine *e, int g, int m, double *host_out) {
   HIPCHK(hipStreamSynchronize(e->stream));
   memcpy(host_out, e->h_prdct, sizeof(double) * m);
   return 0;
+}
+
+extern "C" int hphi_vec_lanczos_resid(hphi_engine *e, double *r, const double *w, double *vg, double f, double cg,
+                                      double *r2) {
+  if (!e || !r || !w || !r2) return hphi_set_error("hphi_vec_lanczos_resid: NULL argument");
+  int g = vo_grid(e->dim);
+  hipLaunchKernelGGL(k_lz_resid, dim3(g), dim3(VO_TPB), 0, e->stream, (double2 *)r, (const double2 *)w,
+                     (double2 *)vg, e->dim, f, cg, (double *)e->d_partial);
+  return finish_sum(e, g, 1, r2);
 }
 
 extern "C" int hphi_vec_lanczos_beta(hphi_engine *e, const double *v0, const double *v1, double alpha,

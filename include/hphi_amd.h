@@ -131,6 +131,11 @@ int hphi_diagonalcalc(hphi_engine *e, double *list_Diagonal);
 int hphi_mltply(hphi_engine *e, double *v0, const double *v1, double *prdct);
 /* Device-resident form: v0/v1 are 0-indexed device arrays of dim complex128. */
 int hphi_mltply_dev(hphi_engine *e, double *v0, const double *v1, double *prdct);
+/* Scaled device form: v0 = c0 v0 + c1 H v1 (v0 = c1 H v1 with overwrite, v0
+ * not read), prdct = <v1|H|v1>.  No counterpart in the reference: the device
+ * Lanczos folds its normalisation 1/beta and -beta into this launch. */
+int hphi_mltply_dev_scaled(hphi_engine *e, double *v0, const double *v1, double c0, double c1, int overwrite,
+                           double *prdct);
 /* Block form: k vectors stored state-major, V[(j*k + v)*2 + {0,1}], device
  * memory; prdct[2k].  k in {1, 2, 4, 8, 16}. */
 int hphi_mltply_block_dev(hphi_engine *e, int k, double *V0, const double *V1, double *prdct);
@@ -153,6 +158,11 @@ int hphi_fill_random(double *v, uint64_t dim, int k, uint64_t seed);
  *   norm2 / dot     per-column squared norms / <x_c|y_c> of [dim][k] blocks
  *   axpby           y_c <- a_c x_c + b_c y_c, ab = {a_0, b_0, a_1, b_1, ...} complex */
 int hphi_vec_lanczos_beta(hphi_engine *e, const double *v0, const double *v1, double alpha, double *beta2);
+/* Residual of the recurrence with the normalisation deferred (the device
+ * Lanczos): r <- r - f w, r2 = |r|^2; with vg non-NULL also vg += cg w.  With
+ * r = H q_m - beta_{m-1} q_{m-1}, w = q_m / s_m and f = alpha_m s_m this is
+ * (Lanczos_EigenValue.c:154-177) with q_{m+1} = r / beta_m left unscaled. */
+int hphi_vec_lanczos_resid(hphi_engine *e, double *r, const double *w, double *vg, double f, double cg, double *r2);
 int hphi_vec_lanczos_update(hphi_engine *e, double *v0, double *v1, double alpha, double beta);
 /* the update, and vg += c * v1_new (Lanczos_EigenVector.c:170-185) */
 int hphi_vec_lanczos_update_acc(hphi_engine *e, double *v0, double *v1, double *vg, double alpha, double beta,
