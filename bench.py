@@ -105,7 +105,8 @@ def cfg4_leg(args):
     """Workload 4: triangular 13x2 XXZ in a transverse field (SpinGC, dim 2^26).
     mTPQ with k = 16 complex Gaussian samples: per-step time of the block
     layout (one block H·v for all samples, hipGraph-captured step) and of the
-    per-sample layout (k single-vector H·v); then LOBPCG with a block of 8."""
+    per-sample layout (k single-vector H·v); canonical TPQ on the same 16
+    samples (10 block H·v per step); then LOBPCG with a block of 8."""
     from hphi_amd import capi, lattice as L
 
     spec = L.baseline_config(4)
@@ -124,13 +125,25 @@ def cfg4_leg(args):
         ms = 1e3 * (t[1] - t[0]) / (args.tpq_steps - args.tpq_steps // 4)
         out[f"tpq_{name}_ms_per_step"] = round(ms, 3)
         out[f"tpq_{name}_ms_per_sample_step"] = round(ms / k, 3)
-    # off-diagonal elements per H·v: 78 exchange entries act where the two spins differ
-    # (half of the states), 52 transverse entries act where their spin matches (half)
-    nnz = 78 * dim // 2 + 52 * dim // 2
+    # off-diagonal elements per H·v: each distinct exchange pair acts where the two
+    # spins differ (half of the states; the torus lists 78 entries for 65 pairs, and
+    # the engine merges identical terms), 52 transverse entries act where their
+    # spin matches (half)
+    pairs = len({(min(i, j), max(i, j)) for i, j, _ in spec.exchange})
+    out["exchange_pairs"] = pairs
+    nnz = pairs * dim // 2 + 52 * dim // 2
     nnz_state = nnz / dim
     balg = k * (16 * nnz + 16 * dim) + 8 * dim + 48 * dim * k  # overwrite H·v + diag + fused step pass
     out["tpq_block_gbs"] = round(balg / (out["tpq_block_ms_per_step"] / 1e3) / 1e9, 1)
     out["offdiag_per_state"] = nnz_state
+    # canonical TPQ: ExpandCoef (10) block H·v per step, the same 16 samples
+    t = []
+    for steps in (2, 5):
+        _, r = eng.ctpq(k, steps, seed=0, init_kind=-1)
+        t.append(r.seconds)
+    ms = 1e3 * (t[1] - t[0]) / 3
+    out["ctpq_ms_per_step"] = round(ms, 2)
+    out["ctpq_ms_per_block_hv"] = round(ms / 10, 2)
     r, hist = eng.lobpcg(8, max_iter=args.lobpcg_iters, eps=1e-7, seed=0)
     out["lobpcg_k"] = 8
     out["lobpcg_iterations"] = r.iterations

@@ -16,6 +16,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "hphi_amd.h"
@@ -361,9 +363,27 @@ int hphi_engine_alloc_scratch(hphi_engine *e) {
 }
 
 static void classify_and_upload_terms(hphi_engine *e) {
+  // Identical terms (same flip, predicate and sign mask) act on the same pairs
+  // of states: they become one term with the summed coupling, in order of first
+  // appearance (the 13x2 triangular torus lists each a2 bond twice).  Terms
+  // whose couplings cancel exactly are dropped.
+  std::vector<GTerm> merged;
+  std::map<std::tuple<uint64_t, uint64_t, uint64_t, uint64_t, int>, size_t> first;
+  for (const GTerm &t : e->h_terms) {
+    const auto key = std::make_tuple(t.flip, t.tmask, t.tval, t.sgn, t.kind);
+    const auto it = first.find(key);
+    if (it == first.end()) {
+      first.emplace(key, merged.size());
+      merged.push_back(t);
+    } else {
+      merged[it->second].cre += t.cre;
+      merged[it->second].cim += t.cim;
+    }
+  }
   // groups: low (flip within the low half), high (within the high half), mixed
   std::vector<GTerm> low, high, mixed;
-  for (const GTerm &t : e->h_terms) {
+  for (const GTerm &t : merged) {
+    if (t.cre == 0.0 && t.cim == 0.0) continue;
     if (hphi_model_gc(e->model)) {
       low.push_back(t);
       continue;
@@ -856,6 +876,7 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
     e->c2_unroll = value;
   } else if (k == "xcd_map") {
     e->xcd_map = value ? 1 : 0;
+    e->xcd_tuned = 1;
   } else if (k == "use_c2") {
     e->use_c2 = value ? 1 : 0;
   } else if (k == "grid_per_cu") {
@@ -1086,6 +1107,10 @@ static int hv_launch_args(hphi_engine *e, int k, HvArgs &A) {
   }
   int g = hphi_hv_grid(e, k);
   bool canon = !hphi_model_gc(e->model);
+  static_assert(HV_TERM_PAD % HV_UNROLL == 0, "a batch must not cross a term group");
+  A.e_low = std::min(A.n_low, (e->r_low + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
+  A.e_high = std::min(A.n_high, (e->r_high + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
+  A.e_mixed = std::min(A.n_mixed, (e->r_mixed + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
 #define LAUNCH(KK)                                                                                   \
   if (canon) hipLaunchKernelGGL((k_hv<KK, true>), dim3(g), dim3(HV_TPB), 0, e->stream, A);          \
   else hipLaunchKernelGGL((k_hv<KK, false>), dim3(g), dim3(HV_TPB), 0, e->stream, A);

@@ -124,6 +124,7 @@ struct hphi_engine {
   int num_cu = 256;
   int grid_per_cu = 8;  // H·v workgroups per CU (tunable)
   int xcd_map = 1;      // contiguous tile ranges per XCD (tunable)
+  int xcd_tuned = 0;    // xcd_map was set through hphi_engine_tune
   bool tables_ready = false, diag_ready = false;
   uint64_t *d_list1 = nullptr, *d_l22 = nullptr, *d_jb = nullptr;
   uint32_t *d_l21 = nullptr;

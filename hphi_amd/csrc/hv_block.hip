@@ -136,7 +136,7 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
       const uint32_t ia = x & irght, ib = x >> nlow;
       const int64_t b21 = blk_l21<CRANK>(s21, ia);
       // ---- low-half terms: same block, rank lookup in LDS
-      for (int t = 0; t < A.n_low; t += U) {
+      for (int t = 0; t < A.e_low; t += U) {
         uint64_t idx[U];
         bool act[U];
 #pragma unroll
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
       if (uni) {
         const uint32_t xh = ib0 << nlow;
         const int64_t l22_0 = (int64_t)A.l22[ib0];
-        for (int t = 0; t < A.n_high; t += U) {
+        for (int t = 0; t < A.e_high; t += U) {
           uint64_t idx[U];
           bool act[U];
 #pragma unroll
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
             bk_mac<CPL>(ar, ai, s * T.cre, s * T.cim, g[u], rc);
           }
         }
-        for (int t = 0; t < A.n_mixed; t += U) {
+        for (int t = 0; t < A.e_mixed; t += U) {
           uint64_t idx[U];
           bool act[U];
 #pragma unroll
@@ -210,36 +210,40 @@ __global__ __launch_bounds__(TPB) void k_hv_blk(const HvArgs A, const int realc)
         }
       } else {
         // lanes span several blocks: per-lane list_2_2 lookups
-        for (int t = 0; t < A.n_high + A.n_mixed; t += U) {
-          uint64_t idx[U];
-          bool act[U];
+        for (int seg = 0; seg < 2; seg++) {  // the high group, then the mixed group
+          const CTerm tg = seg ? tm : th;
+          const int ne = seg ? A.e_mixed : A.e_high;
+          for (int t = 0; t < ne; t += U) {
+            uint64_t idx[U];
+            bool act[U];
 #pragma unroll
-          for (int u = 0; u < U; u++) {
-            const GTerm T = load_term(th + t + u);
-            const uint32_t y = x ^ (uint32_t)T.flip;
-            bool a = valid & bk_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
-            int64_t la = b21, lb = 0;
-            if (a) {
-              lb = (int64_t)A.l22[y >> nlow];
-              if ((y & irght) != ia) la = blk_l21<CRANK>(s21, y & irght);
+            for (int u = 0; u < U; u++) {
+              const GTerm T = load_term(tg + t + u);
+              const uint32_t y = x ^ (uint32_t)T.flip;
+              bool a = valid & bk_pred(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, x);
+              int64_t la = b21, lb = 0;
+              if (a) {
+                lb = (int64_t)A.l22[y >> nlow];
+                if ((y & irght) != ia) la = blk_l21<CRANK>(s21, y & irght);
+              }
+              idx[u] = (uint64_t)(la + lb - 3);
+              act[u] = a & (la != 0) & (lb != 0) & (idx[u] < dim);
             }
-            idx[u] = (uint64_t)(la + lb - 3);
-            act[u] = a & (la != 0) & (lb != 0) & (idx[u] < dim);
-          }
-          double2 g[U][CPL];
+            double2 g[U][CPL];
 #pragma unroll
-          for (int u = 0; u < U; u++) bk_load<CPL>(v1 + idx[u] * K + c0, act[u], g[u]);
+            for (int u = 0; u < U; u++) bk_load<CPL>(v1 + idx[u] * K + c0, act[u], g[u]);
 #pragma unroll
-          for (int u = 0; u < U; u++) {
-            const GTerm T = load_term(th + t + u);
-            const double s = bk_sign((uint32_t)T.sgn, x);
-            bk_mac<CPL>(ar, ai, s * T.cre, s * T.cim, g[u], rc);
+            for (int u = 0; u < U; u++) {
+              const GTerm T = load_term(tg + t + u);
+              const double s = bk_sign((uint32_t)T.sgn, x);
+              bk_mac<CPL>(ar, ai, s * T.cre, s * T.cim, g[u], rc);
+            }
           }
         }
       }
     } else {
       // grand canonical: the state is its own index
-      for (int t = 0; t < A.n_low; t += U) {
+      for (int t = 0; t < A.e_low; t += U) {
         uint64_t idx[U];
         bool act[U];
 #pragma unroll
@@ -350,15 +354,19 @@ void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_pe
     case 8: c = e->bk_cpl[2]; break;
     case 16: c = e->bk_cpl[3]; break;
   }
-  if (c <= 0 || !hphi_blk_cpl_ok(k, c)) c = k >= 8 ? 4 : k;
-  *cpl = c;
   const bool canon = !hphi_model_gc(e->model);
+  // grand canonical, k = 16: one column per lane and 4 workgroups per CU (13x2
+  // triangular SpinGC: 137 ms per block H·v vs 156-170 ms for 4 or 16 columns per
+  // lane and 8-32 workgroups per CU; mTPQ step 143.8 ms vs 162.6 ms)
+  const bool gc16 = !canon && k == 16;
+  if (c <= 0 || !hphi_blk_cpl_ok(k, c)) c = gc16 ? 1 : (k >= 8 ? 4 : k);
+  *cpl = c;
   int t = e->bk_tpb;
   if (t != 256 && t != 1024) t = (canon && !e->c2_crank && 2 * e->size21 > 32768) ? 1024 : 256;
   if (!canon || e->c2_crank) t = 256;  // the rank tables leave room for many 256-thread workgroups
   *tpb = t;
   int w = e->bk_wgs_per_cu;
-  if (w <= 0) w = (t == 1024) ? 1 : (canon && e->c2_crank ? 48 : 8);  // 8x4 J1-J2, k=4: 110.4 ms/column
+  if (w <= 0) w = (t == 1024) ? 1 : (canon && e->c2_crank ? 48 : (gc16 ? 4 : 8));  // 8x4 J1-J2, k=4: 106 ms/column
   *wgs_per_cu = w;
 }
 
@@ -373,8 +381,16 @@ int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid) {
   g = (g + 7) / 8 * 8;
   if (g > HV_MAX_GRID) g = HV_MAX_GRID;
   *grid = (int)g;
+  // each group's loop stops at its last batch holding a compiled term (the
+  // groups are padded to HV_TERM_PAD, a multiple of every batch size)
+  const int u = cpl >= 8 ? 1 : 8 / cpl;  // terms per batch of k_hv_blk
+  HvArgs B = A;
+  if (!canon && k == 16 && !e->xcd_tuned) B.xcd_map = 1;  // with the grand-canonical k = 16 shape above
+  B.e_low = std::min(A.n_low, (e->r_low + u - 1) / u * u);
+  B.e_high = std::min(A.n_high, (e->r_high + u - 1) / u * u);
+  B.e_mixed = std::min(A.n_mixed, (e->r_mixed + u - 1) / u * u);
 #define CASE(KK, CC) \
-  if (k == KK && cpl == CC) return bk_launch_tpb<KK, CC>(e, A, (int)g, tpb, canon);
+  if (k == KK && cpl == CC) return bk_launch_tpb<KK, CC>(e, B, (int)g, tpb, canon);
   CASE(2, 1)
   CASE(2, 2)
   CASE(4, 1)

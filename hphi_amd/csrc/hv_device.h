@@ -49,7 +49,7 @@ __device__ __forceinline__ void hv_row(const HvArgs &A, uint64_t j, int vec, dou
       const uint64_t ia0 = x & A.irght, ib0 = x >> A.nlow;
       const int64_t b21 = (int64_t)A.l21[ia0], b22 = (int64_t)A.l22[ib0];
       // low group: only the low half changes, same block
-      for (int t = 0; t < A.n_low; t += HV_UNROLL) {
+      for (int t = 0; t < A.e_low; t += HV_UNROLL) {
         double2 g[HV_UNROLL];
         double cr[HV_UNROLL], ci[HV_UNROLL];
 #pragma unroll
@@ -74,7 +74,7 @@ __device__ __forceinline__ void hv_row(const HvArgs &A, uint64_t j, int vec, dou
       }
       // high group: only the high half changes, same rank in another block
       const CTerm th = (CTerm)A.terms + A.n_low;
-      for (int t = 0; t < A.n_high; t += HV_UNROLL) {
+      for (int t = 0; t < A.e_high; t += HV_UNROLL) {
         double2 g[HV_UNROLL];
         double cr[HV_UNROLL], ci[HV_UNROLL];
 #pragma unroll
@@ -99,7 +99,7 @@ __device__ __forceinline__ void hv_row(const HvArgs &A, uint64_t j, int vec, dou
       }
       // mixed group: both halves change, two lookups
       const CTerm tm = th + A.n_high;
-      for (int t = 0; t < A.n_mixed; t += HV_UNROLL) {
+      for (int t = 0; t < A.e_mixed; t += HV_UNROLL) {
         double2 g[HV_UNROLL];
         double cr[HV_UNROLL], ci[HV_UNROLL];
 #pragma unroll
@@ -124,7 +124,7 @@ __device__ __forceinline__ void hv_row(const HvArgs &A, uint64_t j, int vec, dou
         }
       }
     } else {
-      for (int t = 0; t < A.n_low; t += HV_UNROLL) {
+      for (int t = 0; t < A.e_low; t += HV_UNROLL) {
         double2 g[HV_UNROLL];
         double cr[HV_UNROLL], ci[HV_UNROLL];
 #pragma unroll
