@@ -156,6 +156,41 @@ def cfg4_leg(args):
     return out
 
 
+def cfg2_leg(args):
+    """Workload 2: Hubbard 4x4 PBC, t=1, U=4, Nup=Ndown=8 (dim 165,636,900):
+    plain H·v (HIP events over K launches, seed 1) and the Lanczos step."""
+    from hphi_amd import capi, lattice as L
+
+    spec = L.baseline_config(2)
+    eng = capi.Engine.from_spec(spec)
+    eng.sz(copy=False)
+    eng.diagonalcalc(copy=False)
+    dim = eng.dim
+    # a directed hop c+_{i s} c_{j s} acts where orbital (j, s) is filled and (i, s)
+    # empty: C(N-2, n_s - 1) C(N, n_other) states
+    n = spec.nsite
+    nnz = 0
+    for (i, si, j, sj, _, _) in spec.trans:
+        ns, no = (spec.nup, spec.ndown) if si == 0 else (spec.ndown, spec.nup)
+        nnz += math.comb(n - 2, ns - 1) * math.comb(n, no)
+    balg = b_alg(dim, nnz)
+    v1 = capi.DeviceArray(dim)
+    v0 = capi.DeviceArray(dim)
+    eng.fill_random(v1, 1, 1)
+    v0.zero()
+    if args.warmup:
+        eng.time_mltply(1, v0, v1, 0, args.warmup)
+    ms = eng.time_mltply(1, v0, v1, 0, args.steps)
+    v0.free()
+    v1.free()
+    lz = eng.lanczos(n_eigen=1, max_iter=args.lanczos_steps, eps=1e-300, target=2, seed=1)
+    out = {"dim": dim, "nnz_offdiag": nnz, "bytes_alg_per_hv": balg, "hv_ms": round(ms, 3),
+           "hv_gbs": round(balg / (ms / 1e3) / 1e9, 1), "hbm_frac": round(balg / (ms / 1e3) / 8e12, 4),
+           "lanczos_step_ms": round(1e3 * lz.seconds / max(lz.iterations, 1), 3)}
+    eng.close()
+    return out
+
+
 def pmc_probe(args):
     """A few plain H·v launches on the 8x4 J1-J2 sector, then the same number of
     launches of a streaming kernel of known traffic (y = a x + b y over two
@@ -367,6 +402,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--block-k", type=int, default=4, help="also time the block H·v with this many columns")
     ap.add_argument("--pmc-probe", action="store_true", help="H·v + calibration launches for counter runs")
+    ap.add_argument("--cfg2", action="store_true", help="plain H·v and Lanczos step on the Hubbard 4x4 workload")
     ap.add_argument("--cfg4", action="store_true", help="TPQ (k=16) and LOBPCG (block 8) on the 26-site SpinGC workload")
     ap.add_argument("--tpq-steps", type=int, default=40)
     ap.add_argument("--lobpcg-iters", type=int, default=60)
@@ -401,6 +437,14 @@ def main():
         return
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.cfg2:
+        c2 = cfg2_leg(args)
+        print(json.dumps({"metric": "H·v GB/s (% HBM peak)", "value": c2["hv_gbs"], "unit": "GB/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": c2["hv_ms"],
+                          "higher_is_better": True, "dtype": "c128", "data": "synthetic",
+                          "config": {"workload": "Hubbard 4x4 PBC t=1 U=4 Nup=Ndown=8, plain H·v", **c2}}),
+              flush=True)
         return
     if args.cfg4:
         c4 = cfg4_leg(args)
