@@ -587,6 +587,7 @@ struct Job {
   uint64_t seed = 0;
   double t_arrival = 0, t_admit = 0, t_done = 0;
   hphi_engine h;  // host-side preparation only (no device members used)
+  uint64_t shash = 0;  // structure_hash(h)
   Slab *slab = nullptr;
   int col = -1, slot = -1;
   int realc = 1;
@@ -624,6 +625,35 @@ void slab_repad(hphi_engine &h) {
   h.n_mixed = m[2];
 }
 
+// FNV-1a over what same_structure compares: equal structures hash equally
+uint64_t structure_hash(const hphi_engine &h) {
+  uint64_t x = 1469598103934665603ull;
+  auto mix = [&x](uint64_t v) {
+    for (int b = 0; b < 8; b++) {
+      x ^= (v >> (8 * b)) & 0xffu;
+      x *= 1099511628211ull;
+    }
+  };
+  mix((uint64_t)h.model);
+  mix((uint64_t)h.nsite);
+  mix((uint64_t)h.nup);
+  mix((uint64_t)h.ndown);
+  mix(h.dim);
+  mix((uint64_t)h.n_low);
+  mix((uint64_t)h.n_high);
+  mix((uint64_t)h.n_mixed);
+  mix((uint64_t)h.h_terms_dev.size());
+  mix((uint64_t)h.h_dterms.size());
+  for (const GTerm &t : h.h_terms_dev) {
+    mix(t.flip);
+    mix(t.tmask);
+    mix(t.tval);
+    mix(t.sgn);
+    mix((uint64_t)t.kind);
+  }
+  return x;
+}
+
 // Jobs share a slab when basis and term structure agree (couplings may differ).
 bool same_structure(const hphi_engine &a, const hphi_engine &b) {
   if (a.model != b.model || a.nsite != b.nsite || a.nup != b.nup || a.ndown != b.ndown || a.dim != b.dim ||
@@ -640,6 +670,7 @@ bool same_structure(const hphi_engine &a, const hphi_engine &b) {
 
 struct Slab {
   hphi_engine proto;  // structure: the first job's preparation (host side)
+  uint64_t shash = 0;  // structure_hash(proto)
   size_t off = 0, bytes = 0;
   uint64_t *list1 = nullptr, *l22 = nullptr, *jb = nullptr;
   uint32_t *l21 = nullptr;
@@ -779,6 +810,7 @@ extern "C" int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint
     return hphi_set_error("the service runs states of at most 32 bits");
   }
   slab_repad(j->h);
+  j->shash = structure_hash(j->h);
   for (const GTerm &t : j->h.h_terms_dev)
     if (t.cim != 0.0) j->realc = 0;
   const size_t fp = slab_footprint(j->h, s->p.max_iter);
@@ -818,6 +850,7 @@ static int slab_create(hphi_server *s, const Job *j, size_t off, size_t bytes, S
   if (s->stage_used + al256(tb) > s->stage_size) return 1;
   Slab *b = new Slab();
   b->proto = h;
+  b->shash = j->shash;
   b->off = off;
   b->bytes = bytes;
   char *p = s->d_arena + off;
@@ -898,14 +931,55 @@ static void retire(hphi_server *s, Job *j, double now, int converged, const doub
 
 // admission in arrival order: a free column of a slab of the same structure,
 // else a new slab while the arena has room
+// The job takes column c of running slab b (slot and bookkeeping); 1: retry
+// next tick (staging full)
+static int admit_into(hphi_server *s, Job *j, Slab *b, int c, double now) {
+  j->slot = s->free_slots.back();
+  const int rc = column_start(s, b, c, j);
+  if (rc) return rc;
+  s->free_slots.pop_back();
+  j->t_admit = now;
+  s->active.push_back(j);
+  return 0;
+}
+
+// Free columns of running slabs go first to arrived jobs of their structure
+// among the first ADMIT_WINDOW waiting jobs, ahead of queue order: a slab step
+// costs about the same however many of its columns run, so full slabs advance
+// more jobs per launch.  The rest is admitted in arrival order (admit).
+#define ADMIT_WINDOW 256
+static int admit_fill(hphi_server *s, double now) {
+  for (Slab *b : s->slabs) {
+    while (b->used() < SL_K && !s->free_slots.empty()) {
+      int c = -1;
+      for (int q = SL_K - 1; q >= 0; q--)
+        if (!b->col[q]) c = q;
+      auto it = s->waiting.begin();
+      int seen = 0;
+      for (; it != s->waiting.end() && seen < ADMIT_WINDOW; ++it, ++seen) {
+        const Job *w = *it;
+        if (w->t_arrival <= now && w->shash == b->shash && same_structure(b->proto, w->h)) break;
+      }
+      if (it == s->waiting.end() || seen == ADMIT_WINDOW) break;
+      Job *j = *it;
+      const int rc = admit_into(s, j, b, c, now);
+      if (rc == 1) return 0;
+      if (rc) return -1;
+      s->waiting.erase(it);
+    }
+  }
+  return 0;
+}
+
 static int admit(hphi_server *s, double now) {
+  if (admit_fill(s, now)) return -1;
   while (!s->waiting.empty() && !s->free_slots.empty()) {
     Job *j = s->waiting.front();
     if (j->t_arrival > now) break;
     Slab *b = nullptr;
     int c = -1;
     for (Slab *x : s->slabs) {
-      if (x->used() == SL_K || !same_structure(x->proto, j->h)) continue;
+      if (x->used() == SL_K || x->shash != j->shash || !same_structure(x->proto, j->h)) continue;
       b = x;
       for (int q = SL_K - 1; q >= 0; q--)
         if (!x->col[q]) c = q;
