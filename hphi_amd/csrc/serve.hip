@@ -66,7 +66,7 @@
 #define SL_K 4      // columns (jobs) per slab
 #define SL_U 2      // terms per gather batch (SL_U * SL_K gathers in flight)
 #define SL_SMAX 16  // Lanczos steps per tick at most
-#define SL_TPW 4    // tiles per workgroup (a chunk)
+#define SL_TPW 1    // tiles per workgroup (a chunk): 1, 2, 4, 8 measured, 1 fastest at every job size
 
 struct SlabDev {
   uint64_t dim;
