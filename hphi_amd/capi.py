@@ -140,6 +140,7 @@ def lib():
         L.hphi_diagonalcalc.argtypes = [_vp, _f64p]
         L.hphi_mltply.argtypes = [_vp, _vp, _vp, _f64p]
         L.hphi_mltply_dev.argtypes = [_vp, _vp, _vp, _f64p]
+        L.hphi_mltply_dev_scaled.argtypes = [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, _f64p]
         L.hphi_mltply_block_dev.argtypes = [_vp, C.c_int, _vp, _vp, _f64p]
         L.hphi_dev_alloc.argtypes = [C.c_size_t, C.POINTER(_vp)]
         L.hphi_dev_free.argtypes = [_vp]
@@ -148,6 +149,7 @@ def lib():
         L.hphi_dev_zero.argtypes = [_vp, C.c_size_t]
         L.hphi_fill_random.argtypes = [_vp, C.c_uint64, C.c_int, C.c_uint64]
         L.hphi_vec_lanczos_beta.argtypes = [_vp, _vp, _vp, C.c_double, _f64p]
+        L.hphi_vec_lanczos_resid.argtypes = [_vp, _vp, _vp, _vp, C.c_double, C.c_double, _f64p]
         L.hphi_vec_lanczos_update.argtypes = [_vp, _vp, _vp, C.c_double, C.c_double]
         L.hphi_vec_norm2.argtypes = [_vp, _vp, C.c_int, _f64p]
         L.hphi_vec_dot.argtypes = [_vp, _vp, _vp, C.c_int, _f64p]
@@ -344,6 +346,21 @@ class Engine:
         pr = np.zeros(2)
         check(lib().hphi_mltply_dev(self.h, v0.ptr, v1.ptr, _p(pr, _f64p)))
         return complex(pr[0], pr[1])
+
+    def mltply_dev_scaled(self, v0: DeviceArray, v1: DeviceArray, c0: float, c1: float,
+                          overwrite: bool = False) -> complex:
+        """v0 = c0 v0 + c1 H v1 (v0 = c1 H v1 with overwrite); returns <v1|H|v1>."""
+        pr = np.zeros(2)
+        check(lib().hphi_mltply_dev_scaled(self.h, v0.ptr, v1.ptr, c0, c1, 1 if overwrite else 0, _p(pr, _f64p)))
+        return complex(pr[0], pr[1])
+
+    def lanczos_resid(self, r: DeviceArray, w: DeviceArray, vg: Optional[DeviceArray], f: float,
+                      cg: float = 0.0) -> float:
+        """r <- r - f w (and vg += cg w); returns |r|^2."""
+        out = C.c_double()
+        check(lib().hphi_vec_lanczos_resid(self.h, r.ptr, w.ptr, vg.ptr if vg is not None else None, f, cg,
+                                           C.byref(out)))
+        return out.value
 
     def mltply_block_dev(self, k: int, V0: DeviceArray, V1: DeviceArray) -> np.ndarray:
         pr = np.zeros(2 * k)

@@ -336,3 +336,70 @@ def test_tpq_gaussian_start_is_normalised(A):
     # |w_0|^2 of a complex Gaussian vector ~ 2 dim
     assert np.all(np.abs(out[0, :, 3] ** 2 / (2 * eng.dim) - 1.0) < 0.1)
     eng.close()
+
+
+@pytest.mark.parametrize("name", ["chain16", "hubbard_4x2", "tri_4x3_gc", "hubbard_chain6_complex"])
+def test_scaled_hv_matches_oracle(A, oracle_lib, name):
+    """hphi_mltply_dev_scaled: v0 = c0 v0 + c1 H v1 (accumulate) and c1 H v1
+    (overwrite), with prdct = <v1|H|v1>, against the oracle's H·v."""
+    print(f"[gpu] scaled hv {name}", flush=True)
+    spec = golden_cases()[name]
+    eng = A.Engine.from_spec(spec)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    v1h = oracle_lib.fill_vector(SEED, eng.dim)
+    v0h = oracle_lib.fill_vector(SEED + 1, eng.dim)
+    ohv, opr = orc.hv(v1h)
+    c0, c1 = -0.7, 1.3
+    v1 = A.DeviceArray(eng.dim).upload(v1h)
+    for overwrite in (False, True):
+        v0 = A.DeviceArray(eng.dim).upload(v0h)
+        pr = eng.mltply_dev_scaled(v0, v1, c0, c1, overwrite)
+        want = c1 * ohv + (0.0 if overwrite else c0 * v0h)
+        assert rel_inf(v0.download(), want) <= HV_TOL
+        assert prdct_ok(pr, opr, v1h, ohv)
+        v0.free()
+    eng.close()
+
+
+def test_lanczos_residual_pass(A):
+    """hphi_vec_lanczos_resid: r <- r - f w, |r|^2, and vg += cg w."""
+    spec = golden_cases()["chain16"]
+    eng = A.Engine.from_spec(spec)
+    rng = np.random.default_rng(3)
+    n = eng.dim
+    rh, wh, gh = (rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3))
+    r, w, g = (A.DeviceArray(n).upload(x) for x in (rh, wh, gh))
+    f, cg = 0.37, -1.9
+    r2 = eng.lanczos_resid(r, w, g, f, cg)
+    want = rh - f * wh
+    assert rel_inf(r.download(), want) <= 1e-15
+    assert rel_inf(g.download(), gh + cg * wh) <= 1e-15
+    assert abs(r2 - np.vdot(want, want).real) <= 1e-13 * r2
+    r3 = eng.lanczos_resid(w, r, None, 0.0)  # without the eigenvector sum, f = 0: |w|^2
+    assert abs(r3 - np.vdot(wh, wh).real) <= 1e-13 * r3
+    for x in (r, w, g):
+        x.free()
+    eng.close()
+
+
+@pytest.mark.parametrize("which", ["torus_4x2_gc", "chain10_doubled"])
+def test_repeated_terms_merge_to_the_same_hamiltonian(A, oracle_lib, which):
+    """Identical compiled terms are merged into one with the summed coupling;
+    the oracle applies every listed entry separately.  The 4x2 triangular torus
+    lists its a2 bonds twice (24 entries, 20 pairs), as the 13x2 one does; the
+    chain lists every exchange and Ising bond twice."""
+    print(f"[gpu] merged terms {which}", flush=True)
+    if which == "torus_4x2_gc":
+        spec = L.triangular_xxz_gc(4, 2, 1.0, 0.8, 0.1)
+    else:
+        spec = L.heisenberg_chain(10)
+        spec.exchange = spec.exchange + spec.exchange
+        spec.ising = spec.ising + spec.ising
+    eng = A.Engine.from_spec(spec)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    v = oracle_lib.fill_vector(SEED, eng.dim)
+    hv, pr = eng.hv(v)
+    ohv, opr = orc.hv(v)
+    assert rel_inf(hv, ohv) <= HV_TOL
+    assert prdct_ok(pr, opr, v, ohv)
+    eng.close()
