@@ -40,6 +40,28 @@ def b_alg(dim: int, nnz: int, basis_bytes: int = 8, v0_bytes: int = 32) -> int:
     return 16 * nnz + dim * (16 + v0_bytes + 8 + basis_bytes)
 
 
+def plain_nnz(spec) -> int:
+    """Off-diagonal elements of one H·v of a canonical sector: an exchange bond acts
+    where its two spins differ, C(N-2, Nup-1) states each way; a directed hop
+    c+_{i s} c_{j s} where orbital (j, s) is filled and (i, s) empty,
+    C(N-2, n_s - 1) C(N, n_other) states."""
+    from hphi_amd import lattice as L
+
+    n = spec.nsite
+    if spec.model == L.SPIN:
+        return len(spec.exchange) * 2 * math.comb(n - 2, spec.nup - 1)
+    nnz = 0
+    for (i, si, j, sj, _, _) in spec.trans:
+        ns, no = (spec.nup, spec.ndown) if si == 0 else (spec.ndown, spec.nup)
+        nnz += math.comb(n - 2, ns - 1) * math.comb(n, no)
+    return nnz
+
+
+# per workload of the HBM-traffic runs: profile name, title, seed of the probe vector
+PMC_WORKLOADS = {3: ("hv_j1j2_8x4", "J1-J2 8x4 Sz=0, dim 601,080,390", 2),
+                 2: ("hv_hubbard_4x4", "Hubbard 4x4 Nup=Ndown=8, dim 165,636,900", 1)}
+
+
 def traffic_from_profiles(name: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     copy-kernel calibrated), recorded under profiles/ by the profiling runs."""
@@ -62,8 +84,7 @@ def gpu_leg(args):
     eng.diagonalcalc(copy=False)
     t_setup = time.time() - t0
     dim = eng.dim
-    nb = len(spec.exchange)
-    nnz = nb * 2 * math.comb(spec.nsite - 2, spec.nup - 1)
+    nnz = plain_nnz(spec)
     balg = b_alg(dim, nnz)
     v1 = capi.DeviceArray(dim)
     v0 = capi.DeviceArray(dim)
@@ -166,13 +187,7 @@ def cfg2_leg(args):
     eng.sz(copy=False)
     eng.diagonalcalc(copy=False)
     dim = eng.dim
-    # a directed hop c+_{i s} c_{j s} acts where orbital (j, s) is filled and (i, s)
-    # empty: C(N-2, n_s - 1) C(N, n_other) states
-    n = spec.nsite
-    nnz = 0
-    for (i, si, j, sj, _, _) in spec.trans:
-        ns, no = (spec.nup, spec.ndown) if si == 0 else (spec.ndown, spec.nup)
-        nnz += math.comb(n - 2, ns - 1) * math.comb(n, no)
+    nnz = plain_nnz(spec)
     balg = b_alg(dim, nnz)
     v1 = capi.DeviceArray(dim)
     v0 = capi.DeviceArray(dim)
@@ -186,7 +201,8 @@ def cfg2_leg(args):
     lz = eng.lanczos(n_eigen=1, max_iter=args.lanczos_steps, eps=1e-300, target=2, seed=1)
     out = {"dim": dim, "nnz_offdiag": nnz, "bytes_alg_per_hv": balg, "hv_ms": round(ms, 3),
            "hv_gbs": round(balg / (ms / 1e3) / 1e9, 1), "hbm_frac": round(balg / (ms / 1e3) / 8e12, 4),
-           "lanczos_step_ms": round(1e3 * lz.seconds / max(lz.iterations, 1), 3)}
+           "lanczos_step_ms": round(1e3 * lz.seconds / max(lz.iterations, 1), 3),
+           "traffic_bytes_per_hv": traffic_from_profiles("hv_hubbard_4x4")}
     eng.close()
     return out
 
@@ -199,19 +215,20 @@ def pmc_probe(args):
     from hphi_amd import capi, lattice as L
     import numpy as np
 
-    eng = capi.Engine.from_spec(L.baseline_config(3))
+    name, _, seed = PMC_WORKLOADS[args.pmc_cfg]
+    eng = capi.Engine.from_spec(L.baseline_config(args.pmc_cfg))
     eng.sz(copy=False)
     eng.diagonalcalc(copy=False)
     dim = eng.dim
     v1 = capi.DeviceArray(dim)
     v0 = capi.DeviceArray(dim)
-    eng.fill_random(v1, 1, 2)
+    eng.fill_random(v1, 1, seed)
     v0.zero()
     ms = eng.time_mltply(1, v0, v1, 0, args.steps)
     ab = np.array([0.5, 0.0, 0.25, 0.0])
     for _ in range(args.steps):
         capi.check(capi.lib().hphi_vec_axpby(eng.h, v1.ptr, v0.ptr, 1, ab.ctypes.data_as(C.POINTER(C.c_double))))
-    print(json.dumps({"probe": "hv_j1j2_8x4", "dim": dim, "hv_ms": ms, "steps": args.steps,
+    print(json.dumps({"probe": name, "dim": dim, "hv_ms": ms, "steps": args.steps,
                       "calib_read_bytes": 32 * dim, "calib_write_bytes": 16 * dim}), flush=True)
     v0.free()
     v1.free()
@@ -261,21 +278,22 @@ def summarize_stats(csv_path: str, out: str, title: str, cmd: str, kernel: str, 
     print("\n".join(lines[:4]))
 
 
-def reduce_pmc(fetch_dir: str, write_dir: str, out: str, cmd: str):
+def reduce_pmc(fetch_dir: str, write_dir: str, out: str, cmd: str, cfg: int = 3):
     """HBM bytes per H·v launch from a FETCH_SIZE and a WRITE_SIZE run of
     --pmc-probe, each scaled by what the counter reads for the calibration
     kernel of known traffic."""
     from hphi_amd import lattice as L
 
-    dim = L.baseline_config(3).dim
+    spec = L.baseline_config(cfg)
+    dim = spec.dim
     f, w = _pmc_means(fetch_dir, "FETCH_SIZE"), _pmc_means(write_dir, "WRITE_SIZE")
     ff = (32 * dim / 1024) / f["k_axpby"]
     fw = (16 * dim / 1024) / w["k_axpby"]
     fetch_b = f["k_hv_canon"] * 1024 * ff
     write_b = w["k_hv_canon"] * 1024 * fw
-    nnz = 128 * 2 * math.comb(30, 15)
+    nnz = plain_nnz(spec)
     lines = [
-        "# HBM traffic of one plain H·v launch (k_hv_canon), J1-J2 8x4 Sz=0, dim 601,080,390",
+        f"# HBM traffic of one plain H·v launch (k_hv_canon), {PMC_WORKLOADS[cfg][1]}",
         f"# command: {cmd}",
         f"FETCH_SIZE k_hv_canon (KiB/dispatch, raw): {f['k_hv_canon']:.1f}",
         f"FETCH_SIZE k_axpby calibration (KiB/dispatch, raw): {f['k_axpby']:.1f}  known: {32 * dim / 1024:.1f}"
@@ -409,6 +427,7 @@ def main():
     ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic_hv_j1j2_8x4.txt from two rocprofv3 --pmc runs of --pmc-probe")
     ap.add_argument("--pmc-cmd", type=str, default="", help="command line recorded in the traffic file")
+    ap.add_argument("--pmc-cfg", type=int, default=3, choices=(2, 3), help="workload of --pmc-probe / --reduce-pmc")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("KERNEL_STATS_CSV", "OUT"),
                     help="write a text summary of a rocprofv3 --stats run (with --stats-cmd, --stats-title)")
@@ -432,8 +451,9 @@ def main():
                         args.stats_kernel, b_alg(601080390, 39710085120))
         return
     if args.reduce_pmc:
-        reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1], os.path.join(ROOT, "profiles", "traffic_hv_j1j2_8x4.txt"),
-                   args.pmc_cmd)
+        reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1],
+                   os.path.join(ROOT, "profiles", f"traffic_{PMC_WORKLOADS[args.pmc_cfg][0]}.txt"), args.pmc_cmd,
+                   args.pmc_cfg)
         return
     if args.pmc_probe:
         pmc_probe(args)
