@@ -112,10 +112,28 @@ def gpu_leg(args):
         blk = eng.time_mltply(k, V0, V1, 1, 2) / k
         V0.free()
         V1.free()
+    device_gb = (eng.device_bytes + 2 * 16 * dim) / 1e9  # the device-resident path, before any staging buffer
+    host = None
+    if args.host_reps > 0:  # compatibility mode: the reference's 1-indexed host arrays, staged per call
+        import numpy as np
+
+        h1 = np.full(dim + 1, 1.0 / math.sqrt(dim), np.complex128)
+        h1[0] = 0.0
+        h0 = np.zeros(dim + 1, np.complex128)
+        eng.mltply(h0, h1)  # the first call allocates the staging buffers
+        t = time.time()
+        for _ in range(args.host_reps):
+            eng.mltply(h0, h1)
+        hs = (time.time() - t) / args.host_reps
+        host = {"ms_per_call": round(1e3 * hs, 1), "gbs_algorithmic": round(balg / hs / 1e9, 1),
+                "pcie_bytes_per_call": 3 * 16 * dim, "reps": args.host_reps,
+                "note": "hphi_mltply on host arrays: v1 and v0 copied in, v0 copied out on every call; "
+                        "a labelled compatibility rate, never the value"}
+        del h0, h1
     res = {
-        "dim": dim, "nnz": nnz, "balg": balg, "ms": ms, "wall_s": wall, "setup_s": t_setup,
+        "dim": dim, "nnz": nnz, "balg": balg, "ms": ms, "wall_s": wall, "setup_s": t_setup, "host": host,
         "lanczos_step_ms": 1e3 * lz.seconds / max(lz.iterations, 1),
-        "device_gb": (eng.device_bytes + 2 * 16 * dim) / 1e9,
+        "device_gb": device_gb,
         "block_ms_per_column": blk,
     }
     eng.close()
@@ -429,6 +447,7 @@ def main():
     ap.add_argument("--pmc-cmd", type=str, default="", help="command line recorded in the traffic file")
     ap.add_argument("--pmc-cfg", type=int, default=3, choices=(2, 3), help="workload of --pmc-probe / --reduce-pmc")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-reps", type=int, default=2, help="timed host-pointer (compatibility mode) calls; 0: skip")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("KERNEL_STATS_CSV", "OUT"),
                     help="write a text summary of a rocprofv3 --stats run (with --stats-cmd, --stats-title)")
     ap.add_argument("--stats-cmd", type=str, default="")
@@ -503,7 +522,8 @@ def main():
                    "block_ms_per_column": round(g["block_ms_per_column"], 3) if g["block_ms_per_column"] else None,
                    "block_hbm_frac_per_column": round(g["balg"] / (g["block_ms_per_column"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
                    if g["block_ms_per_column"] else None,
-                   "device_gb": round(g["device_gb"], 2), "setup_s": round(g["setup_s"], 2)},
+                   "device_gb": round(g["device_gb"], 2), "setup_s": round(g["setup_s"], 2),
+                   "host_pointer_mode": g["host"]},
         "roofline": {"bound": "hbm", "achieved": round(value, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(value / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles("hv_j1j2_8x4")},
