@@ -449,17 +449,16 @@ __device__ int ck_sturm(const double *a, const double *b, int n, double x) {
   return c;
 }
 
+#define CK_LMAX 256  // largest window of T_m eigenvalues examined
 __global__ __launch_bounds__(64) void k_sl_check(const ColDev *cols, int want, double *out) {
   CCol J = (CCol)cols + blockIdx.x;
   if (!J->check) return;
   const int m = J->step;
   const double *a = J->ahist + 1, *b = J->bhist + 1;
-  __shared__ double ev[32], evh[32];
+  __shared__ double ev[CK_LMAX], evh[CK_LMAX];
   __shared__ double lo_s, hi_s, scale_s;
-  int L = want + CK_EXTRA;
-  if (L > m) L = m;
-  if (L > 32) L = 32;
-  const int Lh = L < m - 1 ? L : m - 1;
+  __shared__ int next_s;
+  const int cap = m < CK_LMAX ? m : CK_LMAX;
   if (threadIdx.x == 0) {
     double lo = a[0], hi = a[0], sc = 0.0;
     for (int i = 0; i < m; i++) {
@@ -475,43 +474,58 @@ __global__ __launch_bounds__(64) void k_sl_check(const ColDev *cols, int want, d
   }
   __syncthreads();
   const int t = threadIdx.x;
-  const bool main = t < L, hat = (t >= 32) && (t - 32 < Lh);
-  if (main || hat) {
-    const int idx = main ? t : t - 32;
-    const double *aa = main ? a : a + 1, *bb = main ? b : b + 1;
-    const int n = main ? m : m - 1;
-    double l = lo_s, h = hi_s;
-    for (int it = 0; it < 200; it++) {
-      const double mid = 0.5 * (l + h);
-      if (mid <= l || mid >= h) break;
-      if (ck_sturm(aa, bb, n, mid) > idx) h = mid;
-      else l = mid;
+  // Without reorthogonalisation converged eigenvalues keep gaining ghost copies,
+  // which fill a fixed window of T_m's lowest eigenvalues as m grows: the window
+  // starts at want + CK_EXTRA and doubles until it holds `want` good values or
+  // reaches min(m, CK_LMAX).  Each round bisects only the new indices.
+  int L0 = 0, L = want + CK_EXTRA < cap ? want + CK_EXTRA : cap;
+  for (;;) {
+    const int Lh = L < m - 1 ? L : m - 1, Lh0 = L0 < m - 1 ? L0 : m - 1;
+    const bool hat = t >= 32;
+    const int lane = t & 31;
+    const double *aa = hat ? a + 1 : a, *bb = hat ? b + 1 : b;
+    const int n = hat ? m - 1 : m;
+    for (int idx = (hat ? Lh0 : L0) + lane; idx < (hat ? Lh : L); idx += 32) {
+      double l = lo_s, h = hi_s;
+      for (int it = 0; it < 200; it++) {
+        const double mid = 0.5 * (l + h);
+        if (mid <= l || mid >= h) break;
+        if (ck_sturm(aa, bb, n, mid) > idx) h = mid;
+        else l = mid;
+      }
+      if (hat) evh[idx] = 0.5 * (l + h);
+      else ev[idx] = 0.5 * (l + h);
     }
-    if (main) ev[idx] = 0.5 * (l + h);
-    else evh[idx] = 0.5 * (l + h);
-  }
-  __syncthreads();
-  if (t == 0) {
-    double *o = out + (uint64_t)J->job * 17;
-    const double tol = 1e-11 * (scale_s > 1.0 ? scale_s : 1.0);
-    int got = 0;
-    if (m == 1) {
-      o[0] = a[0];
-      got = 1;
+    __syncthreads();
+    if (t == 0) {
+      double *o = out + (uint64_t)J->job * 17;
+      const double tol = 1e-11 * (scale_s > 1.0 ? scale_s : 1.0);
+      int got = 0;
+      if (m == 1) {
+        o[0] = a[0];
+        got = 1;
+      }
+      for (int i = 0; i < L && got < want && m > 1; i++) {
+        int mult = 0;
+        for (int j = 0; j < L; j++)
+          if (fabs(ev[j] - ev[i]) <= tol) mult++;
+        bool spurious = false;
+        if (mult == 1 && i < L - 1)
+          for (int j = 0; j < Lh; j++)
+            if (fabs(evh[j] - ev[i]) <= tol) spurious = true;
+        if (spurious) continue;
+        if (got > 0 && fabs(ev[i] - o[got - 1]) <= tol) continue;
+        o[got++] = ev[i];
+      }
+      o[16] = (double)got;
+      next_s = (got >= want || L >= cap) ? 0 : (2 * L < cap ? 2 * L : cap);
     }
-    for (int i = 0; i < L && got < want && m > 1; i++) {
-      int mult = 0;
-      for (int j = 0; j < L; j++)
-        if (fabs(ev[j] - ev[i]) <= tol) mult++;
-      bool spurious = false;
-      if (mult == 1 && i < L - 1)
-        for (int j = 0; j < Lh; j++)
-          if (fabs(evh[j] - ev[i]) <= tol) spurious = true;
-      if (spurious) continue;
-      if (got > 0 && fabs(ev[i] - o[got - 1]) <= tol) continue;
-      o[got++] = ev[i];
-    }
-    o[16] = (double)got;
+    __syncthreads();
+    const int nx = next_s;
+    if (nx == 0) break;
+    L0 = L;
+    L = nx;
+    __syncthreads();  // next_s is read by every thread before thread 0 writes it again
   }
 }
 

@@ -168,3 +168,42 @@ def test_tridiagonal_eigensolver(capi):
     a = np.array([1.0, 1.0, 3.0])
     b = np.array([0.0, 0.0])
     assert np.allclose(capi.tridiag_lowest(a, b, 3), [1.0, 1.0, 3.0])
+
+
+def test_good_eigenvalues_survive_many_ghost_copies(capi, oracle_lib):
+    """Plain Lanczos without reorthogonalisation keeps adding ghost copies of
+    converged eigenvalues.  After hundreds of steps the lowest eigenvalues of T_m
+    are mostly copies, and the Cullum-Willoughby filter must look past them:
+    the lowest 4 distinct good values equal the dense spectrum's at every
+    check, from step 60 to step 600 of a 12-site chain (dim 924)."""
+    from tests import brute
+    spec = L.heisenberg_chain(12, 1.0, 0.3)
+    w = np.linalg.eigvalsh(brute.dense(spec))
+    distinct = [w[0]]
+    for x in w[1:]:
+        if x - distinct[-1] > 1e-8:
+            distinct.append(x)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    n = orc.dim
+    v = oracle_lib.fill_vector(5, n)
+    v = v / np.linalg.norm(v)
+    vprev = np.zeros(n, complex)
+    alpha, beta, b = [], [], 0.0
+    for _ in range(600):
+        x = orc.hv(v)[0] - b * vprev
+        a = np.vdot(v, x).real
+        r = x - a * v
+        b = np.linalg.norm(r)
+        alpha.append(a)
+        beta.append(b)
+        vprev, v = v, r / b
+    A = np.array(alpha)
+    B = np.array(beta)
+    dp = C.POINTER(C.c_double)
+    fn = capi.lib().hphi_lanczos_good_eigenvalues
+    fn.argtypes = [dp, dp, C.c_int, C.c_int, dp]
+    for m in range(60, 601, 10):
+        out = np.zeros(16)
+        k = fn(A.ctypes.data_as(dp), B.ctypes.data_as(dp), m, 4, out.ctypes.data_as(dp))
+        assert k == 4, (m, k, out[:k])
+        assert np.allclose(out[:4], distinct[:4], atol=1e-8, rtol=0), (m, out[:4], distinct[:4])
