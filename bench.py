@@ -447,6 +447,8 @@ def main():
     ap.add_argument("--pmc-cmd", type=str, default="", help="command line recorded in the traffic file")
     ap.add_argument("--pmc-cfg", type=int, default=3, choices=(2, 3), help="workload of --pmc-probe / --reduce-pmc")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-sample", type=int, default=1000,
+                    help="requests of the served stream timed with the default run (0: skip)")
     ap.add_argument("--host-reps", type=int, default=2, help="timed host-pointer (compatibility mode) calls; 0: skip")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("KERNEL_STATS_CSV", "OUT"),
                     help="write a text summary of a rocprofv3 --stats run (with --stats-cmd, --stats-title)")
@@ -528,6 +530,19 @@ def main():
                      "frac": round(value / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles("hv_j1j2_8x4")},
     }
+    if args.serve_sample > 0:  # the metric's served half, on a sample of the stream
+        sa = argparse.Namespace(**vars(args))
+        sa.requests, sa.rates, sa.seq_sample = args.serve_sample, "0,120", min(args.seq_sample, 40)
+        sv = serve_leg(sa)
+        p0 = next(p for p in sv["points"] if p["rate"] == 0.0)
+        p1 = next(p for p in sv["points"] if p["rate"] == 120.0)
+        line["config"]["served_sample"] = {
+            "requests": sa.requests, "solves_per_s": round(p0["solves_per_s"], 2),
+            "sequential_solves_per_s": round(sv["sequential_solves_per_s"], 2),
+            "speedup_vs_sequential": round(p0["solves_per_s"] / sv["sequential_solves_per_s"], 3),
+            "rate_per_s": 120.0, "p50_s": round(p1["p50_s"], 3), "p99_s": round(p1["p99_s"], 3),
+            "converged": [p0["converged"], p1["converged"]],
+            "note": "first requests of the seed-3 stream; bench.py --serve --requests 10000 runs the whole stream"}
     if not args.no_cpu:
         c = cpu_leg(args)
         line["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample")}
