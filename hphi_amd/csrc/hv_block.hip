@@ -355,18 +355,19 @@ void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_pe
     case 16: c = e->bk_cpl[3]; break;
   }
   const bool canon = !hphi_model_gc(e->model);
-  // grand canonical, k = 16: one column per lane and 4 workgroups per CU (13x2
-  // triangular SpinGC: 137 ms per block H·v vs 156-170 ms for 4 or 16 columns per
-  // lane and 8-32 workgroups per CU; mTPQ step 143.8 ms vs 162.6 ms)
-  const bool gc16 = !canon && k == 16;
-  if (c <= 0 || !hphi_blk_cpl_ok(k, c)) c = gc16 ? 1 : (k >= 8 ? 4 : k);
+  // grand canonical, k = 8 and 16: one column per lane and 4 workgroups per CU,
+  // XCD-contiguous (13x2 triangular SpinGC: k = 16 137 ms per block H·v vs 156-170
+  // ms for 4 or 16 columns per lane and 8-32 workgroups per CU, mTPQ step 143.8 ms
+  // vs 162.6 ms; k = 8 62.3 ms vs 69-100 ms)
+  const bool gc_wide = !canon && k >= 8;
+  if (c <= 0 || !hphi_blk_cpl_ok(k, c)) c = gc_wide ? 1 : (k >= 8 ? 4 : k);
   *cpl = c;
   int t = e->bk_tpb;
   if (t != 256 && t != 1024) t = (canon && !e->c2_crank && 2 * e->size21 > 32768) ? 1024 : 256;
   if (!canon || e->c2_crank) t = 256;  // the rank tables leave room for many 256-thread workgroups
   *tpb = t;
   int w = e->bk_wgs_per_cu;
-  if (w <= 0) w = (t == 1024) ? 1 : (canon && e->c2_crank ? 48 : (gc16 ? 4 : 8));  // 8x4 J1-J2, k=4: 106 ms/column
+  if (w <= 0) w = (t == 1024) ? 1 : (canon && e->c2_crank ? 48 : (gc_wide ? 4 : 8));  // 8x4 J1-J2, k=4: 106 ms/column
   *wgs_per_cu = w;
 }
 
@@ -385,7 +386,7 @@ int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid) {
   // groups are padded to HV_TERM_PAD, a multiple of every batch size)
   const int u = cpl >= 8 ? 1 : 8 / cpl;  // terms per batch of k_hv_blk
   HvArgs B = A;
-  if (!canon && k == 16 && !e->xcd_tuned) B.xcd_map = 1;  // with the grand-canonical k = 16 shape above
+  if (!canon && k >= 8 && !e->xcd_tuned) B.xcd_map = 1;  // with the grand-canonical k >= 8 shape above
   B.e_low = std::min(A.n_low, (e->r_low + u - 1) / u * u);
   B.e_high = std::min(A.n_high, (e->r_high + u - 1) / u * u);
   B.e_mixed = std::min(A.n_mixed, (e->r_mixed + u - 1) / u * u);

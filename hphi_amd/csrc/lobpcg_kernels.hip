@@ -171,42 +171,56 @@ __global__ __launch_bounds__(LB_TPB) void k_lb_update(const double2 *w, double2 
                                                       double2 *X, double2 *P, uint64_t dim, const double2 *coef,
                                                       double *partial) {
   constexpr int R = 3 * K;
+  constexpr int SPB = LB_TPB / K;  // states per tile: one (state, column) element per thread
   __shared__ double2 cf[R * K];
+  __shared__ double2 rows[6][LB_TPB];  // the tile's w, x, p, W, X, P rows, one coalesced load each
   for (int e = threadIdx.x; e < R * K; e += LB_TPB) cf[e] = coef[e];
-  __syncthreads();
-  const int c = threadIdx.x % K;
-  constexpr int SPB = LB_TPB / K;
+  const int c = threadIdx.x % K, sl = threadIdx.x / K;
+  const uint64_t n = dim * K, ntiles = (dim + SPB - 1) / SPB;
   double nx = 0.0, np = 0.0;
-  for (uint64_t j = (uint64_t)blockIdx.x * SPB + threadIdx.x / K; j < dim; j += (uint64_t)gridDim.x * SPB) {
-    double2 xa = make_double2(0.0, 0.0), pa = xa, Xa = xa, Pa = xa;
-    const uint64_t row = j * K;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t e = t * LB_TPB + threadIdx.x;  // = j * K + c
+    const bool ok = e < n;
+    const double2 z = make_double2(0.0, 0.0);
+    rows[0][threadIdx.x] = ok ? w[e] : z;
+    rows[1][threadIdx.x] = ok ? x[e] : z;
+    rows[2][threadIdx.x] = ok ? p[e] : z;
+    rows[3][threadIdx.x] = ok ? W[e] : z;
+    rows[4][threadIdx.x] = ok ? X[e] : z;
+    rows[5][threadIdx.x] = ok ? P[e] : z;
+    __syncthreads();
+    if (ok) {
+      double2 xa, pa = z, Xa, Pa = z;
+      const int row = sl * K;
 #pragma unroll 4
-    for (int r = 0; r < K; r++) {  // w rows: into x' and p'
-      const double2 k0 = cf[r * K + c], bv = w[row + r], hv = W[row + r];
-      const double2 tb = make_double2(k0.x * bv.x - k0.y * bv.y, k0.x * bv.y + k0.y * bv.x);
-      const double2 th = make_double2(k0.x * hv.x - k0.y * hv.y, k0.x * hv.y + k0.y * hv.x);
-      pa.x += tb.x, pa.y += tb.y, Pa.x += th.x, Pa.y += th.y;
-    }
+      for (int r = 0; r < K; r++) {  // w rows: into x' and p'
+        const double2 k0 = cf[r * K + c], bv = rows[0][row + r], hv = rows[3][row + r];
+        const double2 tb = make_double2(k0.x * bv.x - k0.y * bv.y, k0.x * bv.y + k0.y * bv.x);
+        const double2 th = make_double2(k0.x * hv.x - k0.y * hv.y, k0.x * hv.y + k0.y * hv.x);
+        pa.x += tb.x, pa.y += tb.y, Pa.x += th.x, Pa.y += th.y;
+      }
 #pragma unroll 4
-    for (int r = 0; r < K; r++) {  // p rows: into x' and p'
-      const double2 k0 = cf[(2 * K + r) * K + c], bv = p[row + r], hv = P[row + r];
-      pa.x += k0.x * bv.x - k0.y * bv.y, pa.y += k0.x * bv.y + k0.y * bv.x;
-      Pa.x += k0.x * hv.x - k0.y * hv.y, Pa.y += k0.x * hv.y + k0.y * hv.x;
-    }
-    xa = pa;
-    Xa = Pa;
+      for (int r = 0; r < K; r++) {  // p rows: into x' and p'
+        const double2 k0 = cf[(2 * K + r) * K + c], bv = rows[2][row + r], hv = rows[5][row + r];
+        pa.x += k0.x * bv.x - k0.y * bv.y, pa.y += k0.x * bv.y + k0.y * bv.x;
+        Pa.x += k0.x * hv.x - k0.y * hv.y, Pa.y += k0.x * hv.y + k0.y * hv.x;
+      }
+      xa = pa;
+      Xa = Pa;
 #pragma unroll 4
-    for (int r = 0; r < K; r++) {  // x rows: into x' only
-      const double2 k0 = cf[(K + r) * K + c], bv = x[row + r], hv = X[row + r];
-      xa.x += k0.x * bv.x - k0.y * bv.y, xa.y += k0.x * bv.y + k0.y * bv.x;
-      Xa.x += k0.x * hv.x - k0.y * hv.y, Xa.y += k0.x * hv.y + k0.y * hv.x;
+      for (int r = 0; r < K; r++) {  // x rows: into x' only
+        const double2 k0 = cf[(K + r) * K + c], bv = rows[1][row + r], hv = rows[4][row + r];
+        xa.x += k0.x * bv.x - k0.y * bv.y, xa.y += k0.x * bv.y + k0.y * bv.x;
+        Xa.x += k0.x * hv.x - k0.y * hv.y, Xa.y += k0.x * hv.y + k0.y * hv.x;
+      }
+      x[e] = xa;
+      X[e] = Xa;
+      p[e] = pa;
+      P[e] = Pa;
+      nx += xa.x * xa.x + xa.y * xa.y;
+      np += pa.x * pa.x + pa.y * pa.y;
     }
-    x[row + c] = xa;
-    X[row + c] = Xa;
-    p[row + c] = pa;
-    P[row + c] = Pa;
-    nx += xa.x * xa.x + xa.y * xa.y;
-    np += pa.x * pa.x + pa.y * pa.y;
+    __syncthreads();  // the next tile overwrites rows
   }
   // two column sums: reuse the per-column reduction twice
   __shared__ double red2[LB_TPB / 64][2][K];
