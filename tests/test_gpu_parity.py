@@ -403,3 +403,59 @@ def test_repeated_terms_merge_to_the_same_hamiltonian(A, oracle_lib, which):
     assert rel_inf(hv, ohv) <= HV_TOL
     assert prdct_ok(pr, opr, v, ohv)
     eng.close()
+
+
+@pytest.mark.parametrize("which", ["hubbard17_n2", "chain40_one_magnon"])
+def test_states_wider_than_32_bits(A, oracle_lib, which):
+    """Sectors of 34- and 40-bit states run the 64-bit kernels: H·v against the
+    oracle, and for the 40-site chain with one flipped spin the closed-form
+    one-magnon spectrum N J/4 - J (1 - cos 2 pi m / N) of the dense matrix the
+    engine's H·v builds column by column."""
+    print(f"[gpu] wide states {which}", flush=True)
+    if which == "hubbard17_n2":
+        spec = L.hubbard_chain(17, 1.0, 3.0, ncond=2, two_sz=0)
+    else:
+        spec = L.heisenberg_chain(40, 1.0, 0.0, two_sz=38)
+    assert spec.nbits > 32
+    eng = A.Engine.from_spec(spec)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    v = oracle_lib.fill_vector(SEED, eng.dim)
+    hv, pr = eng.hv(v)
+    ohv, opr = orc.hv(v)
+    assert rel_inf(hv, ohv) <= HV_TOL
+    assert prdct_ok(pr, opr, v, ohv)
+    if which == "chain40_one_magnon":
+        n = eng.dim
+        H = np.zeros((n, n), complex)
+        for c in range(n):
+            e = np.zeros(n, complex)
+            e[c] = 1.0
+            H[:, c] = eng.hv(e)[0]
+        assert np.max(np.abs(H - H.conj().T)) <= 1e-14
+        want = np.sort([40 / 4 - (1 - np.cos(2 * np.pi * m / 40)) for m in range(40)])
+        assert np.max(np.abs(np.linalg.eigvalsh(H) - want)) <= 1e-12
+    eng.close()
+
+
+def test_one_state_and_diagonal_only_sectors(A, oracle_lib):
+    """Degenerate inputs: the fully polarised 8-site chain has one state (H·v =
+    diag·v, and Lanczos returns N J/4 = 2 at once); an Ising-only chain has no
+    off-diagonal term at all (every term group empty)."""
+    spec = L.heisenberg_chain(8, 1.0, 0.0, two_sz=8)
+    eng = A.Engine.from_spec(spec)
+    assert eng.dim == 1
+    hv, pr = eng.hv(np.array([0.6 - 0.8j]))
+    assert abs(hv[0] - 2.0 * (0.6 - 0.8j)) <= 1e-15 and abs(pr - 2.0) <= 1e-15
+    r = eng.lanczos(n_eigen=1, max_iter=50, eps=1e-12, seed=1)
+    assert r.iterations == 1 and abs(r.eigenvalues[0] - 2.0) <= 1e-15
+    eng.close()
+    ising = L.heisenberg_chain(10, 1.0, 0.0, two_sz=0)
+    ising.exchange = []
+    eng = A.Engine.from_spec(ising)
+    orc = oracle_lib.Oracle(ising, L.processed_terms(ising)).prepare()
+    v = oracle_lib.fill_vector(SEED, eng.dim)
+    hv, pr = eng.hv(v)
+    ohv, opr = orc.hv(v)
+    assert rel_inf(hv, ohv) <= HV_TOL and prdct_ok(pr, opr, v, ohv)
+    assert np.allclose(hv, orc.diag[1:] * v, rtol=1e-15, atol=1e-15)
+    eng.close()
