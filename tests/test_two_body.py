@@ -178,3 +178,34 @@ def test_hubbard_gc_tpq_matches_restated_step(A, oracle_lib):
         got = out[:, c, :7]
         assert np.allclose(got, ref, rtol=1e-10, atol=1e-12), (got[:3], ref[:3])
     eng.close()
+
+
+@pytest.mark.gpu
+def test_hubbard_gc_canonical_tpq_matches_restated_step(A, oracle_lib):
+    """Canonical TPQ on the HubbardGC ring: per-step rows (inv_temp, E, <H^2>,
+    norm, doublon and electron-number moments, Sz moments) equal the reference's
+    step (CalcByCanonicalTPQ, MultiplyForCanonicalTPQ) restated in numpy on the
+    dense Hamiltonian, Taylor order 6."""
+    from tests.tpq_restated import ctpq_restated
+    print("[gpu] ctpq hubbard_gc", flush=True)
+    spec = two_body_cases()["hubbard_gc"]
+    H = brute.dense(spec)
+    eng = A.Engine.from_spec(spec)
+    k, steps, lv = 2, 12, 2.5
+    out, res = eng.ctpq(k, steps, seed=3, init_kind=0, expand_coef=6, large_value=lv)
+    assert res.large_value == lv
+    E = 0x5555555555555555
+    xs = [int(x) for x in range(eng.dim)]
+    D = np.array([bin(x & (x >> 1) & E).count("1") for x in xs], float)
+    up = np.array([bin(x & E).count("1") for x in xs], float)
+    dn = np.array([bin(x & (E << 1)).count("1") for x in xs], float)
+    N, S = up + dn, up - dn
+
+    def obs(v):
+        w = np.abs(v) ** 2
+        return (w @ D, w @ D ** 2, w @ N, w @ N ** 2, 0.5 * (w @ S), 0.25 * (w @ S ** 2))
+    for c in range(k):
+        ref = ctpq_restated(lambda v: H @ v, oracle_lib.fill_vector(3 + c, eng.dim), steps, 1.0 / lv, 6, obs)
+        got = out[:, c, :10]
+        assert np.allclose(got, ref, rtol=1e-10, atol=1e-12), (got[:3], ref[:3])
+    eng.close()
