@@ -459,3 +459,33 @@ def test_one_state_and_diagonal_only_sectors(A, oracle_lib):
     assert rel_inf(hv, ohv) <= HV_TOL and prdct_ok(pr, opr, v, ohv)
     assert np.allclose(hv, orc.diag[1:] * v, rtol=1e-15, atol=1e-15)
     eng.close()
+
+
+@pytest.mark.parametrize("name,k", [("chain16", 1), ("hubbard_4x2", 4), ("tri_4x3_gc", 8)])
+def test_autotuner_applies_a_correct_configuration(A, oracle_lib, name, k):
+    """hphi_autotune times every candidate shape (kernel, workgroup size,
+    workgroups per CU, columns per lane, XCD map) and leaves the fastest
+    applied: the report lists the timed candidates, the winner is no slower
+    than the default shape, and the tuned engine's H·v still equals the oracle's."""
+    print(f"[gpu] autotune {name} k={k}", flush=True)
+    spec = golden_cases()[name]
+    eng = A.Engine.from_spec(spec)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    dim = eng.dim
+    V1 = A.DeviceArray(dim * k)
+    eng.fill_random(V1, k, 40)
+    V0 = A.DeviceArray(dim * k).zero()
+    t_default = eng.time_mltply(k, V0, V1, 1, 3)
+    report = eng.autotune(k, V0, V1, reps=2)
+    times = [float(ln.rsplit(":", 1)[1].split()[0]) for ln in report.strip().split("\n")]
+    assert len(times) >= 8, report
+    assert min(times) <= 1.05 * t_default + 0.01, (min(times), t_default)
+    V0.zero()
+    pr = eng.mltply_block_dev(k, V0, V1) if k > 1 else [eng.mltply_dev(V0, V1)]
+    out = V0.download().reshape(dim, k)
+    for c in range(k):
+        vc = oracle_lib.fill_vector(40 + c, dim)
+        ohv, opr = orc.hv(vc)
+        assert rel_inf(out[:, c], ohv) <= HV_TOL
+        assert prdct_ok(pr[c], opr, vc, ohv)
+    eng.close()
