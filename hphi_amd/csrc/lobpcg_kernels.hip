@@ -23,6 +23,7 @@
 
 #define LB_TPB 256
 #define LB_GRID 1024
+#define LB_GRAM_GRID 4096  // workgroups of the Gram pass: its load and compute phases overlap across more groups (cfg4 block 8: 2 % per iteration over 1024)
 
 struct LbCols {
   double a[16], b[16];
@@ -279,7 +280,7 @@ extern "C" int hphi_lb_ws_create(hphi_engine *e, int k, hphi_lb_ws **out) {
   hphi_lb_ws *w = new hphi_lb_ws();
   w->k = k;
   w->stream = e->stream;
-  const size_t gram = (size_t)LB_GRID * 18 * k * k * 2;  // doubles
+  const size_t gram = (size_t)(LB_GRAM_GRID > LB_GRID ? LB_GRAM_GRID : LB_GRID) * 18 * k * k * 2;  // doubles
   if (hipMalloc(&w->part, sizeof(double) * gram) != hipSuccess ||
       hipMalloc(&w->res, sizeof(double) * 18 * k * k * 2) != hipSuccess ||
       hipMalloc(&w->coef, sizeof(double2) * 3 * k * k) != hipSuccess ||
@@ -355,7 +356,7 @@ extern "C" int hphi_lb_precond(hphi_lb_ws *ws, uint64_t dim, double *w, const do
 extern "C" int hphi_lb_gram(hphi_lb_ws *ws, uint64_t dim, const double *w, const double *x, const double *p,
                             const double *W, const double *X, const double *P, double *gram) {
   const int k = ws->k;
-  const int g = (int)std::min<uint64_t>(LB_GRID, std::max<uint64_t>(1, (dim + 15) / 16));
+  const int g = (int)std::min<uint64_t>(LB_GRAM_GRID, std::max<uint64_t>(1, (dim + 15) / 16));
 #define CALL(K)                                                                                              \
   hipLaunchKernelGGL(k_lb_gram<K>, dim3(g), dim3(LB_TPB), 0, ws->stream, (const double2 *)w,                 \
                      (const double2 *)x, (const double2 *)p, (const double2 *)W, (const double2 *)X,         \
