@@ -469,7 +469,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(jobs, TPQ samples and basis shards would each map to one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.summarize_stats:

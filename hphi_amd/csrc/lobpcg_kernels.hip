@@ -3,7 +3,7 @@
 // The k vectors of each LOBPCG block (w, x, p and their images W, X, P) are
 // stored state-major, element j*k + c, so the block H·v applies to them
 // directly.  Normalisations of the reference (src/CalcByLOBPCG.c:430-455,
-// 508-556) are carried as per-column scale factors by the driver instead of
+// 508-556) are carried as per-column scale factors by the host loop (lobpcg.c) instead of
 // separate passes over the vectors: a stored column times its scale is the
 // reference's vector.  Every reduction writes per-workgroup partials that a
 // fixed-order sum finishes, so results do not depend on scheduling.
