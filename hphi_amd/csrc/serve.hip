@@ -11,17 +11,21 @@
 // to SL_K jobs pay for one index decode and read one 64-byte row per term.
 //
 // A tick
-//   1. admits waiting jobs in arrival order: into a free column of a slab of
-//      the same structure, or into a new slab while its HBM footprint (tables,
-//      SL_K columns of diagonal and two Lanczos vectors, alpha/beta histories)
-//      fits the arena; a new column gets its couplings, its diagonal and its
-//      start vector on the device;
+//   1. admits waiting jobs: first, free columns of running slabs go to arrived
+//      jobs of the same structure among the first ADMIT_WINDOW waiting (slab
+//      steps cost about the same however many columns run); then jobs are
+//      admitted in arrival order, into a free column of a slab of the same
+//      structure, or into a new slab while its HBM footprint (tables, SL_K
+//      columns of diagonal and two Lanczos vectors, alpha/beta histories) fits
+//      the arena; a new column gets its couplings, its diagonal and its start
+//      vector on the device;
 //   2. advances every running job by S steps (S = the convergence-check
 //      interval, at most SL_SMAX; fewer when a job reaches its dimension or
 //      the step limit) of the reference's plain Lanczos recurrence
 //      (src/Lanczos_EigenValue.c:153-177): per step four packed launches over
 //      all slabs (residual + norm, beta, H·v, alpha), then one launch of the
-//      batched eigenvalue check, with no host round trip between the steps;
+//      batched eigenvalue check, with no host round trip between the steps
+//      (a workgroup takes one tile of SV_TPB states, XCD-contiguous);
 //   3. copies the (alpha, beta) of every step and the check results of all
 //      jobs to the host in one transfer each, tests convergence of the lowest
 //      n distinct Ritz values (Cullum-Willoughby filtered) and retires
