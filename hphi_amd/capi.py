@@ -58,7 +58,9 @@ class ModelDesc(C.Structure):
 class ModPara(C.Structure):
     _fields_ = [("calc_type", C.c_int), ("lanczos_max", C.c_int), ("initial_iv", C.c_int),
                 ("exct", C.c_int), ("lanczos_eps", C.c_int), ("lanczos_target", C.c_int),
-                ("num_ave", C.c_int), ("expec_interval", C.c_int), ("large_value", C.c_double)]
+                ("num_ave", C.c_int), ("expec_interval", C.c_int), ("large_value", C.c_double),
+                ("restart", C.c_int), ("initial_vec_type", C.c_int), ("calc_eigenvec", C.c_int),
+                ("output_eigenvec", C.c_int), ("input_eigenvec", C.c_int), ("expand_coef", C.c_int)]
 
 
 class LanczosParams(C.Structure):

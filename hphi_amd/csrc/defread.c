@@ -226,8 +226,13 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
   for (i = 0; i < 3; i++)
     if (!files[i][0]) return hphi_set_error("%s: %s file is required", namelist_path, kw_names[i]);
 
-  /* calcmod.def */
-  int calc_type = 0, calc_model = 0;
+  /* calcmod.def (ReadcalcmodFile, readdef.c:233-380): every keyword of the
+   * reference is accepted and an unknown one is an error, as there; the
+   * driver flags (ReStart, InitialVecType, CalcEigenVec, OutputEigenVec,
+   * InputEigenVec) go to the caller in hphi_modpara; the inputs that select
+   * paths this library does not have (InputHam / OutputHam, CalcSpec,
+   * OutputExcitedVec) are rejected by name. */
+  int calc_type = 0, calc_model = 0, restart = 0, ini_type = 0, calc_vec = 0, out_vec = 0, in_vec = 0;
   f = fopen(files[KW_CALCMOD], "r");
   if (!f) return hphi_set_error("cannot open %s", files[KW_CALCMOD]);
   while (fgets(line, sizeof line, f)) {
@@ -237,11 +242,46 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
     if (sscanf(line, "%255s %d", kw, &v) != 2) continue;
     if (ci_eq(kw, "CalcType")) calc_type = v;
     else if (ci_eq(kw, "CalcModel")) calc_model = v;
+    else if (ci_eq(kw, "ReStart")) restart = v;
+    else if (ci_eq(kw, "InitialVecType")) ini_type = v;
+    else if (ci_eq(kw, "CalcEigenVec")) calc_vec = v;
+    else if (ci_eq(kw, "OutputEigenVec") || ci_eq(kw, "OEV")) out_vec = v;
+    else if (ci_eq(kw, "InputEigenVec") || ci_eq(kw, "IEV")) in_vec = v;
+    else if (ci_eq(kw, "OutputMode") || ci_eq(kw, "FlgFiniteTemperature") || ci_eq(kw, "NGPU") ||
+             ci_eq(kw, "ScaLAPACK")) {
+      /* no effect on this library's results */
+    } else if (ci_eq(kw, "InputHam") || ci_eq(kw, "OutputHam") || ci_eq(kw, "CalcSpec") ||
+               ci_eq(kw, "CalcSpectrum") || ci_eq(kw, "OutputExcitedVec") || ci_eq(kw, "OutputExVec")) {
+      if (v != 0) {
+        fclose(f);
+        return hphi_set_error("%s: %s %d is not supported by this library", files[KW_CALCMOD], kw, v);
+      }
+    } else {
+      fclose(f);
+      return hphi_set_error("%s: unknown keyword %s", files[KW_CALCMOD], kw); /* readdef.c:317-320 */
+    }
   }
   fclose(f);
+  if (restart < 0 || restart > 3) return hphi_set_error("%s: ReStart must be 0..3", files[KW_CALCMOD]);
+  if (ini_type < -1 || ini_type > 1) return hphi_set_error("%s: InitialVecType must be -1..1", files[KW_CALCMOD]);
+  if (calc_vec < -1 || calc_vec > 1) return hphi_set_error("%s: CalcEigenVec must be -1..1", files[KW_CALCMOD]);
 
   /* modpara.def */
-  hphi_modpara mp = {calc_type, 2000, -1, 1, 14, 2, 1, 1, 0.0};
+  hphi_modpara mp;
+  memset(&mp, 0, sizeof mp);
+  mp.calc_type = calc_type;
+  mp.lanczos_max = 2000;
+  mp.initial_iv = -1;
+  mp.exct = 1;
+  mp.lanczos_eps = 14;
+  mp.lanczos_target = 2;
+  mp.num_ave = 1;
+  mp.expec_interval = 1;
+  mp.restart = restart;
+  mp.initial_vec_type = ini_type;
+  mp.calc_eigenvec = calc_vec;
+  mp.output_eigenvec = out_vec;
+  mp.input_eigenvec = in_vec;
   int nsite = 0, two_sz = 0, have_sz = 0, ncond = -1, nup = -1, ndown = -1;
   f = fopen(files[KW_MODPARA], "r");
   if (!f) return hphi_set_error("cannot open %s", files[KW_MODPARA]);
@@ -268,6 +308,7 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
     else if (ci_eq(kw, "LargeValue")) mp.large_value = v;
     else if (ci_eq(kw, "NumAve")) mp.num_ave = (int)v;
     else if (ci_eq(kw, "ExpecInterval")) mp.expec_interval = (int)v;
+    else if (ci_eq(kw, "ExpandCoef")) mp.expand_coef = (int)v;
   }
   fclose(f);
   if (nsite <= 0) return hphi_set_error("%s: Nsite must be positive", files[KW_MODPARA]);
@@ -331,7 +372,13 @@ int hphi_read_def(const char *namelist_path, hphi_model_desc **dout, hphi_modpar
     /* merge duplicates, keeping first-occurrence order (readdef.c:1130-1150) */
     tlist m = {0};
     m.w = 4;
-    for (int a = 0; a < tr.n; a++) {
+    for (int a = 0; a < tr.n && calc_model == HPHI_MODEL_SPIN; a++)
+      if (tr.idx[4 * a] != tr.idx[4 * a + 2]) { /* readdef.c:1106-1110 */
+        rc = hphi_set_error("%s: a Spin transfer must act on one site (%d %d)", files[KW_TRANS], tr.idx[4 * a],
+                            tr.idx[4 * a + 2]);
+        break;
+      }
+    for (int a = 0; a < tr.n && !rc; a++) {
       int hit = -1;
       for (int b = 0; b < m.n && hit < 0; b++)
         if (!memcmp(m.idx + 4 * b, tr.idx + 4 * a, 4 * sizeof(int))) hit = b;

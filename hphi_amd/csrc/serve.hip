@@ -1178,6 +1178,10 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
         const double *o = s->h_ev + 17 * (size_t)i;
         nc = (int)o[16];
         for (int q = 0; q < nc; q++) cur[q] = o[q];
+        // the device window stops at CK_LMAX values of T_m; with more ghost copies
+        // than that below the wanted values the host filter widens it up to m
+        if (nc < (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim) && m > CK_LMAX)
+          nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m, s->p.n_eigen, cur);
       }
       const int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
       const bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) || (exhausted && nc >= 1);

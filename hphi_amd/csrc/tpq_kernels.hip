@@ -230,6 +230,24 @@ extern "C" int hphi_diag_maxabs(hphi_engine *e, double *out) {
   return 0;
 }
 
+// LargeValue when the caller gives none: the reference has no default rule
+// (modpara LargeValue, global.c); here l = max(3 max|diag|, B) / Nsite with
+// B = max|diag| + sum of |coupling| over the compiled off-diagonal terms, a
+// Gershgorin bound of |E| (each gather term contributes at most one element
+// per row), so l - H/Nsite stays positive definite also for Hamiltonians
+// without a diagonal part (pure hopping, XY, transverse field).
+static int default_large_value(hphi_engine *e, double *lv) {
+  double m = 0.0;
+  if (hphi_diag_maxabs(e, &m)) return -1;
+  double off = 0.0;
+  for (const GTerm &t : e->h_terms_dev) off += std::hypot(t.cre, t.cim);
+  const double b = std::max(3.0 * m, m + off);
+  if (!(b > 0.0))
+    return hphi_set_error("LargeValue not given and the Hamiltonian is zero: no bound of its spectrum");
+  *lv = b / (double)e->nsite;
+  return 0;
+}
+
 struct TpqPlan {
   double *V0, *V1, *state, *part, *out;
   int k, layout, g, grid_hv;
@@ -279,20 +297,13 @@ extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_ho
   P.steps = p->steps;
   if (p->large_value > 0.0) {
     P.lv = p->large_value;
-  } else {
-    double m = 0.0;
-    if (hphi_diag_maxabs(e, &m)) return -1;
-    P.lv = 3.0 * m / P.Ns;  // l = 3 max_j |list_Diagonal[j]| / Ns
+  } else if (default_large_value(e, &P.lv)) {
+    return -1;
   }
   P.obs_kind = e->model == HPHI_MODEL_SPINGC ? 1 : hphi_model_hubbard(e->model) ? 2 : 0;
   P.sz_const = 0.5 * (double)(e->nup - e->ndown);  // canonical Spin (expec_energy_flct.c:104-118)
   P.g = tq_grid(dim, k);
   size_t vbytes = sizeof(double2) * dim * k;
-  HIPCHK(hipMalloc(&P.V0, vbytes));
-  HIPCHK(hipMalloc(&P.V1, vbytes));
-  HIPCHK(hipMalloc(&P.state, sizeof(double) * 64));
-  HIPCHK(hipMalloc(&P.part, sizeof(double) * (size_t)TQ_GRID * k * TQ_NRED));
-  HIPCHK(hipMalloc(&P.out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT));
   int rc = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -300,6 +311,15 @@ extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_ho
   (void)hipEventCreate(&ev0);
   (void)hipEventCreate(&ev1);
   do {
+    // every buffer is freed below, also when a later allocation fails
+    if (hipMalloc(&P.V0, vbytes) != hipSuccess || hipMalloc(&P.V1, vbytes) != hipSuccess ||
+        hipMalloc(&P.state, sizeof(double) * 64) != hipSuccess ||
+        hipMalloc(&P.part, sizeof(double) * (size_t)TQ_GRID * k * TQ_NRED) != hipSuccess ||
+        hipMalloc(&P.out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = hphi_set_error("hphi_tpq: device allocation failed");
+      break;
+    }
     uint64_t n = dim * (uint64_t)k;
     hipLaunchKernelGGL(k_tpq_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, (double2 *)P.V1, dim, k,
                        P.sj, P.sc, p->seed, p->init_kind);
@@ -490,11 +510,7 @@ extern "C" int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out_
   if (hphi_build_tables(e) || hphi_build_diag(e)) return -1;
   const uint64_t dim = e->dim, n = dim * (uint64_t)k;
   double lv = p->large_value;
-  if (lv <= 0.0) {
-    double m = 0.0;
-    if (hphi_diag_maxabs(e, &m)) return -1;
-    lv = 3.0 * m / (double)e->nsite;
-  }
+  if (lv <= 0.0 && default_large_value(e, &lv)) return -1;
   const double dtau = 1.0 / lv;  // CalcByCanonicalTPQ.c:143
   const int obs_kind = e->model == HPHI_MODEL_SPINGC ? 1 : hphi_model_hubbard(e->model) ? 2 : 0;
   const double sz_const = 0.5 * (double)(e->nup - e->ndown);
@@ -514,6 +530,7 @@ extern "C" int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out_
         hipMalloc(&state, sizeof(double) * 64) != hipSuccess ||
         hipMalloc(&part, sizeof(double) * (size_t)TQ_GRID * k * TQ_NRED) != hipSuccess ||
         hipMalloc(&out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT) != hipSuccess) {
+      (void)hipGetLastError();
       rc = hphi_set_error("hphi_ctpq: device allocation failed");
       break;
     }

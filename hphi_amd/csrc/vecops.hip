@@ -192,6 +192,9 @@ extern "C" int hphi_vec_lanczos_update_acc(hphi_engine *e, double *v0, double *v
   hipLaunchKernelGGL(k_lanczos_update_acc, dim3(vo_grid(e->dim)), dim3(VO_TPB), 0, e->stream, (double2 *)v0,
                      (double2 *)v1, (double2 *)vg, e->dim, alpha, beta, c);
   HIPCHK(hipGetLastError());
+  // the copy / zero / fill helpers run on the null stream, which is not ordered
+  // against the engine's non-blocking stream: finish before returning
+  HIPCHK(hipStreamSynchronize(e->stream));
   return 0;
 }
 extern "C" int hphi_vec_lanczos_update(hphi_engine *e, double *v0, double *v1, double alpha, double beta) {
@@ -199,6 +202,7 @@ extern "C" int hphi_vec_lanczos_update(hphi_engine *e, double *v0, double *v1, d
   hipLaunchKernelGGL(k_lanczos_update, dim3(vo_grid(e->dim)), dim3(VO_TPB), 0, e->stream, (double2 *)v0,
                      (double2 *)v1, e->dim, alpha, beta);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
   return 0;
 }
 

@@ -76,11 +76,16 @@ typedef struct {
   int n_pairhop;       const int *pairhop_ij;        const double *pairhop_J;
 } hphi_model_desc;
 
-/* Modelling parameters read from modpara.def (src/readdef.c:555-650). */
+/* Modelling parameters read from modpara.def (src/readdef.c:555-650) and the
+ * driver flags of calcmod.def (src/readdef.c:233-320): restart = ReStart
+ * (0 none, 1 out, 2 in+out, 3 in), initial_vec_type = InitialVecType (0
+ * uniform complex, 1 real, -1 Gaussian), calc_eigenvec, output_eigenvec,
+ * input_eigenvec as there; expand_coef = ExpandCoef (0: not given). */
 typedef struct {
   int calc_type, lanczos_max, initial_iv, exct, lanczos_eps, lanczos_target, num_ave,
       expec_interval;
   double large_value;
+  int restart, initial_vec_type, calc_eigenvec, output_eigenvec, input_eigenvec, expand_coef;
 } hphi_modpara;
 
 const char *hphi_last_error(void);
@@ -271,8 +276,11 @@ int hphi_lanczos_good_eigenvalues(const double *alpha, const double *beta, int m
 int hphi_lanczos_converged(const double *prev, int nprev, const double *cur, int ncur, int want, double tol);
 /* Microcanonical TPQ (CalcByTPQ, src/CalcByTPQ.c:52-294) for k samples at once.
  * Sample c starts from a random vector of seed + c (init_kind 0: uniform
- * complex, -1: complex Gaussian); l = large_value, or 3 max|list_Diagonal|/Nsite
- * when large_value <= 0.  layout 1 keeps the k vectors state-major and uses
+ * complex, -1: complex Gaussian); l = large_value, or, when large_value <= 0,
+ * max(3 max|list_Diagonal|, max|list_Diagonal| + sum of the off-diagonal
+ * |couplings|) / Nsite: the larger of 3 max|diag| and a Gershgorin bound of
+ * |E|, so that l - H/Nsite is positive definite (the reference has no default
+ * and reads LargeValue from modpara.def).  layout 1 keeps the k vectors state-major and uses
  * the block H·v; layout 0 keeps them vector-major with one H·v per sample.
  * use_graph captures one step in a hipGraph and replays it.
  * out[(s*k + c)*12 + q], s = 0..steps-1 (the reference's step_i), q:

@@ -35,11 +35,15 @@ class OrModel(C.Structure):
 
 
 def build(quiet: bool = True) -> None:
-    """Compile liboracle.so, and the reference harness when the reference tree is present."""
+    """Compile liboracle.so, and the reference harness (plain and with the drop-in
+    binding) when the reference tree is present."""
     out = subprocess.DEVNULL if quiet else None
     subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True, stdout=out)
     if os.path.isdir("/root/reference/src"):
         subprocess.run(["make", "-C", HERE, "-j8", "ref"], check=True, stdout=out)
+        # the reference's callers with the drop-in binding (needs libhphi_amd.so)
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "hphi_amd", "libhphi_amd.so")):
+            subprocess.run(["make", "-C", HERE, "dropin"], check=True, stdout=out)
 
 
 _lib = None

@@ -27,12 +27,31 @@
  *                 Norm_rand*.dat, Flct_rand*.dat
  *   mode ctpq   : the same for CalcByCanonicalTPQ (src/CalcByCanonicalTPQ.c:49-300,
  *                 HPhiMain.c:323-329; CalcType 5)
+ *   mode lanczos: the recurrence of src/Lanczos_EigenValue.c:102-177 (v1 = the
+ *                 synthetic vector(seed) normalised, v0 = 0; mltply; alpha = Re prdct;
+ *                 beta = |v0 - alpha v1|; then v1 <- (v0 - alpha v1)/beta,
+ *                 v0 <- -beta v1_old, mltply, ...) for [reps] steps, driven by the
+ *                 reference mltply; ab.txt gets "step alpha beta" per step, flushed
+ *                 as it goes (the tridiagonal eigenvalues are taken from these
+ *                 numbers by the caller: Lanczos_EigenValue's DSEVvalue needs
+ *                 LAPACK, which is absent)
  *                 Both TPQ modes first write each sample's start vector as the
  *                 reference makes it in this process (MakeIniVec(rand_i), which
  *                 depends on the thread count): ini_rand<i>.bin = global_1st_norm
  *                 (double) then v1[1..dim] (complex128, normalised)
  * The input directory is entered with chdir, as the reference opens its def
  * files by relative name and writes ./output/.
+ *
+ * Built with -DHPHI_DROPIN (oracle/Makefile target `dropin`, _ref/ref_dropin)
+ * the same program is the reference with this project's drop-in binding
+ * (integration/mltply_hphi_amd.c) linked in place of src/mltply.c:
+ * hphi_amd_setup() replaces sz() and diagonalcalc(), and every mltply call of
+ * the reference's own callers (CalcByTPQ, CalcByCanonicalTPQ, expec_*, the
+ * modes above) runs on the GPU through libhphi_amd.  One more mode there:
+ *   mode desc   : host only, right after the reference's reader: the
+ *                 binding's descriptor of X->Def against hphi_read_def's of the
+ *                 same namelist, field by field; prints "desc ok" or the first
+ *                 difference (exit 1)
  */
 #include <errno.h>
 #include <math.h>
@@ -59,6 +78,79 @@
 #include "sz.h"
 #include "wrapperMPI.h"
 #include "xsetmem.h"
+
+#ifdef HPHI_DROPIN
+#include "hphi_amd.h"
+struct hphi_amd_desc_buf {
+  int *ex, *pl, *tr, *ci, *cv, *hd, *ia, *iad, *ph;
+  double *trt, *iav;
+};
+int hphi_amd_fill_desc(const struct DefineList *D, hphi_model_desc *d, struct hphi_amd_desc_buf *b);
+void hphi_amd_desc_buf_free(struct hphi_amd_desc_buf *b);
+int hphi_amd_setup(struct BindStruct *X);
+
+static int cmp_i(const char *what, int n, int w, const int *a, const int *b) {
+  for (int i = 0; i < n * w; i++)
+    if (a[i] != b[i]) {
+      printf("desc differs: %s[%d][%d] %d vs %d\n", what, i / w, i % w, a[i], b[i]);
+      return 1;
+    }
+  return 0;
+}
+static int cmp_d(const char *what, int n, int w, const double *a, const double *b) {
+  for (int i = 0; i < n * w; i++)
+    if (a[i] != b[i]) {
+      printf("desc differs: %s[%d][%d] %.17g vs %.17g\n", what, i / w, i % w, a[i], b[i]);
+      return 1;
+    }
+  return 0;
+}
+/* the binding's descriptor of the reference's X->Def against hphi_read_def's */
+static int desc_check(struct DefineList *D) {
+  hphi_model_desc a, *bp = NULL;
+  struct hphi_amd_desc_buf buf;
+  if (hphi_amd_fill_desc(D, &a, &buf)) return 4;
+  if (hphi_read_def("namelist.def", &bp, NULL)) {
+    printf("hphi_read_def: %s\n", hphi_last_error());
+    return 1;
+  }
+  const hphi_model_desc *b = bp;
+  int bad = 0;
+#define N(f)                                                                       \
+  if (!bad && a.f != b->f) {                                                       \
+    printf("desc differs: %s %d vs %d\n", #f, a.f, b->f);                          \
+    bad = 1;                                                                       \
+  }
+  N(model) N(nsite) N(nup) N(ndown) N(n_exchange) N(n_pairlift) N(n_transfer) N(n_chemi) N(n_intra) N(n_inter)
+  N(n_hund) N(n_interall) N(n_interall_diag) N(n_pairhop)
+#undef N
+  bad = bad || cmp_i("exchange_ij", a.n_exchange, 2, a.exchange_ij, b->exchange_ij) ||
+        cmp_d("exchange_J", a.n_exchange, 1, a.exchange_J, b->exchange_J) ||
+        cmp_i("pairlift_ij", a.n_pairlift, 2, a.pairlift_ij, b->pairlift_ij) ||
+        cmp_d("pairlift_J", a.n_pairlift, 1, a.pairlift_J, b->pairlift_J) ||
+        cmp_i("transfer_idx", a.n_transfer, 4, a.transfer_idx, b->transfer_idx) ||
+        cmp_d("transfer_t", a.n_transfer, 2, a.transfer_t, b->transfer_t) ||
+        cmp_i("chemi_site", a.n_chemi, 1, a.chemi_site, b->chemi_site) ||
+        cmp_i("chemi_spin", a.n_chemi, 1, a.chemi_spin, b->chemi_spin) ||
+        cmp_d("chemi_mu", a.n_chemi, 1, a.chemi_mu, b->chemi_mu) ||
+        cmp_i("intra_site", a.n_intra, 1, a.intra_site, b->intra_site) ||
+        cmp_d("intra_U", a.n_intra, 1, a.intra_U, b->intra_U) ||
+        cmp_i("inter_ij", a.n_inter, 2, a.inter_ij, b->inter_ij) ||
+        cmp_d("inter_V", a.n_inter, 1, a.inter_V, b->inter_V) ||
+        cmp_i("hund_ij", a.n_hund, 2, a.hund_ij, b->hund_ij) ||
+        cmp_d("hund_J", a.n_hund, 1, a.hund_J, b->hund_J) ||
+        cmp_i("interall_idx", a.n_interall, 8, a.interall_idx, b->interall_idx) ||
+        cmp_d("interall_V", a.n_interall, 2, a.interall_V, b->interall_V) ||
+        cmp_i("interall_diag_idx", a.n_interall_diag, 4, a.interall_diag_idx, b->interall_diag_idx) ||
+        cmp_d("interall_diag_V", a.n_interall_diag, 1, a.interall_diag_V, b->interall_diag_V) ||
+        cmp_i("pairhop_ij", a.n_pairhop, 2, a.pairhop_ij, b->pairhop_ij) ||
+        cmp_d("pairhop_J", a.n_pairhop, 1, a.pairhop_J, b->pairhop_J);
+  hphi_model_desc_free(bp);
+  hphi_amd_desc_buf_free(&buf);
+  if (!bad) printf("desc ok\n");
+  return bad;
+}
+#endif
 
 static uint64_t sm64(uint64_t x) {
   uint64_t z = x + 0x9E3779B97F4A7C15ULL;
@@ -94,7 +186,6 @@ int main(int argc, char *argv[]) {
     if (!getcwd(cwd, sizeof cwd)) return 2;
     snprintf(outdir, sizeof outdir, "%s/%s", cwd, argv[3]);
   }
-  const char *mode = argv[2];
   uint64_t seed = argc > 4 ? strtoull(argv[4], NULL, 10) : 0;
   int reps = argc > 5 ? atoi(argv[5]) : 3;
   mkdir(outdir, 0777);
@@ -114,14 +205,24 @@ int main(int argc, char *argv[]) {
   if (ReadDefFileIdxPara(&(X.Bind.Def), &(X.Bind.Boost)) != 0) return 3;
   SetConvergenceFactor(&(X.Bind.Def));
   if (HPhiTrans(&(X.Bind)) != 0) return 3;
+  const char *mode = argv[2];
+#ifdef HPHI_DROPIN
+  if (strcmp(mode, "desc") == 0) return desc_check(&X.Bind.Def);
+#endif
   if (check(&(X.Bind)) == MPIFALSE) return 3;
   if (setmem_large(&X.Bind) != 0) return 3;
   double t0 = now();
+#ifdef HPHI_DROPIN
+  /* the binding's set-up in place of sz() and diagonalcalc() (HPhiMain.c:278,288) */
+  if (hphi_amd_setup(&(X.Bind)) != 0) return 3;
+  double t_sz = now() - t0, t_diag = 0.0;
+#else
   if (sz(&(X.Bind), list_1, list_2_1, list_2_2) != 0) return 3;
   double t_sz = now() - t0;
   t0 = now();
   diagonalcalc(&(X.Bind));
   double t_diag = now() - t0;
+#endif
 
   unsigned long dim = X.Bind.Check.idim_max;
   int model = X.Bind.Def.iCalcModel;
@@ -190,6 +291,43 @@ int main(int argc, char *argv[]) {
     fclose(f);
     if (expec_cisajs(&(X.Bind), v1) != 0) return 6;
     if (expec_cisajscktaltdc(&(X.Bind), v1) != 0) return 6;
+    return 0;
+  }
+
+  if (strcmp(mode, "lanczos") == 0) {
+    double n2 = 0.0;
+#pragma omp parallel for reduction(+ : n2)
+    for (unsigned long j = 1; j <= dim; j++) n2 += creal(conj(v1[j]) * v1[j]);
+    const double inv = 1.0 / sqrt(n2);
+#pragma omp parallel for
+    for (unsigned long j = 1; j <= dim; j++) v1[j] *= inv;
+    char path[4200];
+    snprintf(path, sizeof path, "%s/ab.txt", outdir);
+    FILE *f = fopen(path, "w");
+    if (!f) return 4;
+    double alpha1 = 0.0, beta1 = 0.0;
+    for (int stp = 1; stp <= reps; stp++) {
+      if (stp > 1) {
+#pragma omp parallel for
+        for (unsigned long i = 1; i <= dim; i++) {
+          double complex temp1 = v1[i];
+          double complex temp2 = (v0[i] - alpha1 * v1[i]) / beta1;
+          v0[i] = -beta1 * temp1;
+          v1[i] = temp2;
+        }
+      }
+      t0 = now();
+      if (mltply(&(X.Bind), v0, v1) != 0) return 5;
+      const double dt = now() - t0;
+      alpha1 = creal(X.Bind.Large.prdct);
+      double complex cbeta1 = 0.0;
+#pragma omp parallel for reduction(+ : cbeta1)
+      for (unsigned long i = 1; i <= dim; i++) cbeta1 += conj(v0[i] - alpha1 * v1[i]) * (v0[i] - alpha1 * v1[i]);
+      beta1 = sqrt(creal(cbeta1));
+      fprintf(f, "%d %.17g %.17g %.3f\n", stp, alpha1, beta1, dt);
+      fflush(f);
+    }
+    fclose(f);
     return 0;
   }
 

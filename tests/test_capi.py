@@ -125,6 +125,42 @@ def test_def_reader_errors(capi, tmp_path, mutate, msg):
     assert msg in capi.lib().hphi_last_error().decode()
 
 
+def test_def_reader_rejects_offsite_spin_transfer(capi, tmp_path):
+    """A canonical Spin Trans entry must act on one site (readdef.c:1106-1110)."""
+    spec = L.heisenberg_chain(6)
+    spec.trans = [(0, 0, 1, 0, 0.1, 0.0), (1, 0, 0, 0, 0.1, 0.0)]
+    nl = L.write_expert(spec, str(tmp_path))
+    rc, _, _ = _read(capi, nl)
+    assert rc == -1 and "one site" in capi.lib().hphi_last_error().decode()
+
+
+def _calcmod(tmp_path, text):
+    nl = L.write_expert(L.heisenberg_chain(6), str(tmp_path))
+    p = os.path.join(str(tmp_path), "calcmod.def")
+    open(p, "w").write(text)
+    return nl
+
+
+def test_def_reader_passes_driver_flags(capi, tmp_path):
+    nl = _calcmod(tmp_path, "CalcType 1\nCalcModel 1\nReStart 2\nInitialVecType -1\nCalcEigenVec 1\n"
+                            "OutputEigenVec 1\nInputEigenVec 0\nOutputMode 1\nNGPU 2\n")
+    rc, pd, mp = _read(capi, nl)
+    assert rc == 0, capi.lib().hphi_last_error()
+    assert (mp.calc_type, mp.restart, mp.initial_vec_type, mp.calc_eigenvec, mp.output_eigenvec,
+            mp.input_eigenvec) == (1, 2, -1, 1, 1, 0)
+    capi.lib().hphi_model_desc_free(pd)
+
+
+@pytest.mark.parametrize("line,msg", [
+    ("Bogus 1", "unknown keyword Bogus"), ("InputHam 1", "InputHam 1 is not supported"),
+    ("CalcSpec 1", "CalcSpec 1 is not supported"), ("ReStart 4", "ReStart must be"),
+])
+def test_def_reader_calcmod_errors(capi, tmp_path, line, msg):
+    nl = _calcmod(tmp_path, f"CalcType 0\nCalcModel 1\n{line}\n")
+    rc, _, _ = _read(capi, nl)
+    assert rc == -1 and msg in capi.lib().hphi_last_error().decode()
+
+
 def test_def_reader_rejects_unsupported_keyword(capi, tmp_path):
     nl = L.write_expert(L.heisenberg_chain(6), str(tmp_path))
     with open(nl, "a") as f:

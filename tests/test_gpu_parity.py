@@ -242,6 +242,46 @@ def test_every_tuning_matches_oracle(A, oracle_lib, name, tuning):
     eng.close()
 
 
+def _extreme_eigenvalues(orc):
+    """lowest and highest eigenvalue of the oracle's H (ARPACK, both ends)"""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    op = LinearOperator((orc.dim, orc.dim), matvec=lambda v: orc.hv(np.asarray(v, np.complex128).ravel())[0],
+                        dtype=np.complex128)
+    lo = eigsh(op, k=1, which="SA", tol=1e-10, return_eigenvectors=False)[0]
+    hi = eigsh(op, k=1, which="LA", tol=1e-10, return_eigenvectors=False)[0]
+    return lo, hi
+
+
+def check_default_large_value(spec, orc, lv):
+    """The default l (no LargeValue given) is at least 3 max|diag| / Nsite, as the
+    cfg4 workload defines it, and l Nsite bounds |E| so that l - H/Nsite is
+    positive definite (the mTPQ recurrence converges to the lowest state)."""
+    m = np.max(np.abs(orc.diag[1:]))
+    assert lv >= 3.0 * m / spec.nsite * (1 - 1e-15)
+    lo, hi = _extreme_eigenvalues(orc)
+    assert lv * spec.nsite > max(abs(lo), abs(hi))
+
+
+@pytest.mark.parametrize("make", [
+    lambda: L.spin_model("xy_chain10", 10, [(i, j, 1.0) for i, j in L.chain_bonds(10)], two_sz=0, jz_scale=0.0),
+    lambda: L.hubbard_chain(6, U=0.0),
+    lambda: L.spin_model("tfield_gc8", 8, [(i, j, 1.0) for i, j in L.chain_bonds(8)], two_sz=None, jz_scale=0.0,
+                         hx=0.7),
+], ids=["xy_chain10", "hubbard_chain6_U0", "xy_field_gc8"])
+def test_tpq_default_large_value_bounds_zero_diagonal_models(A, oracle_lib, make):
+    """Hamiltonians without a diagonal part: 3 max|diag| = 0 does not bound the
+    spectrum, the Gershgorin rule does; mTPQ then lowers the energy."""
+    spec = make()
+    print(f"[gpu] tpq default l {spec.name}", flush=True)
+    eng = A.Engine.from_spec(spec)
+    orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
+    assert np.max(np.abs(orc.diag[1:])) == 0.0
+    out, res = eng.tpq(1, 40, seed=5, init_kind=0)
+    check_default_large_value(spec, orc, res.large_value)
+    assert out[-1, 0, 1] < out[0, 0, 1]
+    eng.close()
+
+
 @pytest.mark.parametrize("layout,graph", [(1, True), (0, True), (1, False)])
 @pytest.mark.parametrize("name,k", [("tri_4x3_gc", 4), ("hubbard_4x2", 2), ("j1j2_4x4", 2)])
 def test_tpq_matches_restated_reference_step(A, oracle_lib, name, k, layout, graph):
@@ -254,8 +294,8 @@ def test_tpq_matches_restated_reference_step(A, oracle_lib, name, k, layout, gra
     steps = 25
     out, res = eng.tpq(k, steps, seed=3, init_kind=0, layout=layout, use_graph=graph)
     orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
-    lv = 3.0 * np.max(np.abs(orc.diag[1:])) / spec.nsite
-    assert abs(res.large_value - lv) <= 1e-15 * lv
+    lv = res.large_value
+    check_default_large_value(spec, orc, lv)
     sz = None  # per-state observable: SpinGC 2 Sz(x), Hubbard doublon count
     if spec.model == L.SPIN_GC:
         sz = np.array([2 * bin(x).count("1") - spec.nsite for x in range(orc.dim)], float)
@@ -310,8 +350,8 @@ def test_ctpq_matches_restated_reference_step(A, oracle_lib, name, k, coef):
     steps = 15
     out, res = eng.ctpq(k, steps, seed=3, init_kind=0, expand_coef=coef)
     orc = oracle_lib.Oracle(spec, L.processed_terms(spec)).prepare()
-    lv = 3.0 * np.max(np.abs(orc.diag[1:])) / spec.nsite
-    assert abs(res.large_value - lv) <= 1e-15 * lv
+    lv = res.large_value
+    check_default_large_value(spec, orc, lv)
     D, N, S = _state_moments(spec, orc)
 
     def obs(v):
