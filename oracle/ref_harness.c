@@ -17,6 +17,8 @@
  *                 oracle), v0 = 0, one mltply; write v0.bin (dim complex128,
  *                 0-based), prdct.txt
  *   mode sums   : as hv but write only checksums (sums.txt) — for large dims
+ *   mode sums_seeds: one mltply per seed s = seed .. seed+reps-1 of the synthetic
+ *                 vector of s; sums_<s>.txt and prdct_<s>.txt
  *   mode bench  : reps timed mltply calls; print one JSON line
  *   mode green  : v0 = the synthetic vector(seed) normalised; expec_energy_flct
  *                 (src/expec_energy_flct.c:40-160: moments on v0, then v1 = v0,
@@ -174,6 +176,27 @@ static int dump(const char *dir, const char *name, const void *p, size_t bytes) 
 
 static double now(void) { return omp_get_wtime(); }
 
+/* checksums of v0[1..dim]: S1 = sum v0, S2 = sum |v0|^2, S3 = sum w_j v0[j] with
+ * w_j = (j mod 97) + 1, and 64 sampled entries at j = 1 + (k * 2654435761) mod dim */
+static int write_sums(const char *path, const double complex *v0, unsigned long dim) {
+  double s1r = 0, s1i = 0, s2 = 0, s3r = 0, s3i = 0;
+#pragma omp parallel for reduction(+ : s1r, s1i, s2, s3r, s3i)
+  for (unsigned long j = 1; j <= dim; j++) {
+    double a = creal(v0[j]), b = cimag(v0[j]);
+    double w = (double)((j % 97) + 1);
+    s1r += a; s1i += b; s2 += a * a + b * b; s3r += w * a; s3i += w * b;
+  }
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  fprintf(f, "S1 %.17g %.17g\nS2 %.17g\nS3 %.17g %.17g\n", s1r, s1i, s2, s3r, s3i);
+  for (unsigned long k = 0; k < 64; k++) {
+    unsigned long j = 1 + (unsigned long)((k * 2654435761ULL) % dim);
+    fprintf(f, "E %lu %.17g %.17g\n", j, creal(v0[j]), cimag(v0[j]));
+  }
+  fclose(f);
+  return 0;
+}
+
 int main(int argc, char *argv[]) {
   if (argc < 4) {
     fprintf(stderr, "usage: %s <def-dir> tables|hv|sums|bench <out-dir> [seed] [reps]\n", argv[0]);
@@ -263,6 +286,29 @@ int main(int argc, char *argv[]) {
     }
     if (canonical) return CalcByCanonicalTPQ(NumAve, X.Bind.Def.Param.ExpecInterval, &X) == TRUE ? 0 : 7;
     return CalcByTPQ(NumAve, X.Bind.Def.Param.ExpecInterval, &X) == TRUE ? 0 : 7;
+  }
+
+  if (strcmp(mode, "sums_seeds") == 0) {
+    /* one mltply per seed s = seed .. seed+reps-1 on the synthetic vector of s:
+     * sums_<s>.txt as mode sums, prdct_<s>.txt = Re Im of prdct */
+    for (int r = 0; r < reps; r++) {
+      const uint64_t s = seed + (uint64_t)r;
+#pragma omp parallel for
+      for (unsigned long j = 1; j <= dim; j++) {
+        v1[j] = (2.0 * unit_rng(s, 2 * (j - 1)) - 1.0) + I * (2.0 * unit_rng(s, 2 * (j - 1) + 1) - 1.0);
+        v0[j] = 0.0;
+      }
+      if (mltply(&(X.Bind), v0, v1) != 0) return 5;
+      char path[4200];
+      snprintf(path, sizeof path, "%s/sums_%lu.txt", outdir, (unsigned long)s);
+      if (write_sums(path, v0, dim)) return 4;
+      snprintf(path, sizeof path, "%s/prdct_%lu.txt", outdir, (unsigned long)s);
+      FILE *f = fopen(path, "w");
+      if (!f) return 4;
+      fprintf(f, "%.17g %.17g\n", creal(X.Bind.Large.prdct), cimag(X.Bind.Large.prdct));
+      fclose(f);
+    }
+    return 0;
   }
 
   /* synthetic input, element j (1-based) from counter 2(j-1), 2(j-1)+1 */
@@ -358,23 +404,8 @@ int main(int argc, char *argv[]) {
   if (strcmp(mode, "hv") == 0) {
     if (dump(outdir, "v0.bin", v0 + 1, dim * sizeof(double complex))) return 4;
   } else if (strcmp(mode, "sums") == 0) {
-    /* checksums: S1 = sum v0, S2 = sum |v0|^2, S3 = sum w_j v0[j] with w_j = (j mod 97) + 1
-     * and 64 sampled entries at j = 1 + (k * 2654435761) mod dim */
-    double s1r = 0, s1i = 0, s2 = 0, s3r = 0, s3i = 0;
-#pragma omp parallel for reduction(+ : s1r, s1i, s2, s3r, s3i)
-    for (unsigned long j = 1; j <= dim; j++) {
-      double a = creal(v0[j]), b = cimag(v0[j]);
-      double w = (double)((j % 97) + 1);
-      s1r += a; s1i += b; s2 += a * a + b * b; s3r += w * a; s3i += w * b;
-    }
     snprintf(path, sizeof path, "%s/sums.txt", outdir);
-    f = fopen(path, "w");
-    fprintf(f, "S1 %.17g %.17g\nS2 %.17g\nS3 %.17g %.17g\n", s1r, s1i, s2, s3r, s3i);
-    for (unsigned long k = 0; k < 64; k++) {
-      unsigned long j = 1 + (unsigned long)((k * 2654435761ULL) % dim);
-      fprintf(f, "E %lu %.17g %.17g\n", j, creal(v0[j]), cimag(v0[j]));
-    }
-    fclose(f);
+    if (write_sums(path, v0, dim)) return 4;
   } else {
     fprintf(stderr, "unknown mode %s\n", mode);
     return 2;

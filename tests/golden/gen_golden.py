@@ -30,7 +30,12 @@ reference made it (ini_rand<i>.bin: global_1st_norm, then the normalised v1);
 the file writers are checked byte for byte against them and the TPQ
 restatements against their numbers.
 
-usage: python tests/golden/gen_golden.py [--large | --two-body | --green | --files]
+Block columns (pass --large-block): the same checksums for each of the 16
+synthetic vectors of seeds 0..15 of the triangular 13x2 workload (harness mode
+`sums_seeds`, one reference mltply per seed), i.e. the columns of the k = 16
+block its mTPQ runs.
+
+usage: python tests/golden/gen_golden.py [--large | --large-block | --two-body | --green | --files]
 """
 
 from __future__ import annotations
@@ -180,6 +185,34 @@ def run_large(n, work):
     return rec
 
 
+def run_large_block(n, k, work):
+    """Reference checksums of H v_s for the k synthetic vectors of seeds
+    s0 .. s0+k-1: the columns of a k-column block filled from seed s0."""
+    spec = L.baseline_config(n)
+    d = os.path.join(work, f"cfg{n}_k{k}")
+    L.write_expert(spec, d)
+    s0 = {2: 1, 3: 2, 4: 0}[n]
+    O.run_reference(d, "sums_seeds", d + "/out", seed=s0, reps=k, timeout=6 * 3600)
+    cols = []
+    for s in range(s0, s0 + k):
+        pr = open(f"{d}/out/prdct_{s}.txt").read().split()
+        rec = {"seed": s, "prdct": [float(pr[0]), float(pr[1])], "samples": []}
+        for ln in open(f"{d}/out/sums_{s}.txt"):
+            p = ln.split()
+            if not p:
+                continue
+            if p[0] == "S1":
+                rec["S1"] = [float(p[1]), float(p[2])]
+            elif p[0] == "S2":
+                rec["S2"] = float(p[1])
+            elif p[0] == "S3":
+                rec["S3"] = [float(p[1]), float(p[2])]
+            elif p[0] == "E":
+                rec["samples"].append([int(p[1]), float(p[2]), float(p[3])])  # 1-based index
+        cols.append(rec)
+    return {"dim": spec.dim, "seed": s0, "k": k, "columns": cols}
+
+
 def main():
     out_path = os.path.join(HERE, "golden.json")
     gold = json.load(open(out_path)) if os.path.exists(out_path) else {}
@@ -195,6 +228,10 @@ def main():
             for name, spec in green_cases().items():
                 gold.setdefault("cases_green", {})[name] = run_green(name, spec, work)
                 print(name, "done", flush=True)
+        elif "--large-block" in sys.argv:
+            n, k = 4, 16
+            gold.setdefault("large", {})[f"cfg{n}_k{k}"] = run_large_block(n, k, work)
+            print("large block", n, k, "done", flush=True)
         elif "--large" in sys.argv:
             for n in [int(a) for a in sys.argv[sys.argv.index("--large") + 1:] if a.isdigit()] or [4, 2, 3]:
                 gold.setdefault("large", {})[f"cfg{n}"] = run_large(n, work)

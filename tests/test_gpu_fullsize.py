@@ -79,3 +79,53 @@ def test_full_size_hv_matches_reference_checksums(A, n, tuning):
     # prdct = <v1|H v1>: rounding scales with |v1| |H v1| = sqrt(2 dim / 3 * S2) for this vector
     scale = np.sqrt(2.0 * dim / 3.0 * s2)
     assert abs(pr - complex(*rec["prdct"])) <= TOL * scale
+
+
+def test_full_size_block16_matches_reference_checksums(A):
+    """The k = 16 block H·v of the triangular 13x2 workload in its default
+    grand-canonical shape (the kernel of its mTPQ step): column c is the
+    synthetic vector of seed c, checked column by column against the
+    reference's mltply on that vector (gen_golden.py --large-block)."""
+    rec = GOLD.get("cfg4_k16")
+    if rec is None:
+        pytest.skip("no reference block checksums")
+    k = rec["k"]
+    spec = L.baseline_config(4)
+    print(f"[gpu] full-size block H·v workload 4 dim {spec.dim} k={k}", flush=True)
+    eng = A.Engine.from_spec(spec)
+    dim = eng.dim
+    assert dim == rec["dim"]
+    V1 = A.DeviceArray(dim * k)
+    eng.fill_random(V1, k, rec["seed"])
+    V0 = A.DeviceArray(dim * k).zero()
+    pr = eng.mltply_block_dev(k, V0, V1)
+    V1.free()
+    h = V0.download().reshape(dim, k)
+    V0.free()
+    eng.close()
+    # per-column checksums in one pass over the block
+    s1 = np.zeros(k, np.complex128)
+    s2 = np.zeros(k)
+    s3 = np.zeros(k, np.complex128)
+    l1 = np.zeros(k)
+    l1w = np.zeros(k)
+    step = 1 << 22
+    for a in range(0, dim, step):
+        c = h[a:a + step]
+        j = np.arange(a + 1, a + 1 + c.shape[0], dtype=np.int64)
+        w = (j % 97 + 1).astype(np.float64)
+        ab = np.abs(c)
+        s1 += c.sum(axis=0)
+        s2 += (ab * ab).sum(axis=0)
+        s3 += w @ c
+        l1 += ab.sum(axis=0)
+        l1w += w @ ab
+    for col, cr in enumerate(rec["columns"]):
+        assert abs(s2[col] - cr["S2"]) <= TOL * cr["S2"], col
+        assert abs(s1[col] - complex(*cr["S1"])) <= TOL * l1[col], col
+        assert abs(s3[col] - complex(*cr["S3"])) <= TOL * l1w[col], col
+        amax = float(np.max(np.abs(h[:, col])))
+        for jj, re, im in cr["samples"]:
+            assert abs(h[jj - 1, col] - complex(re, im)) <= TOL * amax, (col, jj)
+        scale = np.sqrt(2.0 * dim / 3.0 * s2[col])
+        assert abs(pr[col] - complex(*cr["prdct"])) <= TOL * scale, col

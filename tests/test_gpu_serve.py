@@ -135,3 +135,35 @@ def test_many_ghost_copies_do_not_stall_convergence(A):
     for r, p in zip(out, plain):
         assert r.converged and r.n_found == 4 and r.iterations < 2000, (r.iterations, r.n_found)
         assert np.max(np.abs(np.array(r.eigenvalues[:4]) - p)) <= EV_TOL
+
+
+def test_served_stream_sample_matches_independent_solver(A):
+    """Requests of the served stream itself (seed 3; every model name of the
+    first 2,000 requests and a second N >= 22 request of each, N up to 24 spins,
+    dim up to 2,704,156), served together with the benchmark's settings (seed =
+    stream index): every job converges and its lowest 4 distinct eigenvalues
+    are within 1e-10 of ARPACK / dense eigvalsh on the CPU oracle
+    (golden/serve_stream.json, gen_serve_stream.py)."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "serve_stream.json")
+    if not os.path.exists(path):
+        pytest.skip("no served-stream fixtures")
+    gold = json.load(open(path))
+    rng = np.random.default_rng(gold["stream_seed"])
+    last = max(p["index"] for p in gold["picks"])
+    reqs = [L.stream_request(rng) for _ in range(last + 1)]
+    picks = gold["picks"]
+    specs = [reqs[p["index"]] for p in picks]
+    for p, s in zip(picks, specs):
+        assert s.name == p["name"] and s.dim == p["dim"]
+    print(f"[gpu] serve {len(specs)} stream requests, dims {min(s.dim for s in specs)}..{max(s.dim for s in specs)}",
+          flush=True)
+    out = run_server(A, specs, [p["index"] for p in picks], budget=96 << 30, max_jobs=512)
+    worst = 0.0
+    for p, r in zip(picks, out):
+        ref = np.array(p["eigenvalues"][:4])
+        got = np.array(r.eigenvalues[: r.n_found])
+        assert r.converged and r.n_found >= min(4, len(ref)), (p["index"], p["name"], r.iterations)
+        err = float(np.max(np.abs(got[: len(ref)] - ref)))
+        worst = max(worst, err)
+        assert err <= EV_TOL, (p["index"], p["name"], got, ref)
+    print(f"[gpu] worst |dE| {worst:.2e} over {len(picks)} requests", flush=True)
