@@ -331,6 +331,48 @@ def reduce_pmc(fetch_dir: str, write_dir: str, out: str, cmd: str, cfg: int = 3)
     print("\n".join(lines))
 
 
+ISSUE_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_SMEM",
+                  "SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES")
+
+
+def reduce_issue(directory: str, out: str, cmd: str, cfg: int = 2):
+    """Issue-side counters of the plain H·v (one rocprofv3 --pmc pass of
+    --pmc-probe with the SQ counters of ISSUE_COUNTERS): mean per dispatch, per
+    64-state tile (one wave's worth of states), and the VALU issue time a tile
+    needs (4 cycles per wave64 instruction on a 16-lane SIMD) against the
+    cycles a tile may take at the measured launch time."""
+    from hphi_amd import lattice as L
+
+    spec = L.baseline_config(cfg)
+    dim = spec.dim
+    vals = {c: _pmc_means(directory, c).get("k_hv_canon") for c in ISSUE_COUNTERS}
+    tiles = dim / 64.0
+    lines = [f"# issue counters of one plain H·v launch (k_hv_canon), {PMC_WORKLOADS[cfg][1]}",
+             f"# command: {cmd}"]
+    for c in ISSUE_COUNTERS:
+        v = vals[c]
+        lines.append(f"{c}: {v:.6e} per dispatch" + (f", {v / tiles:.1f} per 64-state tile" if v else ""))
+    hv_ms = _probe_hv_ms(directory)
+    if vals["SQ_INSTS_VALU"] and hv_ms:
+        simds = 1024
+        cyc_tile = hv_ms * 1e-3 * 2.4e9 * simds / tiles
+        lines.append(f"H·v {hv_ms:.3f} ms -> {cyc_tile:.0f} SIMD cycles per tile (1024 SIMDs at 2.4 GHz)")
+        lines.append(f"VALU issue: {4 * vals['SQ_INSTS_VALU'] / tiles:.0f} cycles per tile "
+                     f"({100 * 4 * vals['SQ_INSTS_VALU'] / tiles / cyc_tile:.1f} % of the tile's cycles)")
+    open(out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def _probe_hv_ms(directory: str):
+    """hv_ms the probe printed into the run directory's stdout copy, if kept."""
+    p = os.path.join(directory, "probe.json")
+    if os.path.exists(p):
+        for ln in open(p):
+            if ln.startswith("{"):
+                return float(json.loads(ln)["hv_ms"])
+    return None
+
+
 def cpu_leg(args):
     import numpy as np
 
@@ -420,7 +462,10 @@ def serve_leg(args):
         conv = sum(r.converged for r in res)
         unconv = [{"id": int(r.id), "dim": int(r.dim), "iterations": int(r.iterations),
                    "model": reqs[r.id].name} for r in res if not r.converged]
+        hv = {"hv_gbs": round(st["hv_bytes"] / st["hv_s"] / 1e9, 1) if st["hv_s"] > 0 else None,
+              "hv_frac": round(st["hv_bytes"] / st["hv_s"] / 1e9 / HBM_PEAK_GBS, 4) if st["hv_s"] > 0 else None}
         points.append({"rate": rate, "solves_per_s": len(res) / makespan, "p50_s": float(np.percentile(lat, 50)),
+                       **hv,
                        "unconverged": unconv[:8],
                        "p99_s": float(np.percentile(lat, 99)), "makespan_s": makespan,
                        "converged": int(conv), "mean_iterations": float(np.mean([r.iterations for r in res])),
@@ -428,6 +473,54 @@ def serve_leg(args):
     out["points"] = points
     best = max(p["solves_per_s"] for p in points)
     out["speedup_vs_sequential"] = best / out["sequential_solves_per_s"]
+    return out
+
+
+PER_SIZE = (("chain16", lambda j2: ("heisenberg_chain", (16, 1.0, j2))),
+            ("ladder20", lambda j2: ("j1j2_ladder", (10, 1.0, j2))),
+            ("j1j2_6x4", lambda j2: ("j1j2_square", (6, 4, 1.0, j2))))
+
+
+def serve_per_size(args, n_jobs: int = 32, n_plain: int = 4):
+    """Served vs plain per job size: n_jobs requests of one lattice (random
+    J2, seed 5) submitted at once to the service, and the first n_plain of them
+    solved one at a time by the plain path (one engine each), both with the
+    stream's settings (lowest 4 distinct eigenvalues, eps 1e-12, check every
+    5 steps).  ms per solve each way and their ratio."""
+    import numpy as np
+
+    from hphi_amd import capi, lattice as L
+
+    rng = np.random.default_rng(5)
+    out = {}
+    for name, make in PER_SIZE:
+        specs = []
+        for _ in range(n_jobs):
+            fn, a = make(float(rng.uniform(0.0, 1.0)))
+            specs.append(getattr(L, fn)(*a))
+        t = time.time()
+        for i in range(n_plain):
+            eng = capi.Engine.from_spec(specs[i])
+            eng.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=i)
+            eng.close()
+        plain_ms = 1e3 * (time.time() - t) / n_plain
+        srv = capi.Server(hbm_budget=int(args.hbm_gb * (1 << 30)), max_jobs=args.max_jobs, n_eigen=4, eps=1e-12,
+                          check_every=5)
+        t0 = time.time()
+        for i, sp in enumerate(specs):
+            srv.submit(sp, i, 0.0)
+        done = 0
+        while done < n_jobs:
+            srv.tick()
+            done += len(srv.results())
+        served_ms = 1e3 * (time.time() - t0) / n_jobs
+        st = srv.stats()
+        srv.close()
+        out[name] = {"dim": int(specs[0].dim), "plain_ms_per_solve": round(plain_ms, 2),
+                     "served_ms_per_solve": round(served_ms, 2), "ratio": round(plain_ms / served_ms, 2),
+                     "slab_hv_gbs": round(st["hv_bytes"] / st["hv_s"] / 1e9, 1) if st["hv_s"] > 0 else None}
+        print(f"[serve] {name}: plain {plain_ms:.2f} ms, served {served_ms:.2f} ms per solve", file=sys.stderr,
+              flush=True)
     return out
 
 
@@ -448,6 +541,9 @@ def main():
     ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic_hv_j1j2_8x4.txt from two rocprofv3 --pmc runs of --pmc-probe")
     ap.add_argument("--pmc-cmd", type=str, default="", help="command line recorded in the traffic file")
+    ap.add_argument("--reduce-issue", type=str, metavar="DIR",
+                    help="write profiles/issue_<workload>.txt from a rocprofv3 --pmc run of --pmc-probe with the "
+                         "SQ issue counters (DIR/probe.json: the probe's stdout)")
     ap.add_argument("--pmc-cfg", type=int, default=3, choices=(2, 3), help="workload of --pmc-probe / --reduce-pmc")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-sample", type=int, default=1000,
@@ -480,6 +576,10 @@ def main():
         reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1],
                    os.path.join(ROOT, "profiles", f"traffic_{PMC_WORKLOADS[args.pmc_cfg][0]}.txt"), args.pmc_cmd,
                    args.pmc_cfg)
+        return
+    if args.reduce_issue:
+        reduce_issue(args.reduce_issue, os.path.join(ROOT, "profiles", f"issue_{PMC_WORKLOADS[args.pmc_cfg][0]}.txt"),
+                     args.pmc_cmd, args.pmc_cfg)
         return
     if args.pmc_probe:
         pmc_probe(args)
@@ -547,6 +647,11 @@ def main():
             "speedup_vs_sequential": round(p0["solves_per_s"] / sv["sequential_solves_per_s"], 3),
             "rate_per_s": 120.0, "p50_s": round(p1["p50_s"], 3), "p99_s": round(p1["p99_s"], 3),
             "converged": [p0["converged"], p1["converged"]],
+            "slab_hv": {"kernel": "k_sl_hv", "achieved_gbs": p0["hv_gbs"], "frac": p0["hv_frac"],
+                        "launches": p0["hv_launches"],
+                        "note": "algorithmic bytes (per slab 8 dim, per running column 56 dim + 16 per "
+                                "off-diagonal element) over HIP-event time of the slab H·v launches"},
+            "per_size": serve_per_size(args),
             "note": "first requests of the seed-3 stream; bench.py --serve --requests 10000 runs the whole stream"}
     if not args.no_cpu:
         c = cpu_leg(args)

@@ -38,6 +38,8 @@ EXPORTED = [
     "hphi_read_restart_vecs", "hphi_write_tmcomponents", "hphi_read_tmcomponents", "hphi_write_energy",
     "hphi_write_cisajs", "hphi_write_cisajscktalt", "hphi_write_tpq_files", "hphi_lanczos_restart",
     "hphi_ctpq", "hphi_write_ctpq_files", "hphi_mltply_dev_scaled", "hphi_vec_lanczos_resid",
+    "hphi_model_nnz", "hphi_tpq_from", "hphi_ctpq_from",
+    "hphi_append_tpq_files", "hphi_lobpcg_ex", "hphi_write_energy_states",
 ]
 
 
@@ -137,6 +139,7 @@ def lib():
         L.hphi_engine_destroy.restype = None
         L.hphi_engine_info.argtypes = [_vp] + [_u64p] * 6
         L.hphi_engine_device_bytes.argtypes = [_vp]
+        L.hphi_model_nnz.argtypes = [C.POINTER(ModelDesc), _u64p, _u64p]
         L.hphi_engine_device_bytes.restype = C.c_uint64
         L.hphi_sz.argtypes = [_vp, _u64p, _u64p, _u64p]
         L.hphi_diagonalcalc.argtypes = [_vp, _f64p]
@@ -192,6 +195,11 @@ def lib():
         L.hphi_write_cisajs.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), _f64p]
         L.hphi_write_cisajscktalt.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int), _f64p]
         L.hphi_write_tpq_files.argtypes = [C.c_char_p, C.c_int, C.c_int, _f64p, C.c_int]
+        L.hphi_append_tpq_files.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _f64p, C.c_int, C.c_double]
+        L.hphi_write_energy_states.argtypes = [C.c_char_p, C.c_int, C.POINTER(Phys)]
+        L.hphi_tpq_from.argtypes = [_vp, C.POINTER(TpqParams), _vp, C.c_int, _f64p, C.POINTER(TpqResult), _vp]
+        L.hphi_ctpq_from.argtypes = [_vp, C.POINTER(CtpqParams), _vp, C.c_int, _f64p, C.POINTER(TpqResult), _vp]
+        L.hphi_lobpcg_ex.argtypes = [_vp, C.POINTER(LobpcgParams), _vp, _vp, _f64p, C.POINTER(LobpcgResult)]
         L.hphi_write_ctpq_files.argtypes = [C.c_char_p, C.c_int, C.c_int, _f64p, C.c_int]
         L.hphi_lanczos_restart.argtypes = [_vp, C.POINTER(LanczosParams), C.c_char_p, C.c_char_p, C.c_char_p,
                                            C.c_char_p, C.POINTER(LanczosResult)]
@@ -294,6 +302,15 @@ class Engine:
     def from_spec(cls, spec) -> "Engine":
         from .lattice import processed_terms
         return cls.from_terms(spec.model, spec.nsite, spec.nup, spec.ndown, processed_terms(spec))
+
+    @classmethod
+    def model_nnz(cls, spec) -> Tuple[int, int]:
+        """(dim, off-diagonal elements of H) of a canonical sector, host only."""
+        from .lattice import processed_terms
+        d, keep = cls.desc_from_terms(spec.model, spec.nsite, spec.nup, spec.ndown, processed_terms(spec))
+        dim, nnz = C.c_uint64(), C.c_uint64()
+        check(lib().hphi_model_nnz(C.byref(d), C.byref(dim), C.byref(nnz)))
+        return int(dim.value), int(nnz.value)
 
     @classmethod
     def from_def(cls, namelist: str) -> "Engine":
@@ -404,6 +421,39 @@ class Engine:
         r = LobpcgResult()
         check(lib().hphi_lobpcg(self.h, C.byref(p), _p(hist, _f64p), C.byref(r)))
         return r, hist[: r.iterations].copy()
+
+    def lobpcg_ex(self, k: int, X0: Optional[DeviceArray] = None, Xout: Optional[DeviceArray] = None,
+                  max_iter: int = 500, eps: float = 1e-7, precg: bool = False, large_value: float = 0.0,
+                  seed: int = 0):
+        """lobpcg from the start vectors X0 (k vectors one after another, or None
+        for the seeded ones), the final normalised eigenvectors into Xout."""
+        p = LobpcgParams(k, max_iter, eps, 1 if precg else 0, large_value, seed)
+        hist = np.zeros((max(max_iter, 1), k + 1))
+        r = LobpcgResult()
+        check(lib().hphi_lobpcg_ex(self.h, C.byref(p), X0.ptr if X0 is not None else None,
+                                   Xout.ptr if Xout is not None else None, _p(hist, _f64p), C.byref(r)))
+        return r, hist[: r.iterations].copy()
+
+    def tpq_from(self, W0: DeviceArray, k: int, steps: int, step0: int = 0, W_end: Optional[DeviceArray] = None,
+                 layout: int = 1, use_graph: bool = True, large_value: float = 0.0):
+        """Microcanonical TPQ from the vectors W0 (k vectors one after another);
+        step0 > 0 resumes (rows of steps step0..steps-1); returns (out, TpqResult)."""
+        p = TpqParams(k, steps, 0, layout, 1 if use_graph else 0, large_value, 0)
+        out = np.zeros((steps - step0, k, 12))
+        r = TpqResult()
+        check(lib().hphi_tpq_from(self.h, C.byref(p), W0.ptr, step0, _p(out, _f64p), C.byref(r),
+                                  W_end.ptr if W_end is not None else None))
+        return out, r
+
+    def ctpq_from(self, W0: DeviceArray, k: int, steps: int, step0: int = 0, W_end: Optional[DeviceArray] = None,
+                  expand_coef: int = 0, large_value: float = 0.0):
+        """Canonical TPQ from the vectors W0, as tpq_from."""
+        p = CtpqParams(k, steps, 0, expand_coef, large_value, 0)
+        out = np.zeros((steps - step0, k, 12))
+        r = TpqResult()
+        check(lib().hphi_ctpq_from(self.h, C.byref(p), W0.ptr, step0, _p(out, _f64p), C.byref(r),
+                                   W_end.ptr if W_end is not None else None))
+        return out, r
 
     def tpq(self, k: int, steps: int, seed: int = 0, init_kind: int = -1, layout: int = 1,
             use_graph: bool = True, large_value: float = 0.0):
@@ -532,10 +582,11 @@ class Server:
         return a.value, w.value
 
     def stats(self):
-        out = np.zeros(8)
+        out = np.zeros(9)
         check(lib().hphi_server_stats(self.h, _p(out, _f64p)))
         return {"admit_s": out[0], "gpu_s": out[1], "check_s": out[2], "ticks": int(out[3]),
-                "job_steps": int(out[4]), "slab_steps": int(out[5])}
+                "job_steps": int(out[4]), "slab_steps": int(out[5]), "hv_s": out[6], "hv_bytes": out[7],
+                "hv_launches": int(out[8])}
 
     def results(self, max_n: int = 4096):
         buf = (JobResult * max_n)()
@@ -629,3 +680,16 @@ def write_ctpq_files(directory: str, rand_i: int, rows: np.ndarray) -> None:
     """rows[steps, 12] of one canonical TPQ sample (a slice out[:, c] of Engine.ctpq)."""
     r = np.ascontiguousarray(rows, np.float64)
     check(lib().hphi_write_ctpq_files(directory.encode(), rand_i, r.shape[0], _p(r, _f64p), 12))
+
+
+def append_tpq_files(directory: str, rand_i: int, step0: int, rows: np.ndarray, norm1: float = 0.0) -> None:
+    """rows[steps - step0, 12] of one sample of a resumed run, appended."""
+    r = np.ascontiguousarray(rows, np.float64)
+    check(lib().hphi_append_tpq_files(directory.encode(), rand_i, step0, step0 + r.shape[0], _p(r, _f64p), 12,
+                                      norm1))
+
+
+def write_energy_states(path: str, phys) -> None:
+    """The block solvers' energy file from a list of phys dicts."""
+    arr = (Phys * len(phys))(*[Phys(**{k: p.get(k, 0.0) for k, _ in Phys._fields_}) for p in phys])
+    check(lib().hphi_write_energy_states(path.encode(), len(phys), arr))

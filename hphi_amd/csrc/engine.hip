@@ -17,6 +17,8 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <mutex>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -54,6 +56,49 @@ static uint64_t h_binom(int n, int k) {
 
 __device__ __forceinline__ uint64_t dbinom(int n, int k) {
   return (k < 0 || n < 0 || k > n) ? 0ull : c_binom[n][k];
+}
+
+// Off-diagonal elements of H in a canonical sector: for every compiled term,
+// the states x whose bits on M = tmask | flip form a pattern p that passes the
+// term's predicate and whose source x ^ flip keeps the particle numbers, times
+// the ways to fill the other bits (C(N - |M|, n - |p|) for Spin; one binomial
+// per species for Hubbard, up orbitals at even bits).  0 for grand-canonical
+// models (the caller counts those another way).
+uint64_t hphi_term_nnz(const hphi_engine *e) {
+  if (hphi_model_gc(e->model)) return 0;
+  const bool hub = hphi_model_hubbard(e->model);
+  const uint64_t up_bits = hub ? 0x5555555555555555ull : ~0ull;
+  const int nb = e->nbits;
+  const uint64_t all = nb >= 64 ? ~0ull : ((1ull << nb) - 1);
+  uint64_t total = 0;
+  for (const GTerm &t : e->h_terms_dev) {
+    const uint64_t M = (t.tmask | t.flip) & all;
+    int pos[64], k = 0;
+    for (int b = 0; b < nb; b++)
+      if ((M >> b) & 1) pos[k++] = b;
+    if (k > 20) continue;  // no compiled term touches that many bits
+    const int ku = __builtin_popcountll(M & up_bits), kd = k - ku;
+    for (uint64_t m = 0; m < (1ull << k); m++) {
+      uint64_t x = 0;
+      for (int q = 0; q < k; q++)
+        if ((m >> q) & 1) x |= 1ull << pos[q];
+      const uint64_t ym = x & t.tmask;
+      const int pc = __builtin_popcountll(ym);
+      const bool act = (t.kind == P_EQ) ? (ym == t.tval) : ((t.kind == P_DIFF) ? (pc == 1) : (pc != 1));
+      if (!act) continue;
+      const uint64_t y = x ^ t.flip;
+      if (hub) {
+        const int xu = __builtin_popcountll(x & up_bits), xd = __builtin_popcountll(x & ~up_bits);
+        if (xu != __builtin_popcountll(y & up_bits) || xd != __builtin_popcountll(y & ~up_bits)) continue;
+        total += h_binom(e->nsite - ku, e->nup - xu) * h_binom(e->nsite - kd, e->ndown - xd);
+      } else {
+        const int xs = __builtin_popcountll(x);
+        if (xs != __builtin_popcountll(y)) continue;
+        total += h_binom(nb - k, e->nup - xs);
+      }
+    }
+  }
+  return total;
 }
 
 int hphi_ensure_binom_table() {
@@ -379,6 +424,36 @@ static void classify_and_upload_terms(hphi_engine *e) {
       merged[it->second].cre += t.cre;
       merged[it->second].cim += t.cim;
     }
+  }
+  // A hop and its Hermitian partner (c+_a c_b and c+_b c_a, same coupling):
+  // both gather from x ^ (a|b) with the same between-bits sign, one when x
+  // holds a and not b, the other when x holds b and not a, i.e. together
+  // when exactly one of the two bits is set.  The pair becomes one P_DIFF
+  // term: half the predicates and rank lookups for the same gathers.
+  {
+    std::map<std::tuple<uint64_t, uint64_t, double, double>, size_t> open;  // (flip, sgn, c) -> index
+    std::vector<char> drop(merged.size(), 0);
+    for (size_t i = 0; i < merged.size(); i++) {
+      GTerm &t = merged[i];
+      if (t.kind != P_EQ || t.tmask != t.flip || __builtin_popcountll(t.flip) != 2 ||
+          __builtin_popcountll(t.tval) != 1 || (t.tval & ~t.flip))
+        continue;
+      const auto key = std::make_tuple(t.flip, t.sgn, t.cre, t.cim);
+      const auto it = open.find(key);
+      if (it != open.end() && merged[it->second].tval == (t.flip ^ t.tval)) {
+        GTerm &p = merged[it->second];
+        p.kind = P_DIFF;
+        p.tval = 0;
+        drop[i] = 1;
+        open.erase(it);
+      } else if (it == open.end()) {
+        open.emplace(key, i);
+      }
+    }
+    std::vector<GTerm> kept;
+    for (size_t i = 0; i < merged.size(); i++)
+      if (!drop[i]) kept.push_back(merged[i]);
+    merged.swap(kept);
   }
   // groups: low (flip within the low half), high (within the high half), mixed
   std::vector<GTerm> low, high, mixed;
@@ -748,6 +823,16 @@ int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e) {
   return 0;
 }
 
+extern "C" int hphi_model_nnz(const hphi_model_desc *d, uint64_t *dim, uint64_t *nnz) {
+  if (!d) return hphi_set_error("hphi_model_nnz: NULL argument");
+  hphi_engine e;
+  if (hphi_engine_prepare(d, &e)) return -1;
+  if (hphi_model_gc(e.model)) return hphi_set_error("hphi_model_nnz: canonical Spin and Hubbard models only");
+  if (dim) *dim = e.dim;
+  if (nnz) *nnz = hphi_term_nnz(&e);
+  return 0;
+}
+
 extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
   if (!d || !out) return hphi_set_error("hphi_engine_create: NULL argument");
   *out = nullptr;
@@ -761,8 +846,9 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
     delete e;
     return hphi_set_error("hipStreamCreate failed");
   }
+  if (hipGetDevice(&e->device) != hipSuccess) e->device = 0;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) e->num_cu = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, e->device) == hipSuccess) e->num_cu = prop.multiProcessorCount;
   size_t tb = std::max<size_t>(e->h_terms_dev.size(), 1) * sizeof(GTerm);
   size_t db = std::max<size_t>(e->h_dterms.size(), 1) * sizeof(DTerm);
   if (hipMalloc(&e->d_terms, tb) != hipSuccess || hipMalloc(&e->d_dterms, db) != hipSuccess) {
@@ -828,6 +914,14 @@ extern "C" void hphi_engine_destroy(hphi_engine *e) {
   (void)hipFree(e->d_coef);
   (void)hipFree(e->d_stage0);
   (void)hipFree(e->d_stage1);
+  for (int t = 0; t < e->n_host; t++) {
+    if (e->h_stream[t]) (void)hipStreamSynchronize(e->h_stream[t]);
+    for (int b = 0; b < 2; b++) {
+      if (e->h_bounce[t][b]) (void)hipHostFree(e->h_bounce[t][b]);
+      if (e->h_ev[t][b]) (void)hipEventDestroy(e->h_ev[t][b]);
+    }
+    if (e->h_stream[t]) (void)hipStreamDestroy(e->h_stream[t]);
+  }
   if (e->h_prdct) (void)hipHostFree(e->h_prdct);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -884,6 +978,8 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
     e->grid_per_cu = value;
   } else if (k == "use_blk") {
     e->use_blk = value ? 1 : 0;
+  } else if (k == "lb_mfma") {
+    e->lb_mfma = value ? 1 : 0;
   } else if (k == "blk_tpb") {
     if (value != 0 && value != 256 && value != 1024) return hphi_set_error("blk_tpb must be 0, 256 or 1024");
     e->bk_tpb = value;
@@ -914,6 +1010,7 @@ extern "C" int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int 
   else if (k == "use_c2") *value = e->use_c2 && e->c2_ok;
   else if (k == "grid_per_cu") *value = e->grid_per_cu;
   else if (k == "use_blk") *value = e->use_blk && hphi_blk_ok(e);
+  else if (k == "lb_mfma") *value = e->lb_mfma;
   else if (k == "blk_tpb" || k == "blk_wgs_per_cu" || k.rfind("blk_cpl", 0) == 0) {
     int kk = k.rfind("blk_cpl", 0) == 0 ? atoi(k.c_str() + 7) : 16;
     if (kk != 2 && kk != 4 && kk != 8 && kk != 16) return hphi_set_error("unknown tuning knob %s", knob);
@@ -1078,53 +1175,63 @@ int hphi_hv_launch(hphi_engine *e, int k, double *v0, const double *v1) {
   return hphi_hv_launch_ex(e, k, v0, v1, 0);
 }
 
-static int hv_launch_args(hphi_engine *e, int k, HvArgs &A);
+static int hv_launch_args(hphi_engine *e, int k, HvArgs &A, double2 *prdct_out);
 
 int hphi_hv_launch_ex(hphi_engine *e, int k, double *v0, const double *v1, int overwrite) {
   HvArgs A = hphi_hv_args(e, k, v0, v1);
   A.overwrite = overwrite;
-  return hv_launch_args(e, k, A);
+  return hv_launch_args(e, k, A, (double2 *)e->d_prdct);
+}
+
+// which kernel a launch for k columns uses, and its workgroups (each writes k
+// partial sums of prdct)
+enum { HV_PATH_CANON, HV_PATH_BLK, HV_PATH_GENERIC };
+static int hv_path(const hphi_engine *e, int k, int *grid) {
+  if (k == 1 && e->c2_ok && e->use_c2) {
+    *grid = canon_grid(e);
+    return HV_PATH_CANON;
+  }
+  if (k > 1 && e->use_blk && hphi_blk_ok(e)) {
+    *grid = hphi_blk_grid(e, k);
+    return HV_PATH_BLK;
+  }
+  *grid = hphi_hv_grid(e, k);
+  return HV_PATH_GENERIC;
 }
 
 // launch of prepared arguments: the canonical single-vector kernel, the block
-// kernel, or the generic kernel; then the prdct partials are reduced
-static int hv_launch_args(hphi_engine *e, int k, HvArgs &A) {
-  if (k == 1 && e->c2_ok && e->use_c2) {
-    int g = canon_grid(e);
+// kernel, or the generic kernel; then the partials in A.partial are reduced
+// into prdct_out[0..k)
+static int hv_launch_args(hphi_engine *e, int k, HvArgs &A, double2 *prdct_out) {
+  int g = 0;
+  const int path = hv_path(e, k, &g);
+  if (path == HV_PATH_CANON) {
     if (hphi_hv_launch_canon(e, A, g)) return -1;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, e->stream, (const double2 *)e->d_partial,
-                       g, 1, (double2 *)e->d_prdct);
-    HIPCHK(hipGetLastError());
-    return 0;
-  }
-  if (k > 1 && e->use_blk && hphi_blk_ok(e)) {
-    int g = 0;
-    if (hphi_hv_launch_blk(e, A, k, &g)) return -1;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, e->stream, (const double2 *)e->d_partial,
-                       g, k, (double2 *)e->d_prdct);
-    HIPCHK(hipGetLastError());
-    return 0;
-  }
-  int g = hphi_hv_grid(e, k);
-  bool canon = !hphi_model_gc(e->model);
-  static_assert(HV_TERM_PAD % HV_UNROLL == 0, "a batch must not cross a term group");
-  A.e_low = std::min(A.n_low, (e->r_low + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
-  A.e_high = std::min(A.n_high, (e->r_high + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
-  A.e_mixed = std::min(A.n_mixed, (e->r_mixed + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
+  } else if (path == HV_PATH_BLK) {
+    int gb = 0;
+    if (hphi_hv_launch_blk(e, A, k, &gb)) return -1;
+    g = gb;
+  } else {
+    bool canon = !hphi_model_gc(e->model);
+    static_assert(HV_TERM_PAD % HV_UNROLL == 0, "a batch must not cross a term group");
+    A.e_low = std::min(A.n_low, (e->r_low + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
+    A.e_high = std::min(A.n_high, (e->r_high + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
+    A.e_mixed = std::min(A.n_mixed, (e->r_mixed + HV_UNROLL - 1) / HV_UNROLL * HV_UNROLL);
 #define LAUNCH(KK)                                                                                   \
   if (canon) hipLaunchKernelGGL((k_hv<KK, true>), dim3(g), dim3(HV_TPB), 0, e->stream, A);          \
   else hipLaunchKernelGGL((k_hv<KK, false>), dim3(g), dim3(HV_TPB), 0, e->stream, A);
-  switch (k) {
-    case 1: LAUNCH(1); break;
-    case 2: LAUNCH(2); break;
-    case 4: LAUNCH(4); break;
-    case 8: LAUNCH(8); break;
-    case 16: LAUNCH(16); break;
-    default: return hphi_set_error("block size k=%d not in {1,2,4,8,16}", k);
-  }
+    switch (k) {
+      case 1: LAUNCH(1); break;
+      case 2: LAUNCH(2); break;
+      case 4: LAUNCH(4); break;
+      case 8: LAUNCH(8); break;
+      case 16: LAUNCH(16); break;
+      default: return hphi_set_error("block size k=%d not in {1,2,4,8,16}", k);
+    }
 #undef LAUNCH
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, e->stream, (const double2 *)e->d_partial,
-                     g, k, (double2 *)e->d_prdct);
+  }
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, e->stream, (const double2 *)A.partial, g, k,
+                     prdct_out);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1135,11 +1242,15 @@ static int ensure_ready(hphi_engine *e) {
   return 0;
 }
 
-int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct) {
-  HIPCHK(hipMemcpyAsync(e->h_prdct, e->d_prdct, sizeof(double2) * k, hipMemcpyDeviceToHost, e->stream));
+static int fetch_prdct_from(hphi_engine *e, const double2 *d_src, int k, double *prdct) {
+  HIPCHK(hipMemcpyAsync(e->h_prdct, d_src, sizeof(double2) * k, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   if (prdct) memcpy(prdct, e->h_prdct, sizeof(double2) * k);
   return 0;
+}
+
+int hphi_fetch_prdct(hphi_engine *e, int k, double *prdct) {
+  return fetch_prdct_from(e, (const double2 *)e->d_prdct, k, prdct);
 }
 
 extern "C" int hphi_mltply_dev(hphi_engine *e, double *v0, const double *v1, double *prdct) {
@@ -1157,7 +1268,7 @@ extern "C" int hphi_mltply_dev_scaled(hphi_engine *e, double *v0, const double *
   A.overwrite = overwrite ? 1 : 0;
   A.c0 = c0;
   A.c1 = c1;
-  if (hv_launch_args(e, 1, A)) return -1;
+  if (hv_launch_args(e, 1, A, (double2 *)e->d_prdct)) return -1;
   return hphi_fetch_prdct(e, 1, prdct);
 }
 
@@ -1178,22 +1289,118 @@ static bool is_device_ptr(const void *p) {
   return at.type == hipMemoryTypeDevice;
 }
 
+// Host-pointer calls move v1 in and H v1 out through pinned bounce buffers:
+// n_host worker threads each own a contiguous share of the vector, a stream
+// and two HOST_CHUNK buffers, so CPU copies (pageable <-> pinned) of one chunk
+// overlap the DMA of the other and the threads' DMAs run side by side.  v0
+// never crosses to the device: the kernel writes H v1 (overwrite) and the
+// threads add it to the host v0 as it comes back, which is the kernel's own
+// v0 + H v1 rounding (one addition).
+#define HOST_CHUNK ((size_t)16 << 20)
+#define HOST_THREADS 16
+
+static int host_setup(hphi_engine *e) {
+  if (e->n_host) return 0;
+  unsigned hw = std::thread::hardware_concurrency();
+  int n = (int)std::min<unsigned>(HOST_THREADS, std::max(1u, hw));
+  for (int t = 0; t < n; t++) {
+    if (hipStreamCreateWithFlags(&e->h_stream[t], hipStreamNonBlocking) != hipSuccess) break;
+    bool ok = true;
+    for (int b = 0; b < 2 && ok; b++)
+      ok = hipHostMalloc((void **)&e->h_bounce[t][b], HOST_CHUNK, hipHostMallocDefault) == hipSuccess &&
+           hipEventCreateWithFlags(&e->h_ev[t][b], hipEventDisableTiming) == hipSuccess;
+    e->n_host = t + 1;  // destroy frees whatever of this slot exists
+    if (!ok) return hphi_set_error("pinned bounce buffers for host-pointer calls: allocation failed");
+  }
+  if (!e->n_host) return hphi_set_error("streams for host-pointer calls: creation failed");
+  return 0;
+}
+
+// thread t's share [lo, hi) of n bytes, in whole complex128 elements
+static void share(size_t n, int t, int nt, size_t *lo, size_t *hi) {
+  const size_t ne = n / 16;
+  *lo = ne * (size_t)t / (size_t)nt * 16;
+  *hi = ne * (size_t)(t + 1) / (size_t)nt * 16;
+}
+
 extern "C" int hphi_mltply(hphi_engine *e, double *v0, const double *v1, double *prdct) {
   if (!e || !v0 || !v1) return hphi_set_error("hphi_mltply: NULL argument");
   if (ensure_ready(e)) return -1;
   bool d0 = is_device_ptr(v0), d1 = is_device_ptr(v1);
   if (d0 && d1) return hphi_mltply_dev(e, v0 + 2, v1 + 2, prdct);
-  // compatibility path: stage the 1-indexed host arrays through the device
-  size_t bytes = sizeof(double2) * e->dim;
+  if (d0 != d1) return hphi_set_error("hphi_mltply: v0 and v1 must both be host or both be device memory");
+  // compatibility path: the reference's 1-indexed host arrays
+  const size_t bytes = sizeof(double2) * e->dim;
   if (!e->d_stage0) {
     HIPCHK(hipMalloc(&e->d_stage0, bytes));
     HIPCHK(hipMalloc(&e->d_stage1, bytes));
     e->bytes += 2 * bytes;
   }
-  HIPCHK(hipMemcpyAsync(e->d_stage1, v1 + 2, bytes, d1 ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, e->stream));
-  HIPCHK(hipMemcpyAsync(e->d_stage0, v0 + 2, bytes, d0 ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, e->stream));
-  if (hphi_hv_launch(e, 1, e->d_stage0, e->d_stage1)) return -1;
-  HIPCHK(hipMemcpyAsync(v0 + 2, e->d_stage0, bytes, d0 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e->stream));
+  if (host_setup(e)) return -1;
+  const int nt = e->n_host;
+  char *dv1 = (char *)e->d_stage1, *dout = (char *)e->d_stage0;
+  const char *hv1 = (const char *)(v1 + 2);
+  double *hv0 = v0 + 2;
+  std::vector<int> err(nt, 0);
+  // 1. v1 in
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+      th.emplace_back([&, t]() {
+        size_t lo, hi;
+        share(bytes, t, nt, &lo, &hi);
+        int b = 0;
+        for (size_t o = lo; o < hi; o += HOST_CHUNK, b ^= 1) {
+          const size_t n = std::min(HOST_CHUNK, hi - o);
+          if (hipEventSynchronize(e->h_ev[t][b]) != hipSuccess) { err[t] = 1; return; }
+          memcpy(e->h_bounce[t][b], hv1 + o, n);
+          if (hipMemcpyAsync(dv1 + o, e->h_bounce[t][b], n, hipMemcpyHostToDevice, e->h_stream[t]) != hipSuccess ||
+              hipEventRecord(e->h_ev[t][b], e->h_stream[t]) != hipSuccess) {
+            err[t] = 1;
+            return;
+          }
+        }
+      });
+    for (auto &x : th) x.join();
+  }
+  for (int t = 0; t < nt; t++) {
+    if (err[t]) return hphi_set_error("hphi_mltply: host-to-device copy failed");
+    for (int b = 0; b < 2; b++) HIPCHK(hipStreamWaitEvent(e->stream, e->h_ev[t][b], 0));
+  }
+  // 2. H v1 on the device (overwrite)
+  if (hphi_hv_launch_ex(e, 1, e->d_stage0, e->d_stage1, 1)) return -1;
+  hipEvent_t done;
+  HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(done, e->stream));
+  // 3. H v1 out, added to v0 on the host; each thread keeps one chunk in flight
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+      th.emplace_back([&, t]() {
+        size_t lo, hi;
+        share(bytes, t, nt, &lo, &hi);
+        if (hipStreamWaitEvent(e->h_stream[t], done, 0) != hipSuccess) { err[t] = 1; return; }
+        auto issue = [&](size_t o, int b) {
+          const size_t n = std::min(HOST_CHUNK, hi - o);
+          return hipMemcpyAsync(e->h_bounce[t][b], dout + o, n, hipMemcpyDeviceToHost, e->h_stream[t]) == hipSuccess &&
+                 hipEventRecord(e->h_ev[t][b], e->h_stream[t]) == hipSuccess;
+        };
+        int b = 0;
+        if (lo < hi && !issue(lo, 0)) { err[t] = 1; return; }
+        for (size_t o = lo; o < hi; o += HOST_CHUNK, b ^= 1) {
+          if (o + HOST_CHUNK < hi && !issue(o + HOST_CHUNK, b ^ 1)) { err[t] = 1; return; }
+          if (hipEventSynchronize(e->h_ev[t][b]) != hipSuccess) { err[t] = 1; return; }
+          const size_t n = std::min(HOST_CHUNK, hi - o) / sizeof(double);
+          const double *src = (const double *)e->h_bounce[t][b];
+          double *dst = hv0 + o / sizeof(double);
+          for (size_t i = 0; i < n; i++) dst[i] += src[i];
+        }
+      });
+    for (auto &x : th) x.join();
+  }
+  (void)hipEventDestroy(done);
+  for (int t = 0; t < nt; t++)
+    if (err[t]) return hphi_set_error("hphi_mltply: device-to-host copy failed");
   return hphi_fetch_prdct(e, 1, prdct);
 }
 

@@ -137,6 +137,12 @@ struct hphi_engine {
   int r_low = 0, r_high = 0, r_mixed = 0;  // compiled terms per group, before padding
   void *d_partial = nullptr, *d_prdct = nullptr, *h_prdct = nullptr, *d_coef = nullptr;
   double *d_stage0 = nullptr, *d_stage1 = nullptr;
+  // host-pointer calls: per worker thread a stream and two pinned bounce
+  // buffers of HOST_CHUNK bytes, with the events that free them
+  int n_host = 0;
+  hipStream_t h_stream[16] = {};
+  char *h_bounce[16][2] = {};
+  hipEvent_t h_ev[16][2] = {};
   uint64_t bytes = 0;
   // canonical single-vector kernel (hv_canon.hip) configuration
   int c2_ok = 0, c2_realc = 1, c2_lds21 = 0, c2_max_lds = 0;
@@ -147,6 +153,8 @@ struct hphi_engine {
   // workgroup size (0: by table size), workgroups per CU (0: by workgroup size)
   int bk_cpl[4] = {0, 0, 0, 0};
   int bk_tpb = 0, bk_wgs_per_cu = 0, use_blk = 1;
+  int device = 0;  // the device current when the engine was created
+  int lb_mfma = 1;  // LOBPCG block-8 Gram on the FP64 matrix cores
 };
 
 extern "C" int hphi_set_error(const char *fmt, ...);
@@ -161,6 +169,8 @@ void hphi_crank_tables(uint16_t *t);
 void hphi_hrank_tables(uint16_t *t);
 int hphi_engine_prepare(const hphi_model_desc *d, hphi_engine *e);
 int hphi_ensure_binom_table();
+// off-diagonal elements of H in a canonical sector (0 for grand-canonical models)
+uint64_t hphi_term_nnz(const hphi_engine *e);
 SzParams hphi_sz_params(const hphi_engine *e);
 int hphi_launch_tables(const SzParams &P, uint64_t sdim, uint64_t size21, uint64_t *jb, uint32_t *l21, uint64_t *l22, uint64_t *list1, hipStream_t st);
 // diag[j * stride + col] for j < dim
@@ -170,4 +180,5 @@ int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g);
 int hphi_blk_ok(const hphi_engine *e);
 int hphi_blk_cpl_ok(int k, int cpl);
 void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_per_cu);
+int hphi_blk_grid(const hphi_engine *e, int k);
 int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid);

@@ -158,6 +158,20 @@ int hphi_write_energy(const char *path, const hphi_phys *p) {
   return fclose(f) == 0 ? 0 : hphi_set_error("write failed: %s", path);
 }
 
+int hphi_write_energy_states(const char *path, int n, const hphi_phys *p) {
+  if (!path || (n > 0 && !p)) return hphi_set_error("hphi_write_energy_states: NULL argument");
+  FILE *f = open_or_fail(path, "w");
+  if (!f) return -1;
+  for (int i = 0; i < n; i++) {
+    fprintf(f, "State %ld\n", (long)i);
+    fprintf(f, "  Energy  %.16lf \n", p[i].energy);
+    fprintf(f, "  Doublon  %.16lf \n", p[i].doublon);
+    fprintf(f, "  Sz  %.16lf \n", p[i].sz);
+    fprintf(f, "\n");
+  }
+  return fclose(f) == 0 ? 0 : hphi_set_error("write failed: %s", path);
+}
+
 int hphi_write_cisajs(const char *path, int n, const int *idx, const double *val) {
   if (!path || (n > 0 && (!idx || !val))) return hphi_set_error("hphi_write_cisajs: NULL argument");
   FILE *f = open_or_fail(path, "w");
@@ -232,4 +246,34 @@ int hphi_write_tpq_files(const char *dir, int rand_i, int steps, const double *r
 
 int hphi_write_ctpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride) {
   return write_tpq_files(dir, rand_i, steps, rows, row_stride, 0);
+}
+
+int hphi_append_tpq_files(const char *dir, int rand_i, int step0, int steps, const double *rows, int row_stride,
+                          double norm1) {
+  if (!dir || (steps > step0 && !rows)) return hphi_set_error("hphi_append_tpq_files: NULL argument");
+  char p_ss[4096], p_norm[4096], p_flct[4096];
+  snprintf(p_ss, sizeof p_ss, "%s/SS_rand%d.dat", dir, rand_i);
+  snprintf(p_norm, sizeof p_norm, "%s/Norm_rand%d.dat", dir, rand_i);
+  snprintf(p_flct, sizeof p_flct, "%s/Flct_rand%d.dat", dir, rand_i);
+  FILE *fs = open_or_fail(p_ss, "a");
+  if (!fs) return -1;
+  FILE *fn = open_or_fail(p_norm, "a");
+  if (!fn) {
+    fclose(fs);
+    return -1;
+  }
+  FILE *ff = open_or_fail(p_flct, "a");
+  if (!ff) {
+    fclose(fs);
+    fclose(fn);
+    return -1;
+  }
+  for (int s = step0; s < steps; s++) {
+    const double *o = rows + (size_t)(s - step0) * row_stride;
+    fprintf(fs, "%.16lf  %.16lf %.16lf %.16lf %.16lf %d\n", o[0], o[1], o[2], o[4], o[6], s);
+    fprintf(fn, "%.16lf %.16lf %.16lf %d\n", o[0], o[3], norm1, s);
+    fprintf(ff, "%.16lf %.16lf %.16lf %.16lf %.16lf %.16lf %.16lf %d\n", o[0], o[6], o[7], o[4], o[5], o[8], o[9], s);
+  }
+  const int e1 = fclose(fs), e2 = fclose(fn), e3 = fclose(ff);
+  return (e1 || e2 || e3) ? hphi_set_error("write failed in %s", dir) : 0;
 }

@@ -371,17 +371,25 @@ void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_pe
   *wgs_per_cu = w;
 }
 
-int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid) {
+// workgroups of the block kernel's launch for k columns (each writes k partials)
+int hphi_blk_grid(const hphi_engine *e, int k) {
   int cpl, tpb, wpc;
   hphi_blk_shape(e, k, &cpl, &tpb, &wpc);
-  const bool canon = !hphi_model_gc(e->model);
   const uint64_t spt = (uint64_t)tpb / (uint64_t)(k / cpl);
   const uint64_t ntiles = (e->dim + spt - 1) / spt;
   uint64_t g = std::min<uint64_t>(ntiles, (uint64_t)wpc * std::max(e->num_cu, 1));
   g = std::max<uint64_t>(g, 8);
   g = (g + 7) / 8 * 8;
   if (g > HV_MAX_GRID) g = HV_MAX_GRID;
-  *grid = (int)g;
+  return (int)g;
+}
+
+int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid) {
+  int cpl, tpb, wpc;
+  hphi_blk_shape(e, k, &cpl, &tpb, &wpc);
+  const bool canon = !hphi_model_gc(e->model);
+  const int g = hphi_blk_grid(e, k);
+  *grid = g;
   // each group's loop stops at its last batch holding a compiled term (the
   // groups are padded to HV_TERM_PAD, a multiple of every batch size)
   const int u = cpl >= 8 ? 1 : 8 / cpl;  // terms per batch of k_hv_blk
@@ -391,7 +399,7 @@ int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid) {
   B.e_high = std::min(A.n_high, (e->r_high + u - 1) / u * u);
   B.e_mixed = std::min(A.n_mixed, (e->r_mixed + u - 1) / u * u);
 #define CASE(KK, CC) \
-  if (k == KK && cpl == CC) return bk_launch_tpb<KK, CC>(e, B, (int)g, tpb, canon);
+  if (k == KK && cpl == CC) return bk_launch_tpb<KK, CC>(e, B, g, tpb, canon);
   CASE(2, 1)
   CASE(2, 2)
   CASE(4, 1)

@@ -39,6 +39,8 @@ int hphi_lb_resid(hphi_lb_ws *ws, uint64_t dim, const double *X, const double *x
                   const double *s, double *norms2);
 int hphi_lb_precond(hphi_lb_ws *ws, uint64_t dim, double *w, const double *diag, const double *shift, double eps,
                     double *norms2);
+int hphi_lb_load(hphi_lb_ws *ws, uint64_t dim, const double *src, double *x);
+int hphi_lb_store(hphi_lb_ws *ws, uint64_t dim, const double *x, const double *sx, double *dst);
 int hphi_lb_gram(hphi_lb_ws *ws, uint64_t dim, const double *w, const double *x, const double *p, const double *W,
                  const double *X, const double *P, double *gram);
 int hphi_lb_update(hphi_lb_ws *ws, uint64_t dim, const double *w, double *x, double *p, const double *W, double *X,
@@ -190,7 +192,24 @@ static double calc_preshift(double eig, double res, double eps) {
   return trunc(eig / pow(10.0, k + i)) * pow(10.0, k + i);
 }
 
+static int lobpcg_run(hphi_engine *e, const hphi_lobpcg_params *prm, const double *X0, double *Xout, double *history,
+                      hphi_lobpcg_result *r);
+
 int hphi_lobpcg(hphi_engine *e, const hphi_lobpcg_params *prm, double *history, hphi_lobpcg_result *r) {
+  return lobpcg_run(e, prm, NULL, NULL, history, r);
+}
+
+int hphi_lobpcg_ex(hphi_engine *e, const hphi_lobpcg_params *prm, const double *X0, double *Xout, double *history,
+                   hphi_lobpcg_result *r) {
+  return lobpcg_run(e, prm, X0, Xout, history, r);
+}
+
+/* X0 (may be NULL): device, k start vectors of dim complex128 one after
+ * another (a restart's tmpvec_set<c> vectors, Initialize_wave :172-213), in
+ * place of the synthetic vectors of seed + c; Xout (may be NULL): the final
+ * normalised block, the same layout (wxp[1] of :716-724). */
+static int lobpcg_run(hphi_engine *e, const hphi_lobpcg_params *prm, const double *X0, double *Xout, double *history,
+                      hphi_lobpcg_result *r) {
   if (!e || !prm || !r) return hphi_set_error("hphi_lobpcg: NULL argument");
   memset(r, 0, sizeof *r);
   const int k = prm->k;
@@ -221,8 +240,10 @@ int hphi_lobpcg(hphi_engine *e, const hphi_lobpcg_params *prm, double *history, 
     if (hphi_dev_alloc(bytes, (void **)&blk[i])) goto done;
   double *w = blk[0], *x = blk[1], *p = blk[2], *W = blk[3], *X = blk[4], *P = blk[5];
 
-  /* initial block: column c from seed + c, normalised (Initialize_wave, :166-318) */
-  if (hphi_fill_random(x, dim, k, prm->seed) || hphi_vec_norm2(e, x, k, nrm)) goto done;
+  /* initial block: the given vectors, or column c from seed + c; normalised
+   * (Initialize_wave, :166-318) */
+  if (X0 ? hphi_lb_load(ws, dim, X0, x) : hphi_fill_random(x, dim, k, prm->seed)) goto done;
+  if (hphi_vec_norm2(e, x, k, nrm)) goto done;
   for (int c = 0; c < k; c++) sx[c] = 1.0 / sqrt(nrm[c]), sp[c] = 1.0, sw[c] = 1.0;
   if (hphi_dev_zero(p, bytes) || hphi_dev_zero(P, bytes)) goto done;
   {
@@ -313,6 +334,7 @@ int hphi_lobpcg(hphi_engine *e, const hphi_lobpcg_params *prm, double *history, 
   }
   r->converged = conv;
   r->max_residual = dmax;
+  if (Xout && hphi_lb_store(ws, dim, x, sx, Xout)) goto done;
   rc = 0;
 done:
   for (int i = 0; i < 6; i++)

@@ -163,6 +163,150 @@ __global__ __launch_bounds__(LB_TPB) void k_lb_gram(const double2 *w, const doub
   for (int e = tid; e < R * C; e += LB_TPB) out[e] = red[e];
 }
 
+// ---- Gram blocks on the FP64 matrix cores (k = 8) --------------------------
+// Per state the 24 complex columns of (w, x, p) are 48 reals r (block b's 8
+// columns are reals 16b .. 16b+15, interleaved re/im as stored) and the 24 of
+// (W, X, P) are 48 reals y.  Over all states the real products R2 = r^T r and
+// R1 = r^T y hold both complex Gram matrices:
+//   <b_a|c_b> = (R[2a][2b] + R[2a+1][2b+1]) + i (R[2a][2b+1] - R[2a+1][2b]).
+// Only the 16 x 16 real tiles (I, J) with I <= J are formed (6 per matrix):
+// they hold every complex entry a <= b, the triangle zheevd('U') reads
+// (CalcByLOBPCG.c:51,69,105); the other triangle is its conjugate.
+// One v_mfma_f64_16x16x4f64 per tile and 4 states: lane l loads real column
+// l % 16 of state 4q + l / 16 from each block (64 lanes read 512 contiguous
+// bytes), which is operand A[i = l % 16][kk = l / 16] of r and B[kk][j] of y;
+// D[i][j] sits in lane 16 (i % 4) + j, register i / 4 (checked on the device).
+#define LBM_TPB 256
+#define LBM_GRID 768       // 3 workgroups per CU = 3 waves per SIMD (152 VGPRs); each wave owns a contiguous range of states
+#define LBM_TILES 12
+#define LBM_PART (LBM_TILES * 256)
+#define LBM_SPLIT 16       // first reduction pass: partial groups per output
+typedef double lbm_d4 __attribute__((ext_vector_type(4)));
+
+// the 12 tile products of one group of 4 states (operands as loaded)
+__device__ __forceinline__ void lbm_tiles(lbm_d4 *acc, const double *v) {
+  const double a0 = v[0], a1 = v[1], a2 = v[2], b0 = v[3], b1 = v[4], b2 = v[5];
+  acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc[0], 0, 0, 0);    // R2 (0,0)
+  acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a1, acc[1], 0, 0, 0);    // R2 (0,1)
+  acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a2, acc[2], 0, 0, 0);    // R2 (0,2)
+  acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, acc[3], 0, 0, 0);    // R2 (1,1)
+  acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a2, acc[4], 0, 0, 0);    // R2 (1,2)
+  acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, a2, acc[5], 0, 0, 0);    // R2 (2,2)
+  acc[6] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[6], 0, 0, 0);    // R1 (0,0)
+  acc[7] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[7], 0, 0, 0);    // R1 (0,1)
+  acc[8] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b2, acc[8], 0, 0, 0);    // R1 (0,2)
+  acc[9] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[9], 0, 0, 0);    // R1 (1,1)
+  acc[10] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b2, acc[10], 0, 0, 0);  // R1 (1,2)
+  acc[11] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc[11], 0, 0, 0);  // R1 (2,2)
+}
+
+__global__ __launch_bounds__(LBM_TPB) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_lb_gram_mfma8(const double *w, const double *x, const double *p,
+                                                           const double *W, const double *X, const double *P,
+                                                           uint64_t dim, double *partial) {
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col = lane & 15, sub = lane >> 4;
+  const double *src[6] = {w, x, p, W, X, P};
+  lbm_d4 acc[LBM_TILES];
+#pragma unroll
+  for (int t = 0; t < LBM_TILES; t++) acc[t] = lbm_d4{0.0, 0.0, 0.0, 0.0};
+  // whole groups of 4 states, a contiguous range per wave; loads run two
+  // groups ahead of their products through three register sets used in
+  // rotation (no copies, so no wait on a load in flight).  Load indices are
+  // clamped into the range: every load is in bounds, with no branch around it.
+  const uint64_t nq = dim / 4;
+  const uint64_t nw = (uint64_t)gridDim.x * (LBM_TPB / 64);
+  const uint64_t gw = (uint64_t)blockIdx.x * (LBM_TPB / 64) + wave;
+  const uint64_t q0 = nq * gw / nw, q1 = nq * (gw + 1) / nw;
+  if (q1 > q0) {
+    // wave-uniform bases at the wave's first group and 32-bit lane offsets (a
+    // wave's range is a few MB), so a load is a scalar base plus one VGPR
+    const double *bw[6];
+#pragma unroll
+    for (int b = 0; b < 6; b++) bw[b] = src[b] + 64 * q0;
+    const uint32_t lo = (uint32_t)(sub * 16 + col), last = (uint32_t)(64 * (q1 - 1 - q0)) + lo;
+    double ra[6], rb[6], rc[6];
+    auto ld = [&](double *v, uint32_t e) {
+      e = e < last ? e : last;
+#pragma unroll
+      for (int b = 0; b < 6; b++) v[b] = bw[b][e];
+    };
+    ld(ra, lo);
+    ld(rb, lo + 64);
+    uint64_t q = q0;
+    uint32_t e = lo;
+    for (; q + 3 <= q1; q += 3, e += 192) {
+      ld(rc, e + 128);
+      lbm_tiles(acc, ra);
+      ld(ra, e + 192);
+      lbm_tiles(acc, rb);
+      ld(rb, e + 256);
+      lbm_tiles(acc, rc);
+    }
+    if (q < q1) lbm_tiles(acc, ra);
+    if (q + 1 < q1) lbm_tiles(acc, rb);
+  }
+  // the last dim % 4 states: one group, zero-padded, on the grid's first wave
+  if (dim % 4 && blockIdx.x == 0 && wave == 0) {
+    const uint64_t j = 4 * nq + sub;
+    double v[6];
+#pragma unroll
+    for (int b = 0; b < 6; b++) v[b] = j < dim ? src[b][j * 16 + col] : 0.0;
+    lbm_tiles(acc, v);
+  }
+  // the waves' tiles summed in a fixed order through LDS: red[t][i][j]
+  __shared__ double red[LBM_PART];
+  for (int wv = 0; wv < LBM_TPB / 64; wv++) {
+    if (wave == wv) {
+#pragma unroll
+      for (int t = 0; t < LBM_TILES; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int idx = t * 256 + (sub + 4 * r) * 16 + col;
+          red[idx] = wv == 0 ? acc[t][r] : red[idx] + acc[t][r];
+        }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < LBM_PART; i += LBM_TPB) partial[(uint64_t)blockIdx.x * LBM_PART + i] = red[i];
+}
+
+// first pass: out[s][i] = sum of partial[b][i] over the blocks b of group s
+// (LBM_SPLIT contiguous groups), fixed order
+__global__ __launch_bounds__(256) void k_lb_gram_sum1(const double *partial, int nblk, double *out) {
+  const int i = blockIdx.x * 256 + threadIdx.x, s = blockIdx.y;
+  if (i >= LBM_PART) return;
+  const int b0 = (int)((int64_t)nblk * s / LBM_SPLIT), b1 = (int)((int64_t)nblk * (s + 1) / LBM_SPLIT);
+  double a = 0.0;
+  for (int b = b0; b < b1; b++) a += partial[(uint64_t)b * LBM_PART + i];
+  out[s * LBM_PART + i] = a;
+}
+
+// second pass and the complex Gram matrices: gram[(r * 48 + c) * 2 + {re, im}]
+// = <b_r|y_c>, r < 24 over (w, x, p), c < 48 over (w, x, p, W, X, P)
+__device__ __forceinline__ double lbm_real(const double *sums, int t, int i, int j) {
+  double a = 0.0;
+  for (int s = 0; s < LBM_SPLIT; s++) a += sums[s * LBM_PART + t * 256 + i * 16 + j];
+  return a;
+}
+
+__global__ __launch_bounds__(256) void k_lb_gram_fin8(const double *sums, double *gram) {
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= 24 * 48) return;
+  const int r = id / 48, c = id % 48;
+  const bool h = c >= 24;
+  const int cb = h ? c - 24 : c;
+  const bool mirror = r > cb;  // the entry is conj(<b_cb|y_r>)
+  const int a = mirror ? cb : r, b = mirror ? r : cb;
+  const int I = a / 8, J = b / 8;  // I <= J
+  const int t = (I == 0 ? J : (I == 1 ? 2 + J : 5)) + (h ? 6 : 0);  // (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+  const int i0 = 2 * (a % 8), j0 = 2 * (b % 8);
+  const double rr = lbm_real(sums, t, i0, j0), ii = lbm_real(sums, t, i0 + 1, j0 + 1);
+  const double ri = lbm_real(sums, t, i0, j0 + 1), ir = lbm_real(sums, t, i0 + 1, j0);
+  const double re = rr + ii, im = ri - ir;
+  gram[2 * id] = re;
+  gram[2 * id + 1] = mirror ? -im : im;
+}
+
 // x' = sum_r b_r cx[r][c], X' = sum_r B_r cx[r][c]  (all 3K rows);
 // p' = sum_{r in w,p} b_r cx[r][c], P' likewise   (CalcByLOBPCG.c:508-556);
 // in place; |x'_c|^2 and |p'_c|^2 per workgroup.  K lanes own one state, lane
@@ -266,6 +410,7 @@ struct hphi_lb_ws {
   double2 *coef = nullptr;
   hipStream_t stream = nullptr;
   int k = 0;
+  int mfma = 1;  // k = 8 Gram on the matrix cores (0: the scalar kernel)
 };
 
 static int lb_grid(uint64_t n) {
@@ -280,7 +425,9 @@ extern "C" int hphi_lb_ws_create(hphi_engine *e, int k, hphi_lb_ws **out) {
   hphi_lb_ws *w = new hphi_lb_ws();
   w->k = k;
   w->stream = e->stream;
-  const size_t gram = (size_t)(LB_GRAM_GRID > LB_GRID ? LB_GRAM_GRID : LB_GRID) * 18 * k * k * 2;  // doubles
+  w->mfma = e->lb_mfma;
+  size_t gram = (size_t)(LB_GRAM_GRID > LB_GRID ? LB_GRAM_GRID : LB_GRID) * 18 * k * k * 2;  // doubles
+  gram = std::max(gram, (size_t)(LBM_GRID + LBM_SPLIT) * LBM_PART);  // the matrix-core Gram's partials
   if (hipMalloc(&w->part, sizeof(double) * gram) != hipSuccess ||
       hipMalloc(&w->res, sizeof(double) * 18 * k * k * 2) != hipSuccess ||
       hipMalloc(&w->coef, sizeof(double2) * 3 * k * k) != hipSuccess ||
@@ -356,6 +503,21 @@ extern "C" int hphi_lb_precond(hphi_lb_ws *ws, uint64_t dim, double *w, const do
 extern "C" int hphi_lb_gram(hphi_lb_ws *ws, uint64_t dim, const double *w, const double *x, const double *p,
                             const double *W, const double *X, const double *P, double *gram) {
   const int k = ws->k;
+  if (k == 8 && ws->mfma) {
+    double *sums = ws->part + (size_t)LBM_GRID * LBM_PART;
+    hipLaunchKernelGGL(k_lb_gram_mfma8, dim3(LBM_GRID), dim3(LBM_TPB), 0, ws->stream, w, x, p, W, X, P, dim,
+                       ws->part);
+    hipLaunchKernelGGL(k_lb_gram_sum1, dim3((LBM_PART + 255) / 256, LBM_SPLIT), dim3(256), 0, ws->stream,
+                       (const double *)ws->part, LBM_GRID, sums);
+    hipLaunchKernelGGL(k_lb_gram_fin8, dim3((24 * 48 + 255) / 256), dim3(256), 0, ws->stream,
+                       (const double *)sums, ws->res);
+    HIPCHK(hipGetLastError());
+    const size_t m = 18 * k * k * 2;
+    HIPCHK(hipMemcpyAsync(ws->h_res, ws->res, sizeof(double) * m, hipMemcpyDeviceToHost, ws->stream));
+    HIPCHK(hipStreamSynchronize(ws->stream));
+    memcpy(gram, ws->h_res, sizeof(double) * m);
+    return 0;
+  }
   const int g = (int)std::min<uint64_t>(LB_GRAM_GRID, std::max<uint64_t>(1, (dim + 15) / 16));
 #define CALL(K)                                                                                              \
   hipLaunchKernelGGL(k_lb_gram<K>, dim3(g), dim3(LB_TPB), 0, ws->stream, (const double2 *)w,                 \
@@ -381,6 +543,44 @@ extern "C" int hphi_lb_update(hphi_lb_ws *ws, uint64_t dim, const double *w, dou
 #undef CALL
   HIPCHK(hipGetLastError());
   return lb_finish(ws, g, 2 * k, norms2);
+}
+
+// x[j][c] = src[c*dim + j] (k vectors one after another -> state-major block)
+__global__ __launch_bounds__(256) void k_lb_load(const double2 *src, double2 *x, uint64_t dim, int k) {
+  const uint64_t n = dim * (uint64_t)k;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t c = i / dim, j = i % dim;
+    x[j * k + c] = src[i];
+  }
+}
+
+// dst[c*dim + j] = sx_c x[j][c]: the normalised columns, one vector after another
+__global__ __launch_bounds__(256) void k_lb_store(const double2 *x, double2 *dst, uint64_t dim, int k, LbCols cf) {
+  const uint64_t n = dim * (uint64_t)k;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t c = i / dim, j = i % dim;
+    const double2 a = x[j * k + c];
+    const double f = cf.a[c];
+    dst[i] = make_double2(f * a.x, f * a.y);
+  }
+}
+
+extern "C" int hphi_lb_load(hphi_lb_ws *ws, uint64_t dim, const double *src, double *x) {
+  hipLaunchKernelGGL(k_lb_load, dim3(lb_grid(dim * ws->k)), dim3(256), 0, ws->stream, (const double2 *)src,
+                     (double2 *)x, dim, ws->k);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+extern "C" int hphi_lb_store(hphi_lb_ws *ws, uint64_t dim, const double *x, const double *sx, double *dst) {
+  LbCols cf;
+  for (int c = 0; c < 16; c++) cf.a[c] = c < ws->k ? sx[c] : 0.0, cf.b[c] = 0.0;
+  hipLaunchKernelGGL(k_lb_store, dim3(lb_grid(dim * ws->k)), dim3(256), 0, ws->stream, (const double2 *)x,
+                     (double2 *)dst, dim, ws->k, cf);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  return 0;
 }
 
 // W = H w (overwrite) for a block of k columns, no prdct read-back

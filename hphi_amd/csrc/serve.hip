@@ -700,6 +700,7 @@ struct Slab {
   DTerm *dterms = nullptr;  // [SL_K][nd]
   double *hist = nullptr;   // [SL_K][2][max_iter + 2]
   std::vector<double2> coef_host;
+  uint64_t nnz = 0;  // off-diagonal elements of one column's H (hphi_term_nnz)
   Job *col[SL_K] = {nullptr, nullptr, nullptr, nullptr};
   int used() const {
     int n = 0;
@@ -745,6 +746,10 @@ struct hphi_server {
   size_t max_tiles = 0;
   double st_admit = 0, st_gpu = 0, st_check = 0;  // seconds spent per phase
   uint64_t st_ticks = 0, st_job_steps = 0, st_slab_steps = 0;
+  // slab H·v launches: HIP-event time, algorithmic bytes, launches
+  hipEvent_t ev_hv[2 * SL_SMAX] = {};
+  double st_hv_s = 0, st_hv_bytes = 0;
+  uint64_t st_hv_launches = 0;
 };
 
 extern "C" double hphi_server_now(const hphi_server *s) { return s ? mono_now() - s->t0 : 0.0; }
@@ -781,6 +786,8 @@ extern "C" int hphi_server_create(const hphi_server_params *p, hphi_server **out
   s->stage_size = (size_t)16 << 20;
   if (!rc && hipHostMalloc(&s->h_stage, s->stage_size, hipHostMallocDefault) != hipSuccess)
     rc = hphi_set_error("hphi_server_create: pinned staging allocation failed");
+  for (int i = 0; i < 2 * SL_SMAX && !rc; i++)
+    if (hipEventCreate(&s->ev_hv[i]) != hipSuccess) rc = hphi_set_error("hphi_server_create: hipEventCreate failed");
   if (rc) {
     hphi_server_destroy(s);
     return -1;
@@ -809,6 +816,8 @@ extern "C" void hphi_server_destroy(hphi_server *s) {
   (void)hipFree(s->d_ev);
   if (s->h_ev) (void)hipHostFree(s->h_ev);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
+  for (int i = 0; i < 2 * SL_SMAX; i++)
+    if (s->ev_hv[i]) (void)hipEventDestroy(s->ev_hv[i]);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -868,6 +877,7 @@ static int slab_create(hphi_server *s, const Job *j, size_t off, size_t bytes, S
   if (s->stage_used + al256(tb) > s->stage_size) return 1;
   Slab *b = new Slab();
   b->proto = h;
+  b->nnz = hphi_term_nnz(&h);
   b->shash = j->shash;
   b->off = off;
   b->bytes = bytes;
@@ -1062,6 +1072,10 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
     std::vector<ColDev> cd;
     std::vector<uint64_t> prefix;
     std::vector<int> s_ns(S), s_nc(S), s_sd0(S), s_cd0(S), s_pf0(S);
+    // algorithmic bytes of each step's slab H·v: per slab 8 dim (list_1), per
+    // running column 56 dim (own row, diagonal, Q read and write) + 16 per
+    // off-diagonal element (the gathered row entry)
+    std::vector<double> s_bytes(S, 0.0);
     std::map<const Slab *, int> index;
     const size_t hl = (size_t)s->p.max_iter + 2;
     uint64_t tiles = 0, max21 = 0;
@@ -1099,6 +1113,8 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
         d.n_mixed = h.n_mixed;
         d.nlow = h.nlow;
         index[b] = (int)sd.size() - s_sd0[st];
+        s_bytes[st] += 8.0 * (double)h.dim +
+                       (double)__builtin_popcount(d.mask) * (56.0 * (double)h.dim + 16.0 * (double)b->nnz);
         sd.push_back(d);
         prefix.push_back(prefix.back() + (h.dim + (uint64_t)SL_TPW * SV_TPB - 1) / ((uint64_t)SL_TPW * SV_TPB));
         max21 = std::max<uint64_t>(max21, h.size21);
@@ -1134,8 +1150,10 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
                          s->d_part);
       hipLaunchKernelGGL(k_sl_norm, dim3(nc), dim3(256), 0, s->stream, dcl, dpf, (const double *)s->d_part,
                          s->d_js);
+      HIPCHK(hipEventRecord(s->ev_hv[2 * st], s->stream));
       hipLaunchKernelGGL(k_sl_hv, dim3(T), dim3(SV_TPB), lds, s->stream, dsl, dpf, ns, (const double *)s->d_js,
                          (double2 *)s->d_part);
+      HIPCHK(hipEventRecord(s->ev_hv[2 * st + 1], s->stream));
       hipLaunchKernelGGL(k_sl_alpha, dim3(nc), dim3(256), 0, s->stream, dcl, dpf, (const double2 *)s->d_part,
                          s->d_js, out);
       if (s_chk[st]) hipLaunchKernelGGL(k_sl_check, dim3(nc), dim3(64), 0, s->stream, dcl, s->p.n_eigen, s->d_ev);
@@ -1147,6 +1165,14 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
     HIPCHK(hipStreamSynchronize(s->stream));
     now = hphi_server_now(s);
     s->st_gpu += now - tg;
+    for (int st = 0; st < S; st++) {
+      if (!s_nc[st]) continue;
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, s->ev_hv[2 * st], s->ev_hv[2 * st + 1]));
+      s->st_hv_s += 1e-3 * ms;
+      s->st_hv_bytes += s_bytes[st];
+      s->st_hv_launches++;
+    }
     s->st_ticks++;
     // 3. convergence and retirement
     std::vector<Job *> keep;
@@ -1221,6 +1247,9 @@ extern "C" int hphi_server_stats(const hphi_server *s, double *out) {
   out[3] = (double)s->st_ticks;
   out[4] = (double)s->st_job_steps;
   out[5] = (double)s->st_slab_steps;
+  out[6] = s->st_hv_s;
+  out[7] = s->st_hv_bytes;
+  out[8] = (double)s->st_hv_launches;
   return 0;
 }
 

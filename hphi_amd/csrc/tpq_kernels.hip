@@ -153,7 +153,7 @@ __global__ void k_tpq_start(const double *partial, int nblk, int k, double *stat
 // one block: output row of step s, next scale
 __global__ void k_tpq_finalize(const double *partial, int nblk, int k, const double2 *prdct, double *state,
                                double lv, double Ns, int obs_kind, int nsite, double sz_const, double *out,
-                               int max_steps) {
+                               int max_steps, int row0) {
   const int c = threadIdx.x;
   const int s = (int)state[32];
   __syncthreads();
@@ -164,10 +164,10 @@ __global__ void k_tpq_finalize(const double *partial, int nblk, int k, const dou
     const double n = state[c];
     const double E = prdct[c].x / n;
     const double var = r[0] / n;
-    if (s < max_steps) {
+    if (s >= row0 && s < max_steps) {
       // row of SS_rand / Norm_rand / Flct_rand (CalcByTPQ.c:150-210,235-256) with
       // the per-model moments of expec_energy_flct (expec_energy_flct.c:186-500)
-      double *o = out + ((uint64_t)s * k + c) * TQ_NOUT;
+      double *o = out + ((uint64_t)(s - row0) * k + c) * TQ_NOUT;
       o[0] = (s == 0) ? 0.0 : (2.0 * s / Ns) / (lv - E / Ns);
       o[1] = E;
       o[2] = var;
@@ -254,6 +254,7 @@ struct TpqPlan {
   uint64_t sj, sc;
   double lv, Ns, sz_const;
   int obs_kind, steps;
+  int row0;  // first step whose row is kept (out holds steps row0 .. steps-1)
 };
 
 // enqueue one step on e->stream (capturable: kernels only)
@@ -276,12 +277,53 @@ static int tpq_enqueue_step(hphi_engine *e, const TpqPlan &P) {
                      P.part);
   hipLaunchKernelGGL(k_tpq_finalize, dim3(1), dim3(64), 0, e->stream, (const double *)P.part, P.g, P.k,
                      (const double2 *)(P.layout == 1 ? e->d_prdct : e->d_coef), P.state, P.lv, P.Ns, P.obs_kind,
-                     e->nsite, P.sz_const, P.out, P.steps);
+                     e->nsite, P.sz_const, P.out, P.steps, P.row0);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
+// w_c <- W0 column c (vector-major input, element c*dim + j) in the run's layout
+__global__ void k_tpq_load(double2 *v, const double2 *W0, uint64_t dim, int k, uint64_t sj, uint64_t sc) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim * (uint64_t)k) return;
+  const uint64_t c = i / dim, j = i % dim;
+  v[j * sj + c * sc] = W0[i];
+}
+
+// W_end column c = w_c / |w_c|, the normalised vector of the last step
+__global__ void k_tpq_store(double2 *W_end, const double2 *v, uint64_t dim, int k, uint64_t sj, uint64_t sc,
+                            const double *state) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim * (uint64_t)k) return;
+  const uint64_t c = i / dim, j = i % dim;
+  const double2 a = v[j * sj + c * sc];
+  const double f = state[16 + c];
+  W_end[i] = make_double2(a.x * f, a.y * f);
+}
+
+// the step counter of a resumed run (the first enqueued step is step0 - 1)
+__global__ void k_tpq_set_step(double *state, double s) { state[32] = s; }
+
+static int tpq_run(hphi_engine *e, const hphi_tpq_params *p, const double *W0, int step0, double *out_host,
+                   hphi_tpq_result *r, double *W_end);
+
 extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_host, hphi_tpq_result *r) {
+  return tpq_run(e, p, nullptr, 0, out_host, r, nullptr);
+}
+
+extern "C" int hphi_tpq_from(hphi_engine *e, const hphi_tpq_params *p, const double *W0, int step0, double *out_host,
+                             hphi_tpq_result *r, double *W_end) {
+  if (!W0) return hphi_set_error("hphi_tpq_from: NULL start vectors");
+  if (step0 < 0 || step0 >= (p ? p->steps : 0)) return hphi_set_error("hphi_tpq_from: step0 must be in [0, steps)");
+  return tpq_run(e, p, W0, step0, out_host, r, W_end);
+}
+
+// W0 == nullptr: random start vectors (seed); else column c of W0 starts
+// sample c: with step0 = 0 as w_0 (its norm is global_1st_norm), with step0 > 0
+// as the normalised vector of step step0 - 1 of an earlier run, whose row is
+// recomputed and dropped.  Rows of steps step0 .. steps-1 go to out_host.
+static int tpq_run(hphi_engine *e, const hphi_tpq_params *p, const double *W0, int step0, double *out_host,
+                   hphi_tpq_result *r, double *W_end) {
   if (!e || !p || !out_host) return hphi_set_error("hphi_tpq: NULL argument");
   const int k = p->k;
   if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return hphi_set_error("hphi_tpq: k must be 1, 2, 4, 8 or 16");
@@ -315,23 +357,40 @@ extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_ho
     if (hipMalloc(&P.V0, vbytes) != hipSuccess || hipMalloc(&P.V1, vbytes) != hipSuccess ||
         hipMalloc(&P.state, sizeof(double) * 64) != hipSuccess ||
         hipMalloc(&P.part, sizeof(double) * (size_t)TQ_GRID * k * TQ_NRED) != hipSuccess ||
-        hipMalloc(&P.out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT) != hipSuccess) {
+        hipMalloc(&P.out, sizeof(double) * (size_t)(p->steps - step0) * k * TQ_NOUT) != hipSuccess) {
       (void)hipGetLastError();
       rc = hphi_set_error("hphi_tpq: device allocation failed");
       break;
     }
     uint64_t n = dim * (uint64_t)k;
-    hipLaunchKernelGGL(k_tpq_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, (double2 *)P.V1, dim, k,
-                       P.sj, P.sc, p->seed, p->init_kind);
+    const unsigned gn = (unsigned)((n + 255) / 256);
+    if (W0)
+      hipLaunchKernelGGL(k_tpq_load, dim3(gn), dim3(256), 0, e->stream, (double2 *)P.V1, (const double2 *)W0, dim, k,
+                         P.sj, P.sc);
+    else
+      hipLaunchKernelGGL(k_tpq_init, dim3(gn), dim3(256), 0, e->stream, (double2 *)P.V1, dim, k, P.sj, P.sc, p->seed,
+                         p->init_kind);
     hipLaunchKernelGGL(k_tpq_colnorm, dim3(P.g), dim3(TQ_TPB), 0, e->stream, (const double2 *)P.V1, dim, k, P.sj,
                        P.sc, P.part);
     hipLaunchKernelGGL(k_tpq_start, dim3(1), dim3(64), 0, e->stream, (const double *)P.part, P.g, k, P.state);
+    if (step0 > 0) {  // resumed: the given vector is step step0 - 1's, its row is not kept
+      hipLaunchKernelGGL(k_tpq_set_step, dim3(1), dim3(1), 0, e->stream, P.state, (double)(step0 - 1));
+      P.row0 = step0;
+    }
     if (hipGetLastError() != hipSuccess) { rc = hphi_set_error("hphi_tpq: init launch failed"); break; }
     if (hipEventRecord(ev0, e->stream) != hipSuccess) { rc = hphi_set_error("event record failed"); break; }
-    int s0 = 0;
-    if ((rc = tpq_enqueue_step(e, P))) break;  // step 0 eagerly (also loads the code objects)
-    s0 = 1;
-    if (p->use_graph && p->steps > 1) {
+    // W_end receives the vectors of the last step (steps - 1), normalised, as
+    // the reference writes them for a restart (CalcByTPQ.c:272-287): they are
+    // taken right before that step, whose fused pass moves on to the next vector
+    const int s_first = step0 > 0 ? step0 - 1 : 0, last = p->steps - 1;
+    auto store = [&]() {
+      hipLaunchKernelGGL(k_tpq_store, dim3(gn), dim3(256), 0, e->stream, (double2 *)W_end, (const double2 *)P.V1, dim,
+                         k, P.sj, P.sc, (const double *)P.state);
+    };
+    if (W_end && s_first == last) store();
+    if ((rc = tpq_enqueue_step(e, P))) break;  // the first step eagerly (also loads the code objects)
+    int s = s_first + 1;
+    if (p->use_graph && s <= last) {
       if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
         rc = hphi_set_error("hipStreamBeginCapture failed");
         break;
@@ -343,18 +402,24 @@ extern "C" int hphi_tpq(hphi_engine *e, const hphi_tpq_params *p, double *out_ho
         rc = hphi_set_error("hipGraphInstantiate failed");
         break;
       }
-      for (int s = s0; s < p->steps; s++)
-        if (hipGraphLaunch(exec, e->stream) != hipSuccess) { rc = hphi_set_error("hipGraphLaunch failed"); break; }
+      for (; s <= last && !rc; s++) {
+        if (W_end && s == last) store();
+        if (hipGraphLaunch(exec, e->stream) != hipSuccess) rc = hphi_set_error("hipGraphLaunch failed");
+      }
     } else {
-      for (int s = s0; s < p->steps && !rc; s++) rc = tpq_enqueue_step(e, P);
+      for (; s <= last && !rc; s++) {
+        if (W_end && s == last) store();
+        rc = tpq_enqueue_step(e, P);
+      }
     }
     if (rc) break;
+    if (hipGetLastError() != hipSuccess) { rc = hphi_set_error("hphi_tpq: launch failed"); break; }
     if (hipEventRecord(ev1, e->stream) != hipSuccess) { rc = hphi_set_error("event record failed"); break; }
     if (hipStreamSynchronize(e->stream) != hipSuccess) { rc = hphi_set_error("hphi_tpq: stream sync failed"); break; }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
-    if (hipMemcpy(out_host, P.out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT, hipMemcpyDeviceToHost) !=
-        hipSuccess) {
+    if (hipMemcpy(out_host, P.out, sizeof(double) * (size_t)(p->steps - step0) * k * TQ_NOUT,
+                  hipMemcpyDeviceToHost) != hipSuccess) {
       rc = hphi_set_error("hphi_tpq: copy of results failed");
       break;
     }
@@ -469,16 +534,24 @@ __global__ __launch_bounds__(TQ_TPB) void k_ct_moments(const double2 *P, const d
   }
 }
 
+// W_end column c (element c*dim + j) = P[j][c]
+__global__ void k_ct_store(double2 *W_end, const double2 *P, uint64_t dim, int k) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim * (uint64_t)k) return;
+  const uint64_t c = i / dim, j = i % dim;
+  W_end[i] = P[j * (uint64_t)k + c];
+}
+
 // row s of sample c (the mTPQ row layout): inv_temp (s additions of dtau), E, <H^2>, the
 // norm before normalisation (step 0: the start vector's), the moments
 __global__ void k_ct_finalize(const double *partial, int nblk, int k, const double2 *prdct, const double *state,
-                              int s, double dtau, int obs_kind, int nsite, double sz_const, double *out) {
+                              int s, double dtau, int obs_kind, int nsite, double sz_const, double *out, int row0) {
   const int c = threadIdx.x;
-  if (c >= k) return;
+  if (c >= k || s < row0) return;
   double r[TQ_NRED] = {};
   for (int b = 0; b < nblk; b++)
     for (int q = 0; q < TQ_NRED; q++) r[q] += partial[((uint64_t)b * k + c) * TQ_NRED + q];
-  double *o = out + ((uint64_t)s * k + c) * TQ_NOUT;
+  double *o = out + ((uint64_t)(s - row0) * k + c) * TQ_NOUT;
   double it = 0.0;  // inv_temp += delta_tau once per step (CalcByCanonicalTPQ.c:227)
   for (int i = 0; i < s; i++) it += dtau;
   o[0] = it;
@@ -501,7 +574,25 @@ __global__ void k_ct_finalize(const double *partial, int nblk, int k, const doub
   o[11] = 0.0;
 }
 
+static int ctpq_run(hphi_engine *e, const hphi_ctpq_params *p, const double *W0, int step0, double *out_host,
+                    hphi_tpq_result *r, double *W_end);
+
 extern "C" int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out_host, hphi_tpq_result *r) {
+  return ctpq_run(e, p, nullptr, 0, out_host, r, nullptr);
+}
+
+extern "C" int hphi_ctpq_from(hphi_engine *e, const hphi_ctpq_params *p, const double *W0, int step0,
+                              double *out_host, hphi_tpq_result *r, double *W_end) {
+  if (!W0) return hphi_set_error("hphi_ctpq_from: NULL start vectors");
+  if (step0 < 0 || step0 >= (p ? p->steps : 0)) return hphi_set_error("hphi_ctpq_from: step0 must be in [0, steps)");
+  return ctpq_run(e, p, W0, step0, out_host, r, W_end);
+}
+
+// W0 as in tpq_run: the start vectors (step0 = 0), or the normalised vectors of
+// step step0 - 1 of an earlier run, measured again without a row; rows of steps
+// step0 .. steps-1 go to out_host; W_end receives the last step's vectors.
+static int ctpq_run(hphi_engine *e, const hphi_ctpq_params *p, const double *W0, int step0, double *out_host,
+                    hphi_tpq_result *r, double *W_end) {
   if (!e || !p || !out_host) return hphi_set_error("hphi_ctpq: NULL argument");
   const int k = p->k;
   if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return hphi_set_error("hphi_ctpq: k must be 1, 2, 4, 8 or 16");
@@ -529,17 +620,22 @@ extern "C" int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out_
         hipMalloc(&A, vbytes) != hipSuccess || hipMalloc(&Cv, vbytes) != hipSuccess ||
         hipMalloc(&state, sizeof(double) * 64) != hipSuccess ||
         hipMalloc(&part, sizeof(double) * (size_t)TQ_GRID * k * TQ_NRED) != hipSuccess ||
-        hipMalloc(&out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT) != hipSuccess) {
+        hipMalloc(&out, sizeof(double) * (size_t)(p->steps - step0) * k * TQ_NOUT) != hipSuccess) {
       (void)hipGetLastError();
       rc = hphi_set_error("hphi_ctpq: device allocation failed");
       break;
     }
     if (hipEventRecord(ev0, e->stream) != hipSuccess) { rc = hphi_set_error("event record failed"); break; }
     // step 0: start vector (MakeIniVec), normalised, measured (CalcByCanonicalTPQ.c:140-175)
-    hipLaunchKernelGGL(k_tpq_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, A, dim, k,
-                       (uint64_t)k, (uint64_t)1, p->seed, p->init_kind);
-    for (int s = 0; s < p->steps && !rc; s++) {
-      if (s > 0) {  // A = sum_{m <= nc} (-dtau/2)^m / m! H^m P, with H P = Hp (MultiplyForCanonicalTPQ)
+    if (W0)
+      hipLaunchKernelGGL(k_tpq_load, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, A,
+                         (const double2 *)W0, dim, k, (uint64_t)k, (uint64_t)1);
+    else
+      hipLaunchKernelGGL(k_tpq_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, A, dim, k,
+                         (uint64_t)k, (uint64_t)1, p->seed, p->init_kind);
+    const int s_first = step0 > 0 ? step0 - 1 : 0;  // a resumed run re-measures the given vector first
+    for (int s = s_first; s < p->steps && !rc; s++) {
+      if (s > s_first) {  // A = sum_{m <= nc} (-dtau/2)^m / m! H^m P, with H P = Hp (MultiplyForCanonicalTPQ)
         double t = -0.5 * dtau;
         hipLaunchKernelGGL(k_ct_acc0, dim3(gs), dim3(256), 0, e->stream, A, (const double2 *)P,
                            (const double2 *)Hp, n, t);
@@ -564,7 +660,7 @@ extern "C" int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out_
                          dim, k, obs_kind, e->nsite, list1, part);
       hipLaunchKernelGGL(k_ct_finalize, dim3(1), dim3(64), 0, e->stream, (const double *)part, g, k,
                          (const double2 *)e->d_prdct, (const double *)state, s, dtau, obs_kind, e->nsite, sz_const,
-                         out);
+                         out, step0);
       if (hipGetLastError() != hipSuccess) { rc = hphi_set_error("hphi_ctpq: launch failed"); break; }
     }
     if (rc) break;
@@ -574,8 +670,16 @@ extern "C" int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out_
     }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
-    if (hipMemcpy(out_host, out, sizeof(double) * (size_t)p->steps * k * TQ_NOUT, hipMemcpyDeviceToHost) !=
-        hipSuccess) {
+    if (W_end) {  // P holds the last step's normalised vectors, state-major
+      hipLaunchKernelGGL(k_ct_store, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, (double2 *)W_end,
+                         (const double2 *)P, dim, k);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(e->stream) != hipSuccess) {
+        rc = hphi_set_error("hphi_ctpq: copy of the final vectors failed");
+        break;
+      }
+    }
+    if (hipMemcpy(out_host, out, sizeof(double) * (size_t)(p->steps - step0) * k * TQ_NOUT,
+                  hipMemcpyDeviceToHost) != hipSuccess) {
       rc = hphi_set_error("hphi_ctpq: copy of results failed");
       break;
     }

@@ -110,7 +110,8 @@ int hphi_engine_info(const hphi_engine *e, uint64_t *dim, uint64_t *size21, uint
  *   LDS), "xcd_map" 0|1 (contiguous state ranges per XCD), "use_c2" 0|1
  *   (32-bit canonical kernel), "grid_per_cu" 1..64 (generic kernel);
  *   block kernel (k > 1): "use_blk" 0|1, "blk_cpl2|4|8|16" columns per lane
- *   (1, 4 or k; 0 = default), "blk_tpb" 0|256|1024, "blk_wgs_per_cu" 0..64. */
+ *   (1, 4 or k; 0 = default), "blk_tpb" 0|256|1024, "blk_wgs_per_cu" 0..64;
+ *   LOBPCG: "lb_mfma" 0|1 (block-8 Gram matrices on the FP64 matrix cores). */
 int hphi_engine_tune(hphi_engine *e, const char *knob, int value);
 int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int *value);
 /* Times every candidate launch shape on the caller's device vectors (k = block
@@ -118,6 +119,11 @@ int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int *value);
  * one line per candidate goes to `report` when non-NULL. */
 int hphi_autotune(hphi_engine *e, int k, double *v0, const double *v1, int reps, char *report,
                   size_t report_len);
+/* Host only (no device work): dim of the sector and the number of nonzero
+ * off-diagonal elements of H in it, counted per compiled term by
+ * combinatorics (canonical Spin and Hubbard models); the algorithmic bytes of
+ * one H·v are 16 per off-diagonal element plus the per-state streams. */
+int hphi_model_nnz(const hphi_model_desc *desc, uint64_t *dim, uint64_t *nnz);
 /* HBM bytes the engine holds (tables, diagonal, scratch). */
 uint64_t hphi_engine_device_bytes(const hphi_engine *e);
 
@@ -263,12 +269,22 @@ int hphi_read_restart_vecs(const char *path, uint64_t dim, double *v0, double *v
 int hphi_write_tmcomponents(const char *path, int steps, double norm, const double *alpha, const double *beta);
 int hphi_read_tmcomponents(const char *path, int cap, int *steps, double *norm, double *alpha, double *beta);
 int hphi_write_energy(const char *path, const hphi_phys *p);
+/* The energy file of the block solvers, one block per state: "State %ld",
+ * "  Energy  %.16lf ", "  Doublon  %.16lf ", "  Sz  %.16lf ", blank line
+ * (CalcByLOBPCG.c:744-760, <head>_energy.dat) */
+int hphi_write_energy_states(const char *path, int n, const hphi_phys *p);
 int hphi_write_cisajs(const char *path, int n, const int *idx, const double *val);
 int hphi_write_cisajscktalt(const char *path, int model, int n, const int *idx, const double *val);
 int hphi_write_tpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride);
 /* The same files for canonical TPQ rows (hphi_ctpq), which also write a
  * Flct_rand line at step 0 (CalcByCanonicalTPQ.c:160-200). */
 int hphi_write_ctpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride);
+/* A resumed run's rows (steps step0 .. steps-1, rows[(s - step0)*row_stride + q])
+ * appended to the SS_rand, Norm_rand and Flct_rand files of sample rand_i, as
+ * the reference appends them under ReStart (no header); norm1 is the
+ * global_1st_norm column (the reference prints its process value there). */
+int hphi_append_tpq_files(const char *dir, int rand_i, int step0, int steps, const double *rows, int row_stride,
+                          double norm1);
 /* Lowest `want` distinct eigenvalues of the plain-Lanczos tridiagonal
  * (alpha[0..m), beta[0..m-1)) after Cullum-Willoughby removal of spurious
  * values and merging of ghost copies; returns how many were found. */
@@ -314,6 +330,25 @@ typedef struct {
   uint64_t seed;
 } hphi_ctpq_params;
 int hphi_ctpq(hphi_engine *e, const hphi_ctpq_params *p, double *out, hphi_tpq_result *r);
+/* The TPQ runs from caller-supplied vectors and the reference's restart
+ * (ReStart, CalcByTPQ.c:86-117,272-287; CalcByCanonicalTPQ.c:88-117,282-297).
+ * W0: device array of k vectors of dim complex128, vector c at [c*dim ..).
+ *   step0 = 0: the start vectors, unnormalised as MakeIniVec draws them (their
+ *     norm is step 0's global_1st_norm; the reference's ini vector times that
+ *     norm); rows of steps 0 .. steps-1 as hphi_tpq / hphi_ctpq.
+ *   step0 > 0: resume: the normalised vectors of step step0 - 1 of an earlier
+ *     run, as a restart file tmpvec_set<i> holds them with step field step0;
+ *     they are measured again (no row), and out holds the rows of steps
+ *     step0 .. steps-1 only (out[((s - step0)*k + c)*12 + q]).  The reference
+ *     leaves inv_temp of a resumed canonical run uninitialised; here it
+ *     continues as s * dtau.
+ * W_end (may be NULL): device array as W0 receiving the normalised vectors of
+ * step steps - 1, which a restart file stores with step field `steps`
+ * (hphi_write_vec_file). */
+int hphi_tpq_from(hphi_engine *e, const hphi_tpq_params *p, const double *W0, int step0, double *out,
+                  hphi_tpq_result *r, double *W_end);
+int hphi_ctpq_from(hphi_engine *e, const hphi_ctpq_params *p, const double *W0, int step0, double *out,
+                   hphi_tpq_result *r, double *W_end);
 /* max_j |list_Diagonal[j]| */
 int hphi_diag_maxabs(hphi_engine *e, double *out);
 
@@ -340,6 +375,16 @@ typedef struct {
   double seconds, hv_seconds;
 } hphi_lobpcg_result;
 int hphi_lobpcg(hphi_engine *e, const hphi_lobpcg_params *p, double *history, hphi_lobpcg_result *r);
+/* hphi_lobpcg with the reference's start and result blocks (CalcByLOBPCG.c:
+ * Initialize_wave :153-282, Output_restart :283-319, L_vec :716-724).  X0
+ * (may be NULL): device array of k start vectors of dim complex128, vector c
+ * at [c*dim ..) (e.g. the restart files tmpvec_set<c>_rank_0.dat read with
+ * hphi_read_vec_file), normalised here; Xout (may be NULL): the final
+ * eigenvector block, normalised, the same layout: write it with
+ * hphi_write_vec_file for a restart or an eigenvector file, and measure each
+ * column with hphi_expec_* (phys, src/phys.c:48-140). */
+int hphi_lobpcg_ex(hphi_engine *e, const hphi_lobpcg_params *p, const double *X0, double *Xout, double *history,
+                   hphi_lobpcg_result *r);
 /* Eigen-decomposition of an n x n Hermitian matrix (column-major, interleaved
  * complex): w ascending, V (may be NULL) the eigenvectors as columns; the
  * subspace solver of hphi_lobpcg (replaces LAPACK zheevd, :60,100). */
@@ -373,9 +418,12 @@ int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint64_t seed, 
 int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting);
 int hphi_server_results(hphi_server *s, hphi_job_result *out, int max, int *n);
 double hphi_server_now(const hphi_server *s);
-/* out[0..5]: seconds in admission, in launched work (including the wait), in
+/* out[0..8]: seconds in admission, in launched work (including the wait), in
  * convergence checks; number of non-empty ticks; job-steps advanced;
- * slab-steps advanced (a slab holds up to 4 jobs of one structure) */
+ * slab-steps advanced (a slab holds up to 4 jobs of one structure); the slab
+ * H·v launches: seconds between HIP events around them, their algorithmic
+ * bytes (per slab 8 dim, per running column 56 dim + 16 per off-diagonal
+ * element), their number */
 int hphi_server_stats(const hphi_server *s, double *out);
 
 /* k lowest eigenvalues (ascending) of the symmetric tridiagonal matrix with
