@@ -38,7 +38,7 @@ EXPORTED = [
     "hphi_read_restart_vecs", "hphi_write_tmcomponents", "hphi_read_tmcomponents", "hphi_write_energy",
     "hphi_write_cisajs", "hphi_write_cisajscktalt", "hphi_write_tpq_files", "hphi_lanczos_restart",
     "hphi_ctpq", "hphi_write_ctpq_files", "hphi_mltply_dev_scaled", "hphi_vec_lanczos_resid",
-    "hphi_model_nnz", "hphi_tpq_from", "hphi_ctpq_from",
+    "hphi_mltply_block_dev_noalloc", "hphi_model_nnz", "hphi_tpq_from", "hphi_ctpq_from",
     "hphi_append_tpq_files", "hphi_lobpcg_ex", "hphi_write_energy_states",
 ]
 
@@ -147,6 +147,7 @@ def lib():
         L.hphi_mltply_dev.argtypes = [_vp, _vp, _vp, _f64p]
         L.hphi_mltply_dev_scaled.argtypes = [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, _f64p]
         L.hphi_mltply_block_dev.argtypes = [_vp, C.c_int, _vp, _vp, _f64p]
+        L.hphi_mltply_block_dev_noalloc.argtypes = [_vp, C.c_int, _vp, _vp, _f64p]
         L.hphi_dev_alloc.argtypes = [C.c_size_t, C.POINTER(_vp)]
         L.hphi_dev_free.argtypes = [_vp]
         L.hphi_memcpy_h2d.argtypes = [_vp, _vp, C.c_size_t]
@@ -384,6 +385,12 @@ class Engine:
     def mltply_block_dev(self, k: int, V0: DeviceArray, V1: DeviceArray) -> np.ndarray:
         pr = np.zeros(2 * k)
         check(lib().hphi_mltply_block_dev(self.h, k, V0.ptr, V1.ptr, _p(pr, _f64p)))
+        return pr[0::2] + 1j * pr[1::2]
+
+    def mltply_block_dev_noalloc(self, k: int, V0: DeviceArray, V1: DeviceArray) -> np.ndarray:
+        """The allocation-free block H·v (tables and diagonal must be built)."""
+        pr = np.zeros(2 * k)
+        check(lib().hphi_mltply_block_dev_noalloc(self.h, k, V0.ptr, V1.ptr, _p(pr, _f64p)))
         return pr[0::2] + 1j * pr[1::2]
 
     def hv(self, v: np.ndarray) -> Tuple[np.ndarray, complex]:

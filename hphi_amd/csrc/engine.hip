@@ -890,7 +890,7 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
     e->c2_tpb = 512;
     e->c2_wgs_per_cu = 8;
   }
-  if (hphi_engine_alloc_scratch(e)) {
+  if (hphi_engine_alloc_scratch(e) || hphi_noalloc_setup(e->device)) {
     hphi_engine_destroy(e);
     return -1;
   }
@@ -1277,6 +1277,57 @@ extern "C" int hphi_mltply_block_dev(hphi_engine *e, int k, double *V0, const do
   if (ensure_ready(e)) return -1;
   if (hphi_hv_launch(e, k, V0, V1)) return -1;
   return hphi_fetch_prdct(e, k, prdct);
+}
+
+// ------------------------------------------------- allocation-free H·v ----
+// hphi_mltply_block_dev_noalloc keeps the per-workgroup partial sums of prdct
+// and their reduction in this fixed array rather than in the engine's
+// buffers.  Every device holds its own copy; host code never passes the
+// array's name to a kernel (on the host it is only a stand-in), but the
+// device address hipGetSymbolAddress returned for that device, recorded
+// once per device when an engine is created there.
+#define HV_NOALLOC_PARTIALS (1 << 18)  // k x workgroups of one launch (4 MiB)
+static __device__ __align__(256) double2 scratch[HV_NOALLOC_PARTIALS + 16];
+#define HV_MAX_DEVICES 64
+static double2 *g_scratch_dev[HV_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+int hphi_noalloc_setup(int device) {
+  if (device < 0 || device >= HV_MAX_DEVICES) return hphi_set_error("device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return 0;
+  void *ptr = nullptr;
+  hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (err != hipSuccess || !ptr)
+    return hphi_set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(err));
+  g_scratch_dev[device] = (double2 *)ptr;
+  return 0;
+}
+
+extern "C" int hphi_mltply_block_dev_noalloc(hphi_engine *e, int k, double *V0, const double *V1, double *prdct) {
+  if (!e || !V0 || !V1) return hphi_set_error("hphi_mltply_block_dev_noalloc: NULL argument");
+  if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16)
+    return hphi_set_error("hphi_mltply_block_dev_noalloc: block size k=%d not in {1,2,4,8,16}", k);
+  if (!e->tables_ready || !e->diag_ready)
+    return hphi_set_error("hphi_mltply_block_dev_noalloc: basis tables and diagonal are not built; call hphi_sz "
+                          "and hphi_diagonalcalc first (this entry point allocates nothing)");
+  int cur = -1;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != e->device)
+    return hphi_set_error("hphi_mltply_block_dev_noalloc: the engine lives on device %d, device %d is current",
+                          e->device, cur);
+  double2 *scr = g_scratch_dev[e->device];
+  if (!scr) return hphi_set_error("hphi_mltply_block_dev_noalloc: no scratch address for device %d", e->device);
+  int g = 0;
+  (void)hv_path(e, k, &g);
+  if ((uint64_t)g * (uint64_t)k > HV_NOALLOC_PARTIALS)
+    return hphi_set_error("hphi_mltply_block_dev_noalloc: %d workgroups x k=%d partial sums exceed the fixed "
+                          "scratch of %d; use hphi_mltply_block_dev or a smaller launch shape",
+                          g, k, HV_NOALLOC_PARTIALS);
+  HvArgs A = hphi_hv_args(e, k, V0, V1);
+  A.partial = scr;  // every workgroup of the launch writes its slots, nothing needs clearing
+  if (hv_launch_args(e, k, A, scr + HV_NOALLOC_PARTIALS)) return -1;
+  return fetch_prdct_from(e, scr + HV_NOALLOC_PARTIALS, k, prdct);
 }
 
 static bool is_device_ptr(const void *p) {

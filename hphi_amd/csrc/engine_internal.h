@@ -181,4 +181,6 @@ int hphi_blk_ok(const hphi_engine *e);
 int hphi_blk_cpl_ok(int k, int cpl);
 void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_per_cu);
 int hphi_blk_grid(const hphi_engine *e, int k);
+// records the device address of the allocation-free entry point's scratch array
+int hphi_noalloc_setup(int device);
 int hphi_hv_launch_blk(hphi_engine *e, const HvArgs &A, int k, int *grid);

@@ -150,6 +150,18 @@ int hphi_mltply_dev_scaled(hphi_engine *e, double *v0, const double *v1, double 
 /* Block form: k vectors stored state-major, V[(j*k + v)*2 + {0,1}], device
  * memory; prdct[2k].  k in {1, 2, 4, 8, 16}. */
 int hphi_mltply_block_dev(hphi_engine *e, int k, double *V0, const double *V1, double *prdct);
+/* Allocation-free form of hphi_mltply_block_dev (same kernels, same results,
+ * bit for bit).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree:
+ * the prdct partial sums live in a fixed per-device array of 2^18 + 16
+ * complex128 (4 MiB, held until the process ends), so a launch of g
+ * workgroups needs g*k <= 2^18 (every default and tuned shape on <= 256 CUs
+ * with <= 64 workgroups per CU); larger launches are rejected with an error.
+ * hphi_sz and hphi_diagonalcalc must have run on the engine (no lazy build),
+ * and the engine's device must be current.  Calls that overlap on one device
+ * share that array and race. */
+int hphi_mltply_block_dev_noalloc(hphi_engine *e, int k, double *V0, const double *V1, double *prdct);
 
 /* ---- device memory and vector helpers for host drivers ----------------- */
 int hphi_dev_alloc(size_t bytes, void **ptr);
