@@ -126,9 +126,10 @@ def gpu_leg(args):
             eng.mltply(h0, h1)
         hs = (time.time() - t) / args.host_reps
         host = {"ms_per_call": round(1e3 * hs, 1), "gbs_algorithmic": round(balg / hs / 1e9, 1),
-                "pcie_bytes_per_call": 3 * 16 * dim, "reps": args.host_reps,
-                "note": "hphi_mltply on host arrays: v1 and v0 copied in, v0 copied out on every call; "
-                        "a labelled compatibility rate, never the value"}
+                "pcie_bytes_per_call": 2 * 16 * dim, "reps": args.host_reps,
+                "note": "hphi_mltply on host arrays: v1 in and H v1 out through pinned bounce buffers on "
+                        "worker threads, v0 added on the host, on every call; a labelled compatibility "
+                        "rate, never the value"}
         del h0, h1
     res = {
         "dim": dim, "nnz": nnz, "balg": balg, "ms": ms, "wall_s": wall, "setup_s": t_setup, "host": host,
@@ -531,7 +532,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--lanczos-steps", type=int, default=5)
     ap.add_argument("--cpu-sample", type=float, default=4e8)
-    ap.add_argument("--cpu-reps", type=int, default=1)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--block-k", type=int, default=4, help="also time the block H·v with this many columns")
     ap.add_argument("--pmc-probe", action="store_true", help="H·v + calibration launches for counter runs")
     ap.add_argument("--cfg2", action="store_true", help="plain H·v and Lanczos step on the Hubbard 4x4 workload")

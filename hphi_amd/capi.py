@@ -39,7 +39,8 @@ EXPORTED = [
     "hphi_write_cisajs", "hphi_write_cisajscktalt", "hphi_write_tpq_files", "hphi_lanczos_restart",
     "hphi_ctpq", "hphi_write_ctpq_files", "hphi_mltply_dev_scaled", "hphi_vec_lanczos_resid",
     "hphi_mltply_block_dev_noalloc", "hphi_model_nnz", "hphi_tpq_from", "hphi_ctpq_from",
-    "hphi_append_tpq_files", "hphi_lobpcg_ex", "hphi_write_energy_states",
+    "hphi_append_tpq_files", "hphi_lobpcg_ex", "hphi_write_energy_states", "hphi_lanczos_ritz_residual",
+    "hphi_lanczos_residuals_small",
 ]
 
 

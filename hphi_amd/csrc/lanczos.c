@@ -22,6 +22,7 @@
 #include "hphi_amd.h"
 
 int hphi_set_error(const char *fmt, ...);
+void hphi_tridiag_eigvec(const double *a, const double *b, int n, double lam, double *x);
 
 static double wall(void) {
   struct timespec ts;
@@ -179,8 +180,11 @@ loop:
       r->iterations = stp;
       r->n_found = nc;
       for (int i = 0; i < nc; i++) r->eigenvalues[i] = cur[i];
-      if (hphi_lanczos_converged(prevv, nprev, cur, nc, nwant, eps) || fabs(beta[stp]) < 1e-14 ||
-          (uint64_t)stp >= dim) {
+      /* converged: the values stopped moving and their Ritz residuals are small
+       * (a stalled value can pass the first test while still off) */
+      if ((hphi_lanczos_converged(prevv, nprev, cur, nc, nwant, eps) &&
+           hphi_lanczos_residuals_small(alpha + 1, beta + 1, stp, beta[stp], cur, nwant, eps)) ||
+          fabs(beta[stp]) < 1e-14 || (uint64_t)stp >= dim) {
         r->converged = (nc >= nwant || (uint64_t)stp >= dim || fabs(beta[stp]) < 1e-14);
         break;
       }
@@ -296,76 +300,6 @@ end:
   return rc;
 }
 
-/* Eigenvector of the symmetric tridiagonal (a[0..n), b[0..n-1)) for the
- * eigenvalue lam: inverse iteration, two solves of (T - lam) x = y from y = 1,
- * with an LU factorisation by Gaussian elimination with partial pivoting (row
- * i+1 may swap with the running row i).  Normalised, first non-zero entry > 0. */
-static void tridiag_eigvec(const double *a, const double *b, int n, double lam, double *x) {
-  if (n == 1) {
-    x[0] = 1.0;
-    return;
-  }
-  double *U = (double *)calloc(3 * (size_t)n, sizeof(double)), *L = (double *)calloc(n, sizeof(double));
-  int *sw = (int *)calloc(n, sizeof(int));
-  double *y = (double *)malloc(sizeof(double) * n);
-  double nrm_t = 0.0;
-  for (int i = 0; i < n; i++) nrm_t = fmax(nrm_t, fabs(a[i]) + (i ? fabs(b[i - 1]) : 0) + (i < n - 1 ? fabs(b[i]) : 0));
-  const double tiny = 1e-15 * (nrm_t > 0 ? nrm_t : 1.0);
-  /* running row (columns i, i+1, i+2) */
-  double c = a[0] - lam, d = b[0], f = 0.0;
-  for (int i = 0; i < n - 1; i++) {
-    const double r0 = b[i], r1 = a[i + 1] - lam, r2 = (i + 2 < n) ? b[i + 1] : 0.0;
-    if (fabs(r0) > fabs(c)) { /* row i+1 is the pivot */
-      const double m = c / r0;
-      U[3 * i] = r0, U[3 * i + 1] = r1, U[3 * i + 2] = r2;
-      L[i] = m, sw[i] = 1;
-      c = d - m * r1, d = f - m * r2, f = 0.0;
-    } else {
-      if (fabs(c) < tiny) c = tiny;
-      const double m = r0 / c;
-      U[3 * i] = c, U[3 * i + 1] = d, U[3 * i + 2] = f;
-      L[i] = m, sw[i] = 0;
-      c = r1 - m * d, d = r2 - m * f, f = 0.0;
-    }
-  }
-  if (fabs(c) < tiny) c = tiny;
-  U[3 * (n - 1)] = c;
-  for (int i = 0; i < n; i++) y[i] = 1.0;
-  for (int it = 0; it < 2; it++) {
-    /* forward: the row operations of the factorisation on y */
-    double run = y[0];
-    for (int i = 0; i < n - 1; i++) {
-      if (sw[i]) {
-        const double piv = y[i + 1];
-        y[i] = piv;
-        run = run - L[i] * piv;
-      } else {
-        y[i] = run;
-        run = y[i + 1] - L[i] * run;
-      }
-    }
-    y[n - 1] = run;
-    /* back substitution */
-    for (int i = n - 1; i >= 0; i--) {
-      double t = y[i];
-      if (i + 1 < n) t -= U[3 * i + 1] * x[i + 1];
-      if (i + 2 < n) t -= U[3 * i + 2] * x[i + 2];
-      x[i] = t / U[3 * i];
-    }
-    double s = 0.0;
-    for (int i = 0; i < n; i++) s += x[i] * x[i];
-    s = 1.0 / sqrt(s);
-    for (int i = 0; i < n; i++) y[i] = x[i] *= s;
-  }
-  int k0 = 0;
-  while (k0 < n - 1 && x[k0] == 0.0) k0++;
-  if (x[k0] < 0)
-    for (int i = 0; i < n; i++) x[i] = -x[i];
-  free(U);
-  free(L);
-  free(sw);
-  free(y);
-}
 
 int hphi_lanczos_eigenvector(hphi_engine *e, const hphi_lanczos_params *p, int k, double *vout,
                              hphi_lanczos_result *r) {
@@ -383,7 +317,7 @@ int hphi_lanczos_eigenvector(hphi_engine *e, const hphi_lanczos_params *p, int k
   }
   double *ev = (double *)calloc(k + 1, sizeof(double)), *c = (double *)calloc(m, sizeof(double));
   hphi_tridiag_lowest(alpha + 1, beta + 1, m, k + 1, ev);
-  tridiag_eigvec(alpha + 1, beta + 1, m, ev[k], c);
+  hphi_tridiag_eigvec(alpha + 1, beta + 1, m, ev[k], c);
   r->eigenvalues[0] = ev[k];
   double t0 = wall(), thv = 0.0, pr[2], nrm, b2;
   double *ra = NULL, *wb = NULL;

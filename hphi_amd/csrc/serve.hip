@@ -1210,7 +1210,12 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
           nc = hphi_lanczos_good_eigenvalues(j->alpha.data() + 1, j->beta.data() + 1, m, s->p.n_eigen, cur);
       }
       const int want = (int)std::min<uint64_t>((uint64_t)s->p.n_eigen, j->h.dim);
-      const bool conv = hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps) || (exhausted && nc >= 1);
+      // converged: the values stopped moving and their Ritz residuals are small
+      // (T_{m-1} with beta_{m-1}: beta_m comes with the next step)
+      bool conv = (exhausted && nc >= 1);
+      if (!conv && hphi_lanczos_converged(j->prev, j->nprev, cur, nc, want, s->p.eps))
+        conv = m < 2 || hphi_lanczos_residuals_small(j->alpha.data() + 1, j->beta.data() + 1, m - 1, j->beta[m - 1],
+                                                     cur, want, s->p.eps);
       if (conv || m >= s->p.max_iter) {
         retire(s, j, now, conv ? 1 : 0, cur, nc);
         delete j;

@@ -209,7 +209,8 @@ typedef struct {
   uint64_t seed;     /* initial vector seed */
   int check_every;   /* tridiagonal solve interval (1 = every step, as the reference) */
   int mode;          /* 0: the reference's test on E[target]; 1: the lowest n_eigen distinct
-                        eigenvalues (Cullum-Willoughby filtered) change by < eps relative */
+                        eigenvalues (Cullum-Willoughby filtered) change by < eps relative and
+                        their Ritz residuals are <= 0.1 sqrt(eps) max(1, |E|) */
 } hphi_lanczos_params;
 
 typedef struct {
@@ -302,6 +303,16 @@ int hphi_append_tpq_files(const char *dir, int rand_i, int step0, int steps, con
  * values and merging of ghost copies; returns how many were found. */
 int hphi_lanczos_good_eigenvalues(const double *alpha, const double *beta, int m, int want, double *out);
 int hphi_lanczos_converged(const double *prev, int nprev, const double *cur, int ncur, int want, double tol);
+/* Ritz residual beta_m |y_m| of the eigenvalue theta of T_m (alpha[0..m),
+ * beta[0..m-1); y its unit eigenvector, by inverse iteration): |H z - theta z|
+ * for the Ritz vector z (host only) */
+int hphi_lanczos_ritz_residual(const double *alpha, const double *beta, int m, double beta_m, double theta,
+                               double *res);
+/* 1 when every one of the `want` values cur[] has a Ritz residual <= 0.1
+ * sqrt(tol) max(1, |theta|): the second half of the distinct mode's
+ * convergence test (the first: hphi_lanczos_converged) */
+int hphi_lanczos_residuals_small(const double *alpha, const double *beta, int m, double beta_m, const double *cur,
+                                 int want, double tol);
 /* Microcanonical TPQ (CalcByTPQ, src/CalcByTPQ.c:52-294) for k samples at once.
  * Sample c starts from a random vector of seed + c (init_kind 0: uniform
  * complex, -1: complex Gaussian); l = large_value, or, when large_value <= 0,

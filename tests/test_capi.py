@@ -243,3 +243,50 @@ def test_good_eigenvalues_survive_many_ghost_copies(capi, oracle_lib):
         k = fn(A.ctypes.data_as(dp), B.ctypes.data_as(dp), m, 4, out.ctypes.data_as(dp))
         assert k == 4, (m, k, out[:k])
         assert np.allclose(out[:4], distinct[:4], atol=1e-8, rtol=0), (m, out[:4], distinct[:4])
+
+
+def test_ritz_residual_equals_the_ritz_vectors_residual(capi):
+    """hphi_lanczos_ritz_residual (beta_m |y_m| by inverse iteration) against
+    |H z - theta z| of the explicit Ritz vector z = V_m y of a plain Lanczos
+    run on a dense random Hermitian matrix, before and after convergence; and
+    hphi_lanczos_residuals_small follows it."""
+    lib = capi.lib()
+    lib.hphi_lanczos_ritz_residual.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_double,
+                                               C.c_double, C.POINTER(C.c_double)]
+    lib.hphi_lanczos_residuals_small.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_double,
+                                                 C.POINTER(C.c_double), C.c_int, C.c_double]
+    rng = np.random.default_rng(7)
+    n = 300
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = (X + X.conj().T) / 2
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    v /= np.linalg.norm(v)
+    V, a, b = [v], [], []
+    w = H @ v
+    for m in range(1, 61):
+        al = np.vdot(V[-1], w).real
+        w = w - al * V[-1] - (b[-1] * V[-2] if b else 0)
+        a.append(al)
+        b.append(np.linalg.norm(w))
+        V.append(w / b[-1])
+        w = H @ V[-1]
+        if m in (10, 30, 60):
+            T = np.diag(a) + np.diag(b[:-1], 1) + np.diag(b[:-1], -1)
+            th, Y = np.linalg.eigh(T)
+            Vm = np.array(V[:m]).T
+            aa, bb = np.array(a), np.array(b[:-1] if m > 1 else [0.0])
+            for i in (0, 1, m // 2):
+                z = Vm @ Y[:, i]
+                true = np.linalg.norm(H @ z - th[i] * z)
+                got = C.c_double()
+                assert lib.hphi_lanczos_ritz_residual(aa.ctypes.data_as(C.POINTER(C.c_double)),
+                                                      bb.ctypes.data_as(C.POINTER(C.c_double)), m, b[-1], th[i],
+                                                      C.byref(got)) == 0
+                assert abs(got.value - true) <= 1e-8 * max(true, 1.0) + 1e-10, (m, i, got.value, true)
+                assert abs(got.value - b[-1] * abs(Y[-1, i])) <= 1e-9 * max(got.value, 1e-3)
+            cur = th[:2].copy()
+            small = lib.hphi_lanczos_residuals_small(aa.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     bb.ctypes.data_as(C.POINTER(C.c_double)), m, b[-1],
+                                                     cur.ctypes.data_as(C.POINTER(C.c_double)), 2, 1e-12)
+            r = [b[-1] * abs(Y[-1, i]) for i in (0, 1)]
+            assert small == int(all(x <= 1e-7 * max(1.0, abs(t)) for x, t in zip(r, cur))), (m, r)
