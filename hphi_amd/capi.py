@@ -40,7 +40,7 @@ EXPORTED = [
     "hphi_ctpq", "hphi_write_ctpq_files", "hphi_mltply_dev_scaled", "hphi_vec_lanczos_resid",
     "hphi_mltply_block_dev_noalloc", "hphi_model_nnz", "hphi_tpq_from", "hphi_ctpq_from",
     "hphi_append_tpq_files", "hphi_lobpcg_ex", "hphi_write_energy_states", "hphi_lanczos_ritz_residual",
-    "hphi_lanczos_residuals_small",
+    "hphi_lanczos_residuals_small", "hphi_expec_spin_nbody", "hphi_write_nbody_green",
 ]
 
 
@@ -201,6 +201,8 @@ def lib():
         L.hphi_write_energy_states.argtypes = [C.c_char_p, C.c_int, C.POINTER(Phys)]
         L.hphi_tpq_from.argtypes = [_vp, C.POINTER(TpqParams), _vp, C.c_int, _f64p, C.POINTER(TpqResult), _vp]
         L.hphi_ctpq_from.argtypes = [_vp, C.POINTER(CtpqParams), _vp, C.c_int, _f64p, C.POINTER(TpqResult), _vp]
+        L.hphi_expec_spin_nbody.argtypes = [_vp, _vp, C.c_int, C.c_int, _i32p, _f64p, _i32p]
+        L.hphi_write_nbody_green.argtypes = [C.c_char_p, C.c_int, C.c_int, _i32p, _f64p, _i32p]
         L.hphi_lobpcg_ex.argtypes = [_vp, C.POINTER(LobpcgParams), _vp, _vp, _f64p, C.POINTER(LobpcgResult)]
         L.hphi_write_ctpq_files.argtypes = [C.c_char_p, C.c_int, C.c_int, _f64p, C.c_int]
         L.hphi_lanczos_restart.argtypes = [_vp, C.POINTER(LanczosParams), C.c_char_p, C.c_char_p, C.c_char_p,
@@ -520,6 +522,15 @@ class Engine:
         check(lib().hphi_lanczos_eigenvector(self.h, C.byref(p), k, v.ptr, C.byref(r)))
         return v, r
 
+    def expec_spin_nbody(self, v: "DeviceArray", nbody: int, idx):
+        """SpinGC three/four/six-body Green's functions: (values, ok flags)."""
+        q = np.ascontiguousarray(np.asarray(idx, np.int32).reshape(-1, 4 * nbody))
+        out = np.zeros(2 * q.shape[0])
+        ok = np.zeros(q.shape[0], np.int32)
+        check(lib().hphi_expec_spin_nbody(self.h, v.ptr, nbody, q.shape[0], _p(q, _i32p), _p(out, _f64p),
+                                          _p(ok, _i32p)))
+        return out[0::2] + 1j * out[1::2], ok
+
     def expec_energy_flct(self, v: "DeviceArray") -> Dict[str, float]:
         """energy, var, doublon, doublon2, num, num2, sz, sz2 of the device vector v."""
         out = Phys()
@@ -701,3 +712,10 @@ def write_energy_states(path: str, phys) -> None:
     """The block solvers' energy file from a list of phys dicts."""
     arr = (Phys * len(phys))(*[Phys(**{k: p.get(k, 0.0) for k, _ in Phys._fields_}) for p in phys])
     check(lib().hphi_write_energy_states(path.encode(), len(phys), arr))
+
+
+def write_nbody_green(path: str, nbody: int, idx, vals, ok) -> None:
+    q = np.ascontiguousarray(np.asarray(idx, np.int32).reshape(-1, 4 * nbody))
+    v = np.ascontiguousarray(np.column_stack([np.real(vals), np.imag(vals)]).reshape(-1), np.float64)
+    k = np.ascontiguousarray(np.asarray(ok, np.int32))
+    check(lib().hphi_write_nbody_green(path.encode(), nbody, q.shape[0], _p(q, _i32p), _p(v, _f64p), _p(k, _i32p)))

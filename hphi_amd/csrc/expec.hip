@@ -15,6 +15,7 @@
 // reproducible run to run.
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "engine_internal.h"
@@ -356,6 +357,88 @@ extern "C" int hphi_expec_cisajscktalt(hphi_engine *e, const double *v, int n, c
       if (!keep) continue;
     }
     hphi_compile_spin_pair(A, s1, s2, B, s3, s4, 1.0, 0.0, sl);
+  }
+  return run_corr(e, v, slots, out);
+}
+
+// SpinGC three-, four- and six-body Green's functions
+// (expec_Threebody_SpinGCHalf / expec_Fourbody_SpinGCHalf /
+// expec_Sixbody_SpinGCHalf, src/expec_cisajscktaltdc.c:901-1420).  An entry
+// holds nbody pairs (site, sigma)(site, sigma); pair p is O_p = |s><s'| on its
+// site when both sites agree.  The value is <v| O_1 O_2 O_3 ... O_nbody |v>:
+//  * pairs 3 .. nbody are applied to v right to left (mltplyHalfSpinGC_mini,
+//    src/mltplySpin.c:416-490); one on two different sites leaves zero;
+//  * the first two follow Rearray_Interactions (:274-381): (i,i)(k,k) as given,
+//    the cross form (i,k)(k,i) as -|s1><s4|_i |s3><s2|_k; any other first two
+//    pairs are not evaluated (ok = 0; the reference prints the first eight
+//    indices and zeros).
+// The product of on-site projectors and flips is one gather term: per site the
+// composed operator |a><b| (|a><b| after |c><d| is delta(b, c) |a><d|), the
+// target x has bits a, the source is x with bits b.
+extern "C" int hphi_expec_spin_nbody(hphi_engine *e, const double *v, int nbody, int n, const int *idx, double *out,
+                                     int *ok) {
+  if (!e || !v || (n > 0 && (!idx || !out || !ok))) return hphi_set_error("hphi_expec_spin_nbody: NULL argument");
+  if (nbody != 3 && nbody != 4 && nbody != 6) return hphi_set_error("hphi_expec_spin_nbody: nbody must be 3, 4 or 6");
+  if (e->model != HPHI_MODEL_SPINGC)
+    return hphi_set_error("hphi_expec_spin_nbody: the reference defines these for SpinGC (spin 1/2) only");
+  const int N = e->nsite;
+  std::vector<std::vector<GTerm>> slots(n);
+  for (int t = 0; t < n; t++) {
+    const int *q = idx + 4 * nbody * t;
+    for (int u = 0; u < 4 * nbody; u += 2)
+      if (q[u] < 0 || q[u] >= N || q[u + 1] < 0 || q[u + 1] > 1)
+        return hphi_set_error("%d-body Green's function entry %d out of range", nbody, t);
+    ok[t] = 1;
+    // the first two pairs after Rearray_Interactions: (site, s, s') x 2 and the sign
+    int A, a1, a2, B, b1, b2;
+    double sign = 1.0;
+    if (q[0] == q[2] && q[4] == q[6]) {
+      A = q[0], a1 = q[1], a2 = q[3], B = q[4], b1 = q[5], b2 = q[7];
+    } else if (q[0] == q[6] && q[4] == q[2]) {
+      A = q[0], a1 = q[1], a2 = q[7], B = q[4], b1 = q[5], b2 = q[3];
+      sign = -1.0;
+    } else {
+      ok[t] = 0;
+      continue;
+    }
+    if (A > B) {  // the pair with the smaller site first
+      std::swap(A, B);
+      std::swap(a1, b1);
+      std::swap(a2, b2);
+    }
+    // compose, in the order the operators act on the ket
+    int ca[64], cb[64];
+    for (int s = 0; s < N; s++) ca[s] = cb[s] = -1;
+    bool zero = false;
+    auto apply = [&](int site, int s_to, int s_from) {
+      if (ca[site] < 0) {
+        ca[site] = s_to;
+        cb[site] = s_from;
+      } else if (s_from == ca[site]) {
+        ca[site] = s_to;
+      } else {
+        zero = true;
+      }
+    };
+    for (int p = nbody - 1; p >= 2 && !zero; p--) {
+      const int *r = q + 4 * p;
+      if (r[0] != r[2]) zero = true;  // mltplyHalfSpinGC_mini adds nothing for two sites
+      else apply(r[0], r[1], r[3]);
+    }
+    if (!zero) apply(B, b1, b2);
+    if (!zero) apply(A, a1, a2);
+    if (zero) continue;
+    GTerm g{};
+    g.kind = P_EQ;
+    for (int s = 0; s < N; s++) {
+      if (ca[s] < 0) continue;
+      const uint64_t bit = 1ull << s;
+      g.tmask |= bit;
+      if (ca[s]) g.tval |= bit;
+      if (ca[s] != cb[s]) g.flip |= bit;
+    }
+    g.cre = sign;
+    slots[t].push_back(g);
   }
   return run_corr(e, v, slots, out);
 }

@@ -277,3 +277,26 @@ int hphi_append_tpq_files(const char *dir, int rand_i, int step0, int steps, con
   const int e1 = fclose(fs), e2 = fclose(fn), e3 = fclose(ff);
   return (e1 || e2 || e3) ? hphi_set_error("write failed in %s", dir) : 0;
 }
+
+/* SpinGC three-, four- and six-body Green's function files
+ * (expec_cisajscktaltdc.c:1028-1085,1243-1273,1398-1420): every index of an
+ * evaluated entry, with a double space before the 13th and the 17th, or the
+ * first eight indices and zeros for an entry the reference does not evaluate */
+int hphi_write_nbody_green(const char *path, int nbody, int n, const int *idx, const double *val, const int *ok) {
+  if (!path || (n > 0 && (!idx || !val || !ok))) return hphi_set_error("hphi_write_nbody_green: NULL argument");
+  if (nbody != 3 && nbody != 4 && nbody != 6) return hphi_set_error("hphi_write_nbody_green: nbody must be 3, 4 or 6");
+  FILE *f = open_or_fail(path, "w");
+  if (!f) return -1;
+  const int w = 4 * nbody;
+  for (int t = 0; t < n; t++) {
+    const int *q = idx + (size_t)w * t;
+    if (!ok[t]) {
+      for (int u = 0; u < 8; u++) fprintf(f, " %4ld", (long)q[u]);
+      fprintf(f, " %.10lf %.10lf \n", 0.0, 0.0);
+      continue;
+    }
+    for (int u = 0; u < w; u++) fprintf(f, (u == 12 || u == 16) ? "  %4ld" : " %4ld", (long)q[u]);
+    fprintf(f, " %.10lf %.10lf \n", val[2 * t], val[2 * t + 1]);
+  }
+  return fclose(f) == 0 ? 0 : hphi_set_error("write failed: %s", path);
+}

@@ -82,6 +82,10 @@ class ModelSpec:
     # TwoBodyG (i, s1, j, s2, k, s3, l, s4) for <c+_{i s1} c_{j s2} c+_{k s3} c_{l s4}>
     green1: List[Tuple[int, int, int, int]] = field(default_factory=list)
     green2: List[Tuple[int, int, int, int, int, int, int, int]] = field(default_factory=list)
+    # SpinGC ThreeBodyG / FourBodyG / SixBodyG: 3, 4 or 6 pairs (i, s1, j, s2) each
+    green3: List[Tuple[int, ...]] = field(default_factory=list)
+    green4: List[Tuple[int, ...]] = field(default_factory=list)
+    green6: List[Tuple[int, ...]] = field(default_factory=list)
     calc_type: int = CALC_LANCZOS
     params: Dict[str, float] = field(default_factory=dict)
 
@@ -418,6 +422,12 @@ def write_expert(spec: ModelSpec, directory: str, calc_type: Optional[int] = Non
         txt = _header("NCisAjsCktAltDC", len(spec.green2), "======== Green functions for Sq AND Nq ======")
         txt += "".join(" ".join(str(int(v)) for v in e) + "\n" for e in spec.green2)
         put("TwoBodyG", "greentwo.def", txt)
+    for kw, fname, lst in (("ThreeBodyG", "greenthree.def", spec.green3), ("FourBodyG", "greenfour.def", spec.green4),
+                           ("SixBodyG", "greensix.def", spec.green6)):
+        if lst:
+            txt = _header(f"N{kw}", len(lst), f"======== {kw} ======")
+            txt += "".join(" ".join(str(int(v)) for v in e) + "\n" for e in lst)
+            put(kw, fname, txt)
     nl = os.path.join(directory, "namelist.def")
     with open(nl, "w") as fh:
         for kw, fname in files.items():

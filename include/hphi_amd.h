@@ -257,6 +257,14 @@ int hphi_expec_cisajs(hphi_engine *e, const double *v, int n, const int *idx, do
  * (expec_cisajscktaltdc, src/expec_cisajscktaltdc.c:87-1647) for idx[n][8];
  * spin entries need i == j and k == l. */
 int hphi_expec_cisajscktalt(hphi_engine *e, const double *v, int n, const int *idx, double *out);
+/* SpinGC three-, four- and six-body Green's functions (ThreeBodyG, FourBodyG,
+ * SixBodyG; expec_Threebody/Fourbody/Sixbody_SpinGCHalf,
+ * src/expec_cisajscktaltdc.c:901-1420) for idx[n][4*nbody] = nbody pairs
+ * (i, s1, j, s2): <v| O_1 ... O_nbody |v> with O = |s1><s2| on site i when
+ * i == j (a later pair on two sites gives 0); the first two pairs follow
+ * Rearray_Interactions (the cross form (i,k)(k,i) counts with sign -1, other
+ * forms are not evaluated: ok[t] = 0).  nbody in {3, 4, 6}. */
+int hphi_expec_spin_nbody(hphi_engine *e, const double *v, int nbody, int n, const int *idx, double *out, int *ok);
 
 /* ---- the reference's output and restart files (host memory; no device work) ----
  * Vectors are 0-based complex128 host arrays of dim elements; files hold the
@@ -288,6 +296,10 @@ int hphi_write_energy(const char *path, const hphi_phys *p);
 int hphi_write_energy_states(const char *path, int n, const hphi_phys *p);
 int hphi_write_cisajs(const char *path, int n, const int *idx, const double *val);
 int hphi_write_cisajscktalt(const char *path, int model, int n, const int *idx, const double *val);
+/* <head>_ThreeBody.dat / _FourBody.dat / _SixBody.dat lines for the entries
+ * of hphi_expec_spin_nbody (the reference's spacing; ok[t] = 0: eight indices
+ * and zeros) */
+int hphi_write_nbody_green(const char *path, int nbody, int n, const int *idx, const double *val, const int *ok);
 int hphi_write_tpq_files(const char *dir, int rand_i, int steps, const double *rows, int row_stride);
 /* The same files for canonical TPQ rows (hphi_ctpq), which also write a
  * Flct_rand line at step 0 (CalcByCanonicalTPQ.c:160-200). */

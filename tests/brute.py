@@ -224,3 +224,31 @@ def phys(spec, v, H=None) -> dict:
                    num2=np.sum(w * (up + dn) ** 2), Sz=0.5 * np.sum(w * (up - dn)),
                    Sz2=0.25 * np.sum(w * (up - dn) ** 2))
     return out
+
+
+def green_nbody(spec, v, entry):
+    """<v|O_1 O_2 ... O_n|v> for a SpinGC three/four/six-body entry (n pairs
+    (i, s1, j, s2)) from the definitions the reference states in
+    expec_cisajscktaltdc.c: O = (|s1><s2|)_i for i == j, a later pair on two
+    sites gives 0, the first two pairs as Rearray_Interactions takes them
+    ((i,i)(k,k), or (i,k)(k,i) as -(|s1><s4|)_i (|s3><s2|)_k).  None when the
+    reference does not evaluate the entry."""
+    q = list(entry)
+    n = len(q) // 4
+    if q[0] == q[2] and q[4] == q[6]:
+        ops, sign = [(q[0], q[1], q[3]), (q[4], q[5], q[7])], 1.0
+    elif q[0] == q[6] and q[4] == q[2]:
+        ops, sign = [(q[0], q[1], q[7]), (q[4], q[5], q[3])], -1.0
+    else:
+        return None
+    for p in range(2, n):
+        i, s1, j, s2 = q[4 * p:4 * p + 4]
+        if i != j:
+            return 0.0
+        ops.append((i, s1, s2))
+    acc = 0.0
+    for x in range(len(v)):
+        y = _spin_ops(x, ops)
+        if y is not None:
+            acc += np.conj(v[y]) * v[x]
+    return sign * acc

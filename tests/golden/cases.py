@@ -168,6 +168,28 @@ def green_cases():
     return out
 
 
+def nbody_green_cases():
+    """A SpinGC model with ThreeBodyG / FourBodyG / SixBodyG entries covering
+    the branches of expec_Threebody/Fourbody/Sixbody_SpinGCHalf: on-site
+    projectors and flips on distinct and on repeated sites, the cross form of
+    the first two pairs (sign -1), a first pair the reference does not
+    evaluate, and a later pair on two sites (zero)."""
+    gc = spingc_interall()
+    gc.green3 = [(0, 0, 0, 0, 3, 1, 3, 1, 5, 0, 5, 0), (1, 0, 1, 1, 4, 1, 4, 0, 6, 1, 6, 1),
+                 (2, 1, 2, 0, 2, 0, 2, 1, 7, 0, 7, 1), (5, 0, 5, 1, 0, 1, 0, 0, 5, 1, 5, 0),
+                 (1, 0, 4, 1, 4, 0, 1, 1, 3, 1, 3, 0), (0, 0, 2, 0, 3, 0, 3, 0, 4, 1, 4, 1),
+                 (0, 1, 0, 1, 6, 0, 6, 1, 2, 0, 3, 0), (3, 1, 3, 0, 3, 0, 3, 1, 3, 1, 3, 0)]
+    gc.green4 = [(0, 0, 0, 0, 1, 1, 1, 1, 2, 0, 2, 0, 3, 1, 3, 1), (4, 1, 4, 0, 5, 0, 5, 1, 6, 1, 6, 0, 7, 0, 7, 1),
+                 (2, 0, 2, 1, 6, 1, 6, 0, 2, 1, 2, 0, 6, 0, 6, 1), (7, 1, 7, 1, 1, 0, 1, 1, 1, 1, 1, 0, 7, 0, 7, 0),
+                 (3, 0, 5, 1, 5, 0, 3, 1, 0, 1, 0, 0, 4, 0, 4, 1), (1, 0, 1, 0, 2, 0, 3, 0, 4, 1, 4, 1, 5, 0, 5, 0)]
+    gc.green6 = [(0, 0, 0, 1, 1, 1, 1, 0, 2, 0, 2, 1, 3, 1, 3, 0, 4, 0, 4, 1, 5, 1, 5, 0),
+                 (0, 1, 0, 1, 2, 0, 2, 0, 4, 1, 4, 1, 6, 0, 6, 0, 1, 1, 1, 1, 3, 0, 3, 0),
+                 (5, 0, 5, 1, 5, 1, 5, 0, 6, 0, 6, 1, 6, 1, 6, 0, 7, 1, 7, 0, 7, 0, 7, 1),
+                 (2, 0, 2, 0, 1, 1, 1, 1, 0, 0, 0, 0, 3, 0, 4, 0, 5, 1, 5, 1, 6, 0, 6, 0)]
+    gc.name = "green_nbody_" + gc.name
+    return {gc.name: gc}
+
+
 def tpq_file_cases():
     """Small models whose TPQ output files (SS_rand, Norm_rand, Flct_rand) the
     reference writes for the file-format fixtures: one per observable kind."""

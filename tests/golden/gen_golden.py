@@ -35,7 +35,11 @@ synthetic vectors of seeds 0..15 of the triangular 13x2 workload (harness mode
 `sums_seeds`, one reference mltply per seed), i.e. the columns of the k = 16
 block its mTPQ runs.
 
-usage: python tests/golden/gen_golden.py [--large | --large-block | --two-body | --green | --files]
+SpinGC three-, four- and six-body Green's functions (pass --nbody): the same
+green mode on nbody_green_cases(); the ThreeBody / FourBody / SixBody lines
+and the files the reference wrote (tests/golden/files/<case>/).
+
+usage: python tests/golden/gen_golden.py [--large | --large-block | --two-body | --green | --nbody | --files]
 """
 
 from __future__ import annotations
@@ -55,7 +59,7 @@ sys.path.insert(0, HERE)
 
 from hphi_amd import lattice as L  # noqa: E402
 from oracle import oracle as O  # noqa: E402
-from cases import SEED, ctpq_file_cases, golden_cases, green_cases, tpq_file_cases, two_body_cases  # noqa: E402,E501
+from cases import SEED, ctpq_file_cases, golden_cases, green_cases, nbody_green_cases, tpq_file_cases, two_body_cases  # noqa: E402,E501
 import shutil  # noqa: E402
 
 FULL_MAX = 5000
@@ -116,8 +120,16 @@ def run_green(name, spec, work):
             return []
         return [[int(x) for x in p[:nidx]] + [float(p[nidx]), float(p[nidx + 1])]
                 for p in (ln.split() for ln in open(path)) if p]
-    return {"dim": spec.dim, "seed": SEED, "phys": phys,
-            "green1": lines("zvo_cisajs.dat", 4), "green2": lines("zvo_cisajscktalt.dat", 8)}
+    rec = {"dim": spec.dim, "seed": SEED, "phys": phys,
+           "green1": lines("zvo_cisajs.dat", 4), "green2": lines("zvo_cisajscktalt.dat", 8)}
+    for nb, fname in ((3, "zvo_ThreeBody.dat"), (4, "zvo_FourBody.dat"), (6, "zvo_SixBody.dat")):
+        if getattr(spec, f"green{nb}"):
+            # a line holds every index and the value, or eight indices and zeros
+            path = os.path.join(d, "output", fname)
+            rec[f"green{nb}"] = [[int(x) for x in p[:-2]] + [float(p[-2]), float(p[-1])]
+                                 for p in (ln.split() for ln in open(path)) if p]
+            keep_files(name, os.path.join(d, "output"), [fname])
+    return rec
 
 
 FILES = os.path.join(HERE, "files")
@@ -224,6 +236,10 @@ def main():
         elif "--files" in sys.argv:
             gold["files"] = run_files(work)
             print("files done", flush=True)
+        elif "--nbody" in sys.argv:
+            for name, spec in nbody_green_cases().items():
+                gold.setdefault("cases_green", {})[name] = run_green(name, spec, work)
+                print(name, "done", flush=True)
         elif "--green" in sys.argv:
             for name, spec in green_cases().items():
                 gold.setdefault("cases_green", {})[name] = run_green(name, spec, work)
