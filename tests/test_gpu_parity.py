@@ -529,3 +529,33 @@ def test_autotuner_applies_a_correct_configuration(A, oracle_lib, name, k):
         assert rel_inf(out[:, c], ohv) <= HV_TOL
         assert prdct_ok(pr[c], opr, vc, ohv)
     eng.close()
+
+
+def test_host_pointer_mltply_equals_device_mltply_across_chunks(A):
+    """hphi_mltply on the reference's 1-indexed host arrays (pinned bounce
+    buffers, up to 16 workers with 16 MiB chunks, v0 added on the host) equals
+    the device-resident call bit for bit: J1-J2 7x4 Sz=0 (dim 40,116,600, 642 MB
+    per vector, several chunks per worker), v0 non-zero so the accumulation
+    counts, and index 0 untouched."""
+    spec = L.j1j2_square(7, 4, 1.0, 0.35)
+    print(f"[gpu] host-pointer mltply {spec.name} dim {spec.dim}", flush=True)
+    eng = A.Engine.from_spec(spec)
+    dim = eng.dim
+    d1 = A.DeviceArray(dim)
+    d0 = A.DeviceArray(dim)
+    eng.fill_random(d1, 1, 5)
+    eng.fill_random(d0, 1, 6)
+    h1 = np.zeros(dim + 1, np.complex128)
+    h0 = np.zeros(dim + 1, np.complex128)
+    h1[1:] = d1.download()
+    h0[1:] = d0.download()
+    h0[0] = 7.0 + 3.0j  # the reference's arrays are 1-indexed: element 0 is not part of the vector
+    pd = eng.mltply_dev(d0, d1)
+    ph = eng.mltply(h0, h1)
+    ref = d0.download()
+    assert h0[0] == 7.0 + 3.0j
+    assert np.array_equal(h0[1:], ref)
+    assert ph == pd
+    d0.free()
+    d1.free()
+    eng.close()
