@@ -374,6 +374,15 @@ def _probe_hv_ms(directory: str):
     return None
 
 
+# The port against the reference's own mltply (oracle/_ref, compiled from the
+# reference's sources), both on the whole 8x4 J1-J2 sector with the same 8
+# threads of the build container (Intel Xeon @ 2.70GHz), medians of 3 calls:
+# reference 90.71 s (7.43 GB/s algorithmic), port 54.12 s (12.45 GB/s)
+PORT_OVER_REFERENCE = {"speed_ratio": 1.676, "reference_gbs": 7.43, "port_gbs": 12.45, "threads": 8,
+                       "cpu": "Intel(R) Xeon(R) Processor @ 2.70GHz (build container)",
+                       "workload": "whole J1-J2 8x4 Sz=0 sector, one mltply, median of 3"}
+
+
 def cpu_leg(args):
     import numpy as np
 
@@ -401,12 +410,19 @@ def cpu_leg(args):
         orc.mltply(v0, v1, jlo, jhi)
         times.append(time.time() - t)
     t = statistics.median(times)
+    cpu = "unknown CPU"
+    try:
+        cpu = next(ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": b_alg(S, nnz) / t / 1e9, "unit": "GB/s", "cores": O.lib().or_num_threads(),
             "kind": "port",
             "sample": f"{S} contiguous source states (j={jlo}..{jhi}) of the same 8x4 J1-J2 sector, "
                       f"all 128 Exchange sweeps + diagonal, full-length tables and vectors; median of "
-                      f"{args.cpu_reps} runs, {t:.2f} s each; CPU oracle (C+OpenMP restatement of the "
-                      f"reference's per-term scatter)",
+                      f"{args.cpu_reps} runs, {t:.2f} s each ({', '.join(f'{x:.2f}' for x in times)}); CPU "
+                      f"oracle (C+OpenMP restatement of the reference's per-term scatter) on "
+                      f"{O.lib().or_num_threads()} threads of {cpu}",
+            "port_over_reference": PORT_OVER_REFERENCE,
             "seconds": t}
 
 
@@ -656,7 +672,7 @@ def main():
             "note": "first requests of the seed-3 stream; bench.py --serve --requests 10000 runs the whole stream"}
     if not args.no_cpu:
         c = cpu_leg(args)
-        line["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        line["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample", "port_over_reference")}
         line["cpu_baseline"]["value"] = round(c["value"], 3)
     else:
         line["cpu_baseline"] = None
