@@ -471,6 +471,20 @@ static void classify_and_upload_terms(hphi_engine *e) {
     else if (hi) high.push_back(t);
     else mixed.push_back(t);
   }
+  // A mixed term gathers from the block whose high half is x's with the
+  // term's high flip bits changed; terms that share those bits read the same
+  // target block, and together they fill its cache lines (on the 8x4 J1-J2
+  // sector one term alone touches 1.62 bytes of lines per gathered byte, the
+  // three that share a high site 1.05 together).  Consecutive, they reuse
+  // those lines in L2 instead of re-reading them from HBM some tens of
+  // microseconds later: 655 -> 494 GB fetched per H·v there.
+  std::stable_sort(mixed.begin(), mixed.end(), [&](const GTerm &a, const GTerm &b) {
+    return (a.flip & ~e->irght) < (b.flip & ~e->irght);
+  });
+  // The same for the low group (and every grand-canonical term): a 64-state
+  // wave gathers from the region its target bits above the lowest six select,
+  // so terms that agree there read the same lines back to back.
+  std::stable_sort(low.begin(), low.end(), [](const GTerm &a, const GTerm &b) { return (a.flip >> 6) < (b.flip >> 6); });
   auto pad = [](std::vector<GTerm> &v) {
     GTerm z{};
     z.kind = P_EQ;
