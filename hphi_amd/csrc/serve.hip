@@ -67,8 +67,12 @@
   } while (0)
 
 #define SV_TPB 256  // one tile = SV_TPB states of one slab
+#ifndef SL_K
 #define SL_K 4      // columns (jobs) per slab
+#endif
+#ifndef SL_U
 #define SL_U 2      // terms per gather batch (SL_U * SL_K gathers in flight)
+#endif
 #define SL_SMAX 16  // Lanczos steps per tick at most
 #define SL_TPW 1    // tiles per workgroup (a chunk): 1, 2, 4, 8 measured, 1 fastest at every job size
 
@@ -172,28 +176,33 @@ __device__ __forceinline__ double sl_sign(uint32_t sgn, uint32_t x) {
   return (sgn != 0u && (__popc(x & sgn) & 1)) ? -1.0 : 1.0;
 }
 
-// acc_c += coef[t][c] * s * g_c for the running columns
-__device__ __forceinline__ void sl_mac(double *ar, double *ai, CD2 cf, double s, const double2 *g, int mask,
-                                       int realc) {
+// acc_c += coef[t][c] * s * g_c for every column of the slab.  Columns without
+// a running job are computed too (their results are never stored), so the
+// term loop carries no per-column tests.  REAL: the couplings of every
+// running column are real.  The complex form applies the imaginary part in
+// separate fused steps after the real one, so a real column gets the same
+// bits on either path: fma(-0 * ..) leaves the accumulator unchanged.
+template <bool REAL>
+__device__ __forceinline__ void sl_mac(double *ar, double *ai, CD2 cf, double s, const double2 *g) {
 #pragma unroll
   for (int c = 0; c < SL_K; c++) {
-    if (!((mask >> c) & 1)) continue;
-    const double cr = s * cf[2 * c], ci = s * cf[2 * c + 1];
-    if ((realc >> c) & 1) {
-      ar[c] += cr * g[c].x;
-      ai[c] += cr * g[c].y;
-    } else {
-      ar[c] += cr * g[c].x - ci * g[c].y;
-      ai[c] += cr * g[c].y + ci * g[c].x;
+    const double cr = s * cf[2 * c];
+    ar[c] = __builtin_fma(cr, g[c].x, ar[c]);
+    ai[c] = __builtin_fma(cr, g[c].y, ai[c]);
+    if (!REAL) {
+      const double ci = s * cf[2 * c + 1];
+      ar[c] = __builtin_fma(-ci, g[c].y, ar[c]);
+      ai[c] = __builtin_fma(ci, g[c].x, ai[c]);
     }
   }
 }
 
-__device__ __forceinline__ void sl_load(const double2 *__restrict__ row, bool act, int mask, double2 *g) {
+__device__ __forceinline__ void sl_load(const double2 *__restrict__ row, bool act, double2 *g) {
 #pragma unroll
-  for (int c = 0; c < SL_K; c++) {
-    g[c] = make_double2(0.0, 0.0);
-    if (act && ((mask >> c) & 1)) g[c] = row[c];
+  for (int c = 0; c < SL_K; c++) g[c] = make_double2(0.0, 0.0);
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < SL_K; c++) g[c] = row[c];
   }
 }
 
@@ -201,12 +210,13 @@ __device__ __forceinline__ void sl_load(const double2 *__restrict__ row, bool ac
 // Q_c <- cc_c Q_c + s_c (H w)_c and <w_c|H w_c> accumulated into (pre, pim).
 // Gather form of the reference's per-term scatter (mltplySpinCore.c:273-307,
 // mltplyHubbardCore.c:305-347); list_2_1 is read from the LDS copy s21.
+template <bool REAL>
 __device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t *s21, const double *js, double *pre,
                                             double *pim) {
   const uint64_t dim = S->dim;
   const bool valid = j < dim;
   const uint64_t jj = valid ? j : dim - 1;
-  const int mask = S->mask, realc = S->realc;
+  const int mask = S->mask;
   const double2 *__restrict__ R = S->R;
   const uint64_t *__restrict__ l22 = S->l22;
   const uint32_t irght = (uint32_t)S->irght;
@@ -222,9 +232,8 @@ __device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t 
   double2 own[SL_K];
 #pragma unroll
   for (int c = 0; c < SL_K; c++) {
-    own[c] = make_double2(0.0, 0.0);
-    if ((mask >> c) & 1) own[c] = R[jj * SL_K + c];
-    const double d = ((mask >> c) & 1) ? S->diag[jj * SL_K + c] : 0.0;
+    own[c] = R[jj * SL_K + c];
+    const double d = S->diag[jj * SL_K + c];
     ar[c] = d * own[c].x;
     ai[c] = d * own[c].y;
   }
@@ -244,11 +253,11 @@ __device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t 
     }
     double2 g[SL_U][SL_K];
 #pragma unroll
-    for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], mask, g[u]);
+    for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], g[u]);
 #pragma unroll
     for (int u = 0; u < SL_U; u++) {
       const GTerm T = load_term(tl + t + u);
-      sl_mac(ar, ai, cl + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u], mask, realc);
+      sl_mac<REAL>(ar, ai, cl + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u]);
     }
   }
   // ---- high-half and mixed terms
@@ -271,11 +280,11 @@ __device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t 
       }
       double2 g[SL_U][SL_K];
 #pragma unroll
-      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], mask, g[u]);
+      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], g[u]);
 #pragma unroll
       for (int u = 0; u < SL_U; u++) {
         const GTerm T = load_term(th + t + u);
-        sl_mac(ar, ai, ch + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, xh), g[u], mask, realc);
+        sl_mac<REAL>(ar, ai, ch + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, xh), g[u]);
       }
     }
     for (int t = 0; t < S->n_mixed; t += SL_U) {
@@ -293,11 +302,11 @@ __device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t 
       }
       double2 g[SL_U][SL_K];
 #pragma unroll
-      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], mask, g[u]);
+      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], g[u]);
 #pragma unroll
       for (int u = 0; u < SL_U; u++) {
         const GTerm T = load_term(tm + t + u);
-        sl_mac(ar, ai, cm + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u], mask, realc);
+        sl_mac<REAL>(ar, ai, cm + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u]);
       }
     }
   } else {  // lanes span several blocks: per-lane list_2_2 lookups
@@ -319,11 +328,11 @@ __device__ __forceinline__ void sl_hv_state(CSlab S, uint64_t j, const uint16_t 
       }
       double2 g[SL_U][SL_K];
 #pragma unroll
-      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], mask, g[u]);
+      for (int u = 0; u < SL_U; u++) sl_load(R + idx[u] * SL_K, act[u], g[u]);
 #pragma unroll
       for (int u = 0; u < SL_U; u++) {
         const GTerm T = load_term(th + t + u);
-        sl_mac(ar, ai, ch + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u], mask, realc);
+        sl_mac<REAL>(ar, ai, ch + (uint64_t)(t + u) * SL_K * 2, sl_sign((uint32_t)T.sgn, x), g[u]);
       }
     }
   }
@@ -357,11 +366,14 @@ __global__ __launch_bounds__(SV_TPB) void k_sl_hv(const SlabDev *slabs, const ui
 #pragma unroll
   for (int c = 0; c < SL_K; c++) pre[c] = pim[c] = 0.0;
   const uint64_t t0 = (chunk - ((CU64)prefix)[sb]) * SL_TPW;
+  const int full = (1 << SL_K) - 1;
+  const bool real = ((S->realc | ~S->mask) & full) == full;  // every running column has real couplings
   const uint64_t dim = S->dim;
   for (int t = 0; t < SL_TPW; t++) {
     const uint64_t j = (t0 + t) * SV_TPB + threadIdx.x;
     if ((t0 + t) * SV_TPB >= dim) break;  // uniform: the chunk's last tiles may lie past the slab
-    sl_hv_state(S, j, s21, js, pre, pim);
+    if (real) sl_hv_state<true>(S, j, s21, js, pre, pim);
+    else sl_hv_state<false>(S, j, s21, js, pre, pim);
   }
   // per-column prdct partials: [blk][SL_K] double2
   __shared__ double2 red[SV_TPB / 64][SL_K];
@@ -701,7 +713,7 @@ struct Slab {
   double *hist = nullptr;   // [SL_K][2][max_iter + 2]
   std::vector<double2> coef_host;
   uint64_t nnz = 0;  // off-diagonal elements of one column's H (hphi_term_nnz)
-  Job *col[SL_K] = {nullptr, nullptr, nullptr, nullptr};
+  Job *col[SL_K] = {};
   int used() const {
     int n = 0;
     for (int c = 0; c < SL_K; c++) n += col[c] != nullptr;
