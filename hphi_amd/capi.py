@@ -601,11 +601,11 @@ class Server:
         return a.value, w.value
 
     def stats(self):
-        out = np.zeros(9)
+        out = np.zeros(10)
         check(lib().hphi_server_stats(self.h, _p(out, _f64p)))
         return {"admit_s": out[0], "gpu_s": out[1], "check_s": out[2], "ticks": int(out[3]),
                 "job_steps": int(out[4]), "slab_steps": int(out[5]), "hv_s": out[6], "hv_bytes": out[7],
-                "hv_launches": int(out[8])}
+                "hv_launches": int(out[8]), "moves": int(out[9])}
 
     def results(self, max_n: int = 4096):
         buf = (JobResult * max_n)()

@@ -762,6 +762,7 @@ struct hphi_server {
   hipEvent_t ev_hv[2 * SL_SMAX] = {};
   double st_hv_s = 0, st_hv_bytes = 0;
   uint64_t st_hv_launches = 0;
+  uint64_t st_moves = 0;  // columns moved between slabs (compact)
 };
 
 extern "C" double hphi_server_now(const hphi_server *s) { return s ? mono_now() - s->t0 : 0.0; }
@@ -954,6 +955,22 @@ static int column_start(hphi_server *s, Slab *b, int c, Job *j) {
   return 0;
 }
 
+// moves column cs of one slab into column cd of another slab of the same
+// structure: the Lanczos vectors R and Q, the diagonal, the couplings and the
+// alpha/beta history (nh doubles per column)
+__global__ void k_sl_move(const double2 *R0, const double2 *Q0, const double *d0, const double2 *c0, const double *h0,
+                          double2 *R1, double2 *Q1, double *d1, double2 *c1, double *h1, int cs, int cd,
+                          uint64_t dim, uint64_t nt, uint64_t nh) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < dim) {
+    R1[i * SL_K + cd] = R0[i * SL_K + cs];
+    Q1[i * SL_K + cd] = Q0[i * SL_K + cs];
+    d1[i * SL_K + cd] = d0[i * SL_K + cs];
+  }
+  if (i < nt) c1[i * SL_K + cd] = c0[i * SL_K + cs];
+  if (i < nh) h1[(uint64_t)cd * nh + i] = h0[(uint64_t)cs * nh + i];
+}
+
 static void retire(hphi_server *s, Job *j, double now, int converged, const double *ev, int n) {
   j->res.id = j->id;
   j->res.iterations = j->step;
@@ -1055,6 +1072,60 @@ static int admit(hphi_server *s, double now) {
     s->waiting.pop_front();
     j->t_admit = now;
     s->active.push_back(j);
+  }
+  return 0;
+}
+
+// Slabs of one structure whose running columns fit in fewer slabs are packed:
+// columns move from the emptiest slabs into the fullest ones with free
+// columns, so a structure whose queue has drained runs ceil(columns / SL_K)
+// slab steps per step instead of one per slab.  Structures with arrived
+// waiting jobs are left alone (admission fills their free columns).  A moved
+// column keeps its arithmetic (same tiles, same per-column reductions, and a
+// real column computes the same bits on the complex path), so its results do
+// not change.
+static int compact(hphi_server *s, double now) {
+  std::vector<uint64_t> busy;  // structures with arrived waiting jobs in the admission window
+  int seen = 0;
+  for (const Job *w : s->waiting) {
+    if (seen++ == ADMIT_WINDOW) break;
+    if (w->t_arrival <= now) busy.push_back(w->shash);
+  }
+  std::map<uint64_t, std::vector<Slab *>> groups;
+  for (Slab *b : s->slabs) {
+    const int u = b->used();
+    if (u > 0 && u < SL_K && std::find(busy.begin(), busy.end(), b->shash) == busy.end())
+      groups[b->shash].push_back(b);
+  }
+  const uint64_t nh = 2 * ((uint64_t)s->p.max_iter + 2);
+  for (auto &g : groups) {
+    std::vector<Slab *> &L = g.second;
+    int cols = 0;
+    for (const Slab *b : L) cols += b->used();
+    if ((size_t)((cols + SL_K - 1) / SL_K) >= L.size()) continue;  // no slab to free
+    std::sort(L.begin(), L.end(), [](const Slab *a, const Slab *b) { return a->used() > b->used(); });
+    size_t i = 0, k = L.size() - 1;
+    while (i < k) {
+      Slab *A = L[i], *B = L[k];
+      if (A->used() == SL_K) { i++; continue; }
+      if (B->used() == 0 || !same_structure(A->proto, B->proto)) { k--; continue; }
+      int cs = 0, cd = 0;
+      while (!B->col[cs]) cs++;
+      while (A->col[cd]) cd++;
+      const hphi_engine &h = A->proto;
+      const uint64_t nt = h.h_terms_dev.size();
+      const uint64_t n = std::max<uint64_t>(std::max<uint64_t>(h.dim, nt), nh);
+      hipLaunchKernelGGL(k_sl_move, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, B->R, B->Q,
+                         B->diag, B->coef, B->hist, A->R, A->Q, A->diag, A->coef, A->hist, cs, cd, h.dim, nt, nh);
+      HIPCHK(hipGetLastError());
+      for (uint64_t t = 0; t < nt; t++) A->coef_host[t * SL_K + cd] = B->coef_host[t * SL_K + cs];
+      Job *j = B->col[cs];
+      B->col[cs] = nullptr;
+      A->col[cd] = j;
+      j->slab = A;
+      j->col = cd;
+      s->st_moves++;
+    }
   }
   return 0;
 }
@@ -1238,6 +1309,7 @@ extern "C" int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting) {
       keep.push_back(j);
     }
     s->active.swap(keep);
+    if (compact(s, now)) return -1;
     // slabs whose last job left give their memory back
     std::vector<Slab *> live;
     for (Slab *b : s->slabs) {
@@ -1267,6 +1339,7 @@ extern "C" int hphi_server_stats(const hphi_server *s, double *out) {
   out[6] = s->st_hv_s;
   out[7] = s->st_hv_bytes;
   out[8] = (double)s->st_hv_launches;
+  out[9] = (double)s->st_moves;
   return 0;
 }
 

@@ -453,12 +453,13 @@ int hphi_server_submit(hphi_server *s, const hphi_model_desc *d, uint64_t seed, 
 int hphi_server_tick(hphi_server *s, int *n_active, int *n_waiting);
 int hphi_server_results(hphi_server *s, hphi_job_result *out, int max, int *n);
 double hphi_server_now(const hphi_server *s);
-/* out[0..8]: seconds in admission, in launched work (including the wait), in
+/* out[0..9]: seconds in admission, in launched work (including the wait), in
  * convergence checks; number of non-empty ticks; job-steps advanced;
  * slab-steps advanced (a slab holds up to 4 jobs of one structure); the slab
  * H·v launches: seconds between HIP events around them, their algorithmic
  * bytes (per slab 8 dim, per running column 56 dim + 16 per off-diagonal
- * element), their number */
+ * element), their number; jobs moved between slabs of one structure to pack
+ * their running columns into fewer slabs */
 int hphi_server_stats(const hphi_server *s, double *out);
 
 /* k lowest eigenvalues (ascending) of the symmetric tridiagonal matrix with

@@ -66,6 +66,38 @@ def test_server_result_same_alone_and_in_mix(A):
     assert list(alone.eigenvalues) == list(mix.eigenvalues)
 
 
+def run_server_stats(A, specs, seeds, budget=4 << 30, max_jobs=64):
+    srv = A.Server(hbm_budget=budget, max_jobs=max_jobs, n_eigen=4, eps=1e-12, check_every=5)
+    ids = [srv.submit(s, seed, 0.0) for s, seed in zip(specs, seeds)]
+    res = {}
+    for _ in range(100000):
+        a, w = srv.tick()
+        for r in srv.results():
+            res[r.id] = r
+        if a == 0 and w == 0:
+            break
+    st = srv.stats()
+    srv.close()
+    return [res[i] for i in ids], st
+
+
+def test_server_result_same_after_column_moves(A):
+    """Twelve jobs of one structure fill three slabs; as they converge at
+    different steps the service packs the survivors into fewer slabs, moving
+    columns between slabs.  Every job's iterations and eigenvalues equal its
+    run alone, bit for bit."""
+    specs = [L.j1j2_square(4, 3, 1.0, 0.05 + 0.08 * i) for i in range(12)]
+    seeds = [100 + i for i in range(12)]
+    print("[gpu] serve column moves", flush=True)
+    mix, st = run_server_stats(A, specs, seeds)
+    assert st["moves"] > 0, st
+    for sp, seed, m in zip(specs, seeds, mix):
+        alone = run_server(A, [sp], [seed])[0]
+        assert m.converged and alone.converged
+        assert alone.iterations == m.iterations, (sp.name, alone.iterations, m.iterations)
+        assert list(alone.eigenvalues) == list(m.eigenvalues), sp.name
+
+
 def test_server_admits_by_hbm_footprint(A):
     specs = [L.heisenberg_chain(16, 1.0, 0.1 + 0.2 * i) for i in range(6)]  # dim 12870, one structure
     # a slab (4 columns of vectors and diagonal, tables, histories) needs about
