@@ -541,6 +541,85 @@ def serve_per_size(args, n_jobs: int = 32, n_plain: int = 4):
     return out
 
 
+def serve_classes(args, n_jobs: int = 24, n_plain: int = 24):
+    """Served vs plain per structure class of the stream: for each of the
+    stream's structures (seed 3, args.requests requests) the first n_jobs
+    requests of that structure go to the service at once and the first
+    n_plain of them (by default the same requests) are solved one at a time
+    by the plain path (one engine each), with the stream's settings.  Weighting each class's ms per solve by
+    its share of the stream gives the whole-stream ratio and shows which
+    classes cap it."""
+    from collections import defaultdict
+
+    import numpy as np
+
+    from hphi_amd import capi, lattice as L
+
+    rng = np.random.default_rng(3)
+    reqs = [L.stream_request(rng) for _ in range(args.requests)]
+    by = defaultdict(list)
+    for i, r in enumerate(reqs):
+        by[r.name].append(i)
+    warm = capi.Engine.from_spec(reqs[0])  # first-call set-up kept out of the first class's time
+    warm.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=0)
+    warm.close()
+    rows = {}
+    for name, ids in sorted(by.items(), key=lambda kv: -reqs[kv[1][0]].dim):
+        t = time.time()
+        its = []
+        for i in ids[:n_plain]:
+            eng = capi.Engine.from_spec(reqs[i])
+            its.append(eng.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=int(i)).iterations)
+            eng.close()
+        plain_ms = 1e3 * (time.time() - t) / len(ids[:n_plain])
+        take = ids[:n_jobs]
+        srv = capi.Server(hbm_budget=int(args.hbm_gb * (1 << 30)), max_jobs=args.max_jobs, n_eigen=4, eps=1e-12,
+                          check_every=5)
+        t0 = time.time()
+        for i in take:
+            srv.submit(reqs[i], int(i), 0.0)
+        done = []
+        while len(done) < len(take):
+            srv.tick()
+            done += srv.results()
+        served_ms = 1e3 * (time.time() - t0) / len(take)
+        st = srv.stats()
+        srv.close()
+        rows[name] = {"dim": int(reqs[ids[0]].dim), "count": len(ids), "plain_ms": plain_ms, "served_ms": served_ms,
+                      "plain_iters": float(np.mean(its)), "served_iters": float(np.mean([r.iterations for r in done])),
+                      "cols_per_slab_step": st["job_steps"] / max(st["slab_steps"], 1)}
+        print(f"[classes] {name}: plain {plain_ms:.2f} ms, served {served_ms:.2f} ms per solve", file=sys.stderr,
+              flush=True)
+    tp = sum(v["plain_ms"] * v["count"] for v in rows.values())
+    ts = sum(v["served_ms"] * v["count"] for v in rows.values())
+    for v in rows.values():
+        v["plain_share"] = v["plain_ms"] * v["count"] / tp
+        v["served_share"] = v["served_ms"] * v["count"] / ts
+    return {"requests": args.requests, "plain_ms_per_request": tp / args.requests,
+            "served_ms_per_request": ts / args.requests, "ratio": tp / ts, "classes": rows}
+
+
+def write_classes(res: dict, out: str, cmd: str):
+    """Text summary of serve_classes: one line per class, largest served share first."""
+    lines = [f"# served vs plain per structure class of the {res['requests']:,}-request stream (seed 3)",
+             f"# command: {cmd}",
+             "# plain: the class's first requests solved one at a time (engine set-up included);",
+             "# served: the same requests submitted at once to the service;",
+             "# shares weight ms per solve by the class's count in the stream",
+             f"# stream-weighted: plain {res['plain_ms_per_request']:.2f} ms, served "
+             f"{res['served_ms_per_request']:.2f} ms per request -> ratio {res['ratio']:.2f}",
+             f"{'class':18s} {'dim':>9s} {'count':>6s} {'plain ms':>9s} {'served ms':>9s} {'ratio':>6s} "
+             f"{'served share':>12s} {'plain share':>11s} {'iters p/s':>11s} {'cols/step':>9s}"]
+    cum = 0.0
+    for n, v in sorted(res["classes"].items(), key=lambda kv: -kv[1]["served_share"]):
+        cum += v["served_share"]
+        lines.append(f"{n:18s} {v['dim']:9d} {v['count']:6d} {v['plain_ms']:9.2f} {v['served_ms']:9.2f} "
+                     f"{v['plain_ms'] / v['served_ms']:6.2f} {v['served_share']:12.3f} {v['plain_share']:11.3f} "
+                     f"{v['plain_iters']:5.0f}/{v['served_iters']:<5.0f} {v['cols_per_slab_step']:9.2f}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -574,6 +653,8 @@ def main():
     ap.add_argument("--stats-balg", type=float, default=float(b_alg(601080390, 39710085120)),
                     help="algorithmic bytes per launch of --stats-kernel (0: report the average only)")
     ap.add_argument("--serve", action="store_true", help="served-stream benchmark instead of plain H·v")
+    ap.add_argument("--serve-classes", type=str, default="",
+                    help="served vs plain per structure class of the stream; writes the summary to this file")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rates", type=str, default="0", help="arrival rates /s, comma separated; 0 = all at once")
     ap.add_argument("--seq-sample", type=int, default=100)
@@ -616,6 +697,14 @@ def main():
                           "higher_is_better": False, "dtype": "c128", "data": "synthetic",
                           "config": {"workload": "triangular 13x2 XXZ Jz=1 Jxy=0.8 hx=0.1, SpinGC dim 2^26", **c4}}),
               flush=True)
+        return
+    if args.serve_classes:
+        res = serve_classes(args)
+        write_classes(res, args.serve_classes,
+                      f"python bench.py --serve-classes {args.serve_classes} --requests {args.requests}")
+        print(json.dumps({"metric": "served vs plain per structure class", "value": round(res["ratio"], 3),
+                          "unit": "x", "n_gpus": 1, "higher_is_better": True, "dtype": "c128",
+                          "data": "synthetic", "config": {"workload": "served stream classes", **res}}), flush=True)
         return
     if args.serve:
         sv = serve_leg(args)
