@@ -743,7 +743,7 @@ def main():
     }
     if args.serve_sample > 0:  # the metric's served half, on a sample of the stream
         sa = argparse.Namespace(**vars(args))
-        sa.requests, sa.rates, sa.seq_sample = args.serve_sample, "0,120", min(args.seq_sample, 40)
+        sa.requests, sa.rates, sa.seq_sample = args.serve_sample, "0,120", args.seq_sample
         sv = serve_leg(sa)
         p0 = next(p for p in sv["points"] if p["rate"] == 0.0)
         p1 = next(p for p in sv["points"] if p["rate"] == 120.0)
