@@ -498,10 +498,11 @@ PER_SIZE = (("chain16", lambda j2: ("heisenberg_chain", (16, 1.0, j2))),
             ("j1j2_6x4", lambda j2: ("j1j2_square", (6, 4, 1.0, j2))))
 
 
-def serve_per_size(args, n_jobs: int = 32, n_plain: int = 4):
+def serve_per_size(args, n_jobs: int = 32, n_plain: int = 32):
     """Served vs plain per job size: n_jobs requests of one lattice (random
     J2, seed 5) submitted at once to the service, and the first n_plain of them
-    solved one at a time by the plain path (one engine each), both with the
+    (by default the same requests) solved one at a time by the plain path (one
+    engine each), both with the
     stream's settings (lowest 4 distinct eigenvalues, eps 1e-12, check every
     5 steps).  ms per solve each way and their ratio."""
     import numpy as np
