@@ -98,6 +98,10 @@ def gpu_leg(args):
     ms = eng.time_mltply(1, v0, v1, 0, args.steps)  # HIP events on the engine's stream
     capi.lib().hphi_synchronize()
     wall = time.time() - t
+    # spread of single launches (each bracketed by its own pair of HIP events)
+    import numpy as np
+    single = np.array([eng.time_mltply(1, v0, v1, 0, 1) for _ in range(max(args.steps, 5))])
+    spread = {q: round(float(np.percentile(single, p)), 3) for q, p in (("p10", 10), ("p50", 50), ("p90", 90))}
     # Lanczos step (H·v + update + beta + tridiagonal), device-resident driver
     lz = eng.lanczos(n_eigen=1, max_iter=args.lanczos_steps, eps=1e-300, target=2, seed=2)
     v0.free()
@@ -133,6 +137,7 @@ def gpu_leg(args):
         del h0, h1
     res = {
         "dim": dim, "nnz": nnz, "balg": balg, "ms": ms, "wall_s": wall, "setup_s": t_setup, "host": host,
+        "spread": spread,
         "lanczos_step_ms": 1e3 * lz.seconds / max(lz.iterations, 1),
         "device_gb": device_gb,
         "block_ms_per_column": blk,
@@ -731,6 +736,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "dim": g["dim"], "nnz_offdiag": g["nnz"],
                    "bytes_alg_per_hv": g["balg"], "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+                   "hv_ms_single_launch": g["spread"],
                    "lanczos_step_ms": round(g["lanczos_step_ms"], 3),
                    "block_k": args.block_k if g["block_ms_per_column"] else None,
                    "block_ms_per_column": round(g["block_ms_per_column"], 3) if g["block_ms_per_column"] else None,
