@@ -440,43 +440,65 @@ def serve_leg(args):
 
     from hphi_amd import capi, lattice as L
 
+    from collections import defaultdict
+
     rng = np.random.default_rng(3)
     reqs = [L.stream_request(rng) for _ in range(args.requests)]
     arr_rng = np.random.default_rng(3)
     out = {"requests": args.requests, "dims": [int(min(r.dim for r in reqs)), int(max(r.dim for r in reqs))]}
-    # sequential plain solves on a fixed sample of the same requests
-    idx = np.linspace(0, len(reqs) - 1, min(args.seq_sample, len(reqs))).astype(int)
-    t = time.time()
-    for i in idx:
-        eng = capi.Engine.from_spec(reqs[i])
-        r = eng.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=int(i))
-        eng.close()
-    seq_s = (time.time() - t) / len(idx)
+    # sequential plain solves, stratified by structure class: the first
+    # args.seq_per_class requests of every class of the stream (all of them
+    # with 0), each class's mean weighted by its count in the stream
+    by = defaultdict(list)
+    for i, r in enumerate(reqs):
+        by[r.name].append(i)
+    warm = capi.Engine.from_spec(reqs[0])  # first-call set-up kept out of the timing
+    warm.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=0)
+    warm.close()
+    seq_total, n_seq = 0.0, 0
+    for name, ids in by.items():
+        take = ids if args.seq_per_class <= 0 else ids[:args.seq_per_class]
+        t = time.time()
+        for i in take:
+            eng = capi.Engine.from_spec(reqs[i])
+            eng.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=int(i))
+            eng.close()
+        seq_total += (time.time() - t) / len(take) * len(ids)
+        n_seq += len(take)
+        print(f"[serve] sequential {name}: {len(take)} solves, {1e3 * (time.time() - t) / len(take):.2f} ms each",
+              file=sys.stderr, flush=True)
+    seq_s = seq_total / len(reqs)
     out["sequential_solves_per_s"] = 1.0 / seq_s
-    print(f"[serve] sequential: {len(idx)} solves, {1.0 / seq_s:.1f}/s", file=sys.stderr, flush=True)
-    out["sequential_sample"] = int(len(idx))
+    print(f"[serve] sequential: {n_seq} solves over {len(by)} classes, {1.0 / seq_s:.1f}/s", file=sys.stderr,
+          flush=True)
+    out["sequential_sample"] = int(n_seq)
+    out["sequential_method"] = (f"every request" if args.seq_per_class <= 0 else
+                                f"first {args.seq_per_class} of each of the {len(by)} structure classes, "
+                                f"weighted by the class counts")
     rates = [float(x) for x in args.rates.split(",")]
     points = []
     for rate in rates:
+        # latency points (rate > 0) may run on the first args.latency_requests requests
+        nr = len(reqs) if rate <= 0 or args.latency_requests <= 0 else min(len(reqs), args.latency_requests)
         gaps = arr_rng.exponential(1.0 / rate, size=len(reqs)) if rate > 0 else np.zeros(len(reqs))
         arrivals = np.cumsum(gaps) - gaps[0]
         srv = capi.Server(hbm_budget=int(args.hbm_gb * (1 << 30)), max_jobs=args.max_jobs, n_eigen=4,
                           eps=1e-12, check_every=5)
         start = srv.now() + 0.05
-        for i, r in enumerate(reqs):
+        for i, r in enumerate(reqs[:nr]):
             srv.submit(r, i, start + float(arrivals[i]))
         res = []
         ticks = 0
         while srv.now() < start:
             time.sleep(0.001)
         last = time.time()
-        while len(res) < len(reqs):
+        while len(res) < nr:
             srv.tick()
             ticks += 1
             res += srv.results()
             if time.time() - last > 10.0:
                 last = time.time()
-                print(f"[serve] rate {rate}: {len(res)}/{len(reqs)} done", file=sys.stderr, flush=True)
+                print(f"[serve] rate {rate}: {len(res)}/{nr} done", file=sys.stderr, flush=True)
         st = srv.stats()
         srv.close()
         lat = np.array([r.t_done - r.t_arrival for r in res])
@@ -486,7 +508,8 @@ def serve_leg(args):
                    "model": reqs[r.id].name} for r in res if not r.converged]
         hv = {"hv_gbs": round(st["hv_bytes"] / st["hv_s"] / 1e9, 1) if st["hv_s"] > 0 else None,
               "hv_frac": round(st["hv_bytes"] / st["hv_s"] / 1e9 / HBM_PEAK_GBS, 4) if st["hv_s"] > 0 else None}
-        points.append({"rate": rate, "solves_per_s": len(res) / makespan, "p50_s": float(np.percentile(lat, 50)),
+        points.append({"rate": rate, "requests": nr, "solves_per_s": len(res) / makespan,
+                       "p50_s": float(np.percentile(lat, 50)),
                        **hv,
                        "unconverged": unconv[:8],
                        "p99_s": float(np.percentile(lat, 99)), "makespan_s": makespan,
@@ -648,8 +671,10 @@ def main():
                          "SQ issue counters (DIR/probe.json: the probe's stdout)")
     ap.add_argument("--pmc-cfg", type=int, default=3, choices=(2, 3), help="workload of --pmc-probe / --reduce-pmc")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-sample", type=int, default=1000,
+    ap.add_argument("--serve-sample", type=int, default=10000,
                     help="requests of the served stream timed with the default run (0: skip)")
+    ap.add_argument("--latency-requests", type=int, default=0,
+                    help="requests of the stream served at the arrival rates > 0 (0: all); the default run uses 2000")
     ap.add_argument("--host-reps", type=int, default=2, help="timed host-pointer (compatibility mode) calls; 0: skip")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("KERNEL_STATS_CSV", "OUT"),
                     help="write a text summary of a rocprofv3 --stats run (with --stats-cmd, --stats-title)")
@@ -661,9 +686,10 @@ def main():
     ap.add_argument("--serve", action="store_true", help="served-stream benchmark instead of plain H·v")
     ap.add_argument("--serve-classes", type=str, default="",
                     help="served vs plain per structure class of the stream; writes the summary to this file")
-    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--requests", type=int, default=10000)
     ap.add_argument("--rates", type=str, default="0", help="arrival rates /s, comma separated; 0 = all at once")
-    ap.add_argument("--seq-sample", type=int, default=100)
+    ap.add_argument("--seq-per-class", type=int, default=16,
+                    help="sequential plain solves per structure class of the stream (0: every request)")
     ap.add_argument("--max-jobs", type=int, default=512)
     ap.add_argument("--hbm-gb", type=float, default=96.0)
     args = ap.parse_args()
@@ -748,15 +774,17 @@ def main():
                      "frac": round(value / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles("hv_j1j2_8x4")},
     }
-    if args.serve_sample > 0:  # the metric's served half, on a sample of the stream
+    if args.serve_sample > 0:  # the metric's served half: the stream back-to-back, its first 2,000 at 120 req/s
         sa = argparse.Namespace(**vars(args))
-        sa.requests, sa.rates, sa.seq_sample = args.serve_sample, "0,120", args.seq_sample
+        sa.requests, sa.rates = args.serve_sample, "0,120"
+        sa.latency_requests = args.latency_requests if args.latency_requests > 0 else 2000
         sv = serve_leg(sa)
         p0 = next(p for p in sv["points"] if p["rate"] == 0.0)
         p1 = next(p for p in sv["points"] if p["rate"] == 120.0)
         line["config"]["served_sample"] = {
             "requests": sa.requests, "solves_per_s": round(p0["solves_per_s"], 2),
             "sequential_solves_per_s": round(sv["sequential_solves_per_s"], 2),
+            "sequential_method": sv["sequential_method"], "latency_requests": p1["requests"],
             "speedup_vs_sequential": round(p0["solves_per_s"] / sv["sequential_solves_per_s"], 3),
             "rate_per_s": 120.0, "p50_s": round(p1["p50_s"], 3), "p99_s": round(p1["p99_s"], 3),
             "converged": [p0["converged"], p1["converged"]],
@@ -765,7 +793,8 @@ def main():
                         "note": "algorithmic bytes (per slab 8 dim, per running column 56 dim + 16 per "
                                 "off-diagonal element) over HIP-event time of the slab H·v launches"},
             "per_size": serve_per_size(args),
-            "note": "first requests of the seed-3 stream; bench.py --serve --requests 10000 runs the whole stream"}
+            "note": "the seed-3 stream back-to-back; p50/p99 from its first latency_requests requests at 120 "
+                    "req/s (bench.py --serve --rates 0,120,150 runs every rate on the whole stream)"}
     if not args.no_cpu:
         c = cpu_leg(args)
         line["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample", "port_over_reference")}
