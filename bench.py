@@ -431,6 +431,24 @@ def cpu_leg(args):
             "seconds": t}
 
 
+def stream_classes(reqs):
+    """Indices of the stream's requests per structure class (the lattice
+    name), classes in order of first appearance."""
+    from collections import defaultdict
+    by = defaultdict(list)
+    for i, r in enumerate(reqs):
+        by[r.name].append(i)
+    return dict(by)
+
+
+def stratified_seconds_per_request(classes, class_seconds):
+    """Mean seconds per request of the whole stream from per-class samples:
+    classes = {name: request indices}, class_seconds = {name: mean seconds of
+    that class's sampled solves}; each class weighs by its request count."""
+    n = sum(len(ids) for ids in classes.values())
+    return sum(class_seconds[name] * len(ids) for name, ids in classes.items()) / n
+
+
 def serve_leg(args):
     """Served stream (workload 5): Poisson arrivals of random small clusters,
     lowest 4 distinct eigenvalues each, through the continuous-batching
@@ -440,8 +458,6 @@ def serve_leg(args):
 
     from hphi_amd import capi, lattice as L
 
-    from collections import defaultdict
-
     rng = np.random.default_rng(3)
     reqs = [L.stream_request(rng) for _ in range(args.requests)]
     arr_rng = np.random.default_rng(3)
@@ -449,13 +465,11 @@ def serve_leg(args):
     # sequential plain solves, stratified by structure class: the first
     # args.seq_per_class requests of every class of the stream (all of them
     # with 0), each class's mean weighted by its count in the stream
-    by = defaultdict(list)
-    for i, r in enumerate(reqs):
-        by[r.name].append(i)
+    by = stream_classes(reqs)
     warm = capi.Engine.from_spec(reqs[0])  # first-call set-up kept out of the timing
     warm.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=0)
     warm.close()
-    seq_total, n_seq = 0.0, 0
+    class_s, n_seq = {}, 0
     for name, ids in by.items():
         take = ids if args.seq_per_class <= 0 else ids[:args.seq_per_class]
         t = time.time()
@@ -463,11 +477,11 @@ def serve_leg(args):
             eng = capi.Engine.from_spec(reqs[i])
             eng.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=int(i))
             eng.close()
-        seq_total += (time.time() - t) / len(take) * len(ids)
+        class_s[name] = (time.time() - t) / len(take)
         n_seq += len(take)
-        print(f"[serve] sequential {name}: {len(take)} solves, {1e3 * (time.time() - t) / len(take):.2f} ms each",
+        print(f"[serve] sequential {name}: {len(take)} solves, {1e3 * class_s[name]:.2f} ms each",
               file=sys.stderr, flush=True)
-    seq_s = seq_total / len(reqs)
+    seq_s = stratified_seconds_per_request(by, class_s)
     out["sequential_solves_per_s"] = 1.0 / seq_s
     print(f"[serve] sequential: {n_seq} solves over {len(by)} classes, {1.0 / seq_s:.1f}/s", file=sys.stderr,
           flush=True)
@@ -578,17 +592,13 @@ def serve_classes(args, n_jobs: int = 24, n_plain: int = 24):
     by the plain path (one engine each), with the stream's settings.  Weighting each class's ms per solve by
     its share of the stream gives the whole-stream ratio and shows which
     classes cap it."""
-    from collections import defaultdict
-
     import numpy as np
 
     from hphi_amd import capi, lattice as L
 
     rng = np.random.default_rng(3)
     reqs = [L.stream_request(rng) for _ in range(args.requests)]
-    by = defaultdict(list)
-    for i, r in enumerate(reqs):
-        by[r.name].append(i)
+    by = stream_classes(reqs)
     warm = capi.Engine.from_spec(reqs[0])  # first-call set-up kept out of the first class's time
     warm.lanczos(n_eigen=4, max_iter=2000, eps=1e-12, check_every=5, mode=1, seed=0)
     warm.close()

@@ -64,3 +64,26 @@ def test_library_nnz_count_matches_dense_operators():
         assert nnz == int(np.count_nonzero(off)), name
         checked += 1
     assert checked >= 5
+
+
+def test_stratified_sequential_baseline():
+    """The sequential baseline of the served stream weights each structure
+    class's mean solve time by its count: with every request sampled it is the
+    plain mean; with a uniform time per class it is independent of how many of
+    each class were sampled."""
+    import numpy as np
+
+    import bench
+    rng = np.random.default_rng(3)
+    reqs = [L.stream_request(rng) for _ in range(2000)]
+    by = bench.stream_classes(reqs)
+    assert sum(len(v) for v in by.values()) == 2000 and len(by) == 32
+    assert all(reqs[i].name == name for name, ids in by.items() for i in ids)
+    t = {i: 1e-3 * (1 + reqs[i].dim ** 0.5 / 100) + 1e-6 * (i % 7) for i in range(2000)}  # any per-request time
+    exact = sum(t.values()) / 2000
+    full = {name: sum(t[i] for i in ids) / len(ids) for name, ids in by.items()}
+    assert abs(bench.stratified_seconds_per_request(by, full) - exact) < 1e-15
+    # a class-constant time is recovered exactly from one sample per class
+    const = {name: 1e-3 * (k + 1) for k, name in enumerate(by)}
+    want = sum(const[reqs[i].name] for i in range(2000)) / 2000
+    assert abs(bench.stratified_seconds_per_request(by, const) - want) < 1e-15
