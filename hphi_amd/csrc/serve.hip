@@ -68,12 +68,8 @@
   } while (0)
 
 #define SV_TPB 256  // one tile = SV_TPB states of one slab
-#ifndef SL_K
-#define SL_K 4      // columns (jobs) per slab
-#endif
-#ifndef SL_U
+#define SL_K 4      // columns (jobs) per slab (8 with one lane per state: 2-3 waves per SIMD, slower per column)
 #define SL_U 2      // terms per gather batch (SL_U * SL_K gathers in flight)
-#endif
 #define SL_SMAX 16  // Lanczos steps per tick at most
 #define SL_TPW 1    // tiles per workgroup (a chunk): 1, 2, 4, 8 measured, 1 fastest at every job size
 
@@ -182,7 +178,7 @@ __device__ __forceinline__ double sl_sign(uint32_t sgn, uint32_t x) {
 // term loop carries no per-column tests.  REAL: the couplings of every
 // running column are real.  The complex form applies the imaginary part in
 // separate fused steps after the real one, so a real column gets the same
-// bits on either path: fma(-0 * ..) leaves the accumulator unchanged.
+// values on either path: a fused step with a zero factor adds a zero.
 template <bool REAL>
 __device__ __forceinline__ void sl_mac(double *ar, double *ai, CD2 cf, double s, const double2 *g) {
 #pragma unroll
