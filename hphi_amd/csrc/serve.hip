@@ -1085,7 +1085,7 @@ static int admit(hphi_server *s, double now) {
 // slab steps per step instead of one per slab.  Structures with arrived
 // waiting jobs are left alone (admission fills their free columns).  A moved
 // column keeps its arithmetic (same tiles, same per-column reductions, and a
-// real column computes the same bits on the complex path), so its results do
+// real column computes the same values on the complex path), so its results do
 // not change.
 static int compact(hphi_server *s, double now) {
   std::vector<uint64_t> busy;  // structures with arrived waiting jobs in the admission window
