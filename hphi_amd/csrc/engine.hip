@@ -919,6 +919,9 @@ extern "C" void hphi_engine_destroy(hphi_engine *e) {
   (void)hipFree(e->d_l21);
   (void)hipFree(e->d_l22);
   (void)hipFree(e->d_crank);
+  (void)hipFree(e->d_hb);
+  (void)hipFree(e->d_blkpos);
+  (void)hipFree(e->d_v1t);
   (void)hipFree(e->d_jb);
   (void)hipFree(e->d_diag);
   (void)hipFree(e->d_terms);
@@ -982,6 +985,12 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
   } else if (k == "c2_unroll") {
     if (value != 8 && value != 16) return hphi_set_error("c2_unroll must be 8 or 16");
     e->c2_unroll = value;
+  } else if (k == "hi_split") {
+    if (value < -1 || value > 1) return hphi_set_error("hi_split must be -1, 0 or 1");
+    e->hi_split = value;
+  } else if (k == "hi_wgs_per_cu") {
+    if (value < 1 || value > 64) return hphi_set_error("hi_wgs_per_cu out of range");
+    e->hi_wgs_per_cu = value;
   } else if (k == "xcd_map") {
     e->xcd_map = value ? 1 : 0;
     e->xcd_tuned = 1;
@@ -1021,6 +1030,8 @@ extern "C" int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int 
   else if (k == "c2_crank") *value = e->c2_crank;
   else if (k == "c2_unroll") *value = e->c2_unroll;
   else if (k == "xcd_map") *value = e->xcd_map;
+  else if (k == "hi_split") *value = hphi_hi_wanted(e);
+  else if (k == "hi_wgs_per_cu") *value = e->hi_wgs_per_cu;
   else if (k == "use_c2") *value = e->use_c2 && e->c2_ok;
   else if (k == "grid_per_cu") *value = e->grid_per_cu;
   else if (k == "use_blk") *value = e->use_blk && hphi_blk_ok(e);
@@ -1093,6 +1104,8 @@ int hphi_build_tables(hphi_engine *e) {
     return hphi_set_error("basis count %llu differs from idim_max %llu", (unsigned long long)total,
                           (unsigned long long)e->dim);
   e->tables_ready = true;
+  // plan of the high-half pass now, outside any H·v launch (it copies list_2_2 to the host)
+  if (hphi_hi_split_on(e) < 0) return -1;
   return 0;
 }
 
@@ -1220,7 +1233,17 @@ static int hv_launch_args(hphi_engine *e, int k, HvArgs &A, double2 *prdct_out) 
   int g = 0;
   const int path = hv_path(e, k, &g);
   if (path == HV_PATH_CANON) {
-    if (hphi_hv_launch_canon(e, A, g)) return -1;
+    // canonical Spin: the high-half terms in a second pass over a tile-major
+    // copy of v1 (only with the engine's own partial-sum buffer, which holds
+    // both grids)
+    const int split = A.partial == (double2 *)e->d_partial ? hphi_hi_split_on(e) : 0;
+    if (split < 0) return -1;
+    if (hphi_hv_launch_canon(e, A, g, split)) return -1;
+    if (split) {
+      int g2 = 0;
+      if (hphi_hv_launch_high(e, A, A.partial + g, &g2)) return -1;
+      g += g2;
+    }
   } else if (path == HV_PATH_BLK) {
     int gb = 0;
     if (hphi_hv_launch_blk(e, A, k, &gb)) return -1;

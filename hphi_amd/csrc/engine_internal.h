@@ -117,6 +117,20 @@ __device__ __forceinline__ int64_t hrank_l21(const uint16_t *t, uint32_t ia) {
          2;
 }
 
+// Canonical Spin, high-half terms by rank tiles (hv_canon.hip): for each class
+// c (low-half popcount) its blocks (high-half values, ascending), the class
+// size C(nlow, c), the block groups of HI_BPG blocks, the workgroups before
+// the class, and where the class's rank tiles start in the tile-major copy of
+// v1 (tile R of block position p at base[c] + (R nblk[c] + p) HI_RT)
+#define HI_MAXC 33
+#define HI_RT 32              // ranks per tile (8x4 J1-J2: 89.8 ms at 32, 90.5 at 16, 92.8 at 8)
+#define HI_BPG (256 / HI_RT)  // blocks per workgroup
+struct HiPlan {
+  int ncls;
+  uint32_t start[HI_MAXC], nblk[HI_MAXC], size[HI_MAXC], ngrp[HI_MAXC];
+  uint64_t wgp[HI_MAXC + 1], base[HI_MAXC + 1];
+};
+
 struct hphi_engine {
   int model = 0, nsite = 0, nup = 0, ndown = 0, nbits = 0, nlow = 0;
   uint64_t dim = 0, sdim = 0, size21 = 0, size22 = 0, irght = 0, ilft = 0, ihfbit = 0;
@@ -155,6 +169,14 @@ struct hphi_engine {
   int bk_tpb = 0, bk_wgs_per_cu = 0, use_blk = 1;
   int device = 0;  // the device current when the engine was created
   int lb_mfma = 1;  // LOBPCG block-8 Gram on the FP64 matrix cores
+  // canonical Spin: high-half terms in their own pass over a tile-major copy of v1
+  int hi_split = -1;      // -1: by sector size, 0: off, 1: on when the sector allows it
+  int hi_ok = -1;         // -1: not checked yet, 0: the sector does not allow it
+  int hi_wgs_per_cu = 6;  // workgroups per CU of that pass (8x4 J1-J2: 6 best of 4-16)
+  uint32_t *d_hb = nullptr;      // blocks of each class (hi_plan)
+  uint32_t *d_blkpos = nullptr;  // position of every block within its class (~0u: empty block)
+  double2 *d_v1t = nullptr;      // tile-major copy of v1, hi_plan.base[ncls] elements
+  HiPlan hi_plan{};
 };
 
 extern "C" int hphi_set_error(const char *fmt, ...);
@@ -176,7 +198,13 @@ int hphi_launch_tables(const SzParams &P, uint64_t sdim, uint64_t size21, uint64
 // diag[j * stride + col] for j < dim
 int hphi_launch_diag(uint64_t dim, const uint64_t *list1, int gc, const DTerm *dterms, int nd, double *diag,
                      hipStream_t st, int stride = 1, int col = 0);
-int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g);
+int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g, int skip_high = 0);
+// canonical Spin: is the high-half pass usable and wanted for this engine (host check only)
+int hphi_hi_wanted(const hphi_engine *e);
+// the same, building its plan and buffers on first use (-1: error)
+int hphi_hi_split_on(hphi_engine *e);
+// launches the high-half pass: accumulates into A.v0, partials at partial[0..*grid)
+int hphi_hv_launch_high(hphi_engine *e, const HvArgs &A, double2 *partial, int *grid);
 int hphi_blk_ok(const hphi_engine *e);
 int hphi_blk_cpl_ok(int k, int cpl);
 void hphi_blk_shape(const hphi_engine *e, int k, int *cpl, int *tpb, int *wgs_per_cu);

@@ -262,6 +262,253 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
   }
 }
 
+// ---- canonical Spin: the high-half terms over a tile-major copy of v1 -----
+// A state of a canonical Spin sector is (block ib = its high-half bits, rank r
+// of its low-half bits among the low-half values of the same popcount), at
+// index list_2_2[ib] - 1 + r.  An exchange term that flips two high-half bits
+// maps (ib, r) to (ib ^ m, r): a block of the same class (low-half popcount)
+// at the same rank.  k_hv_canon visits all ranks of a block before the next,
+// so each high term streams a whole other block and every block's rows come
+// from HBM again for each block that gathers from them.  Here the high group
+// runs in its own pass ordered by (class, tile of HI_RT ranks, group of HI_BPG
+// blocks), after a copy of v1 into tile-major order (tile R of the class's
+// block at position p at base[c] + (R nblk[c] + p) HI_RT): one rank tile of a
+// class is then one contiguous slab (HI_RT x 16 B per block: 6.6 MB for the
+// largest class of the 8x4 sector), each XCD takes a contiguous range of
+// tiles, and every high term of the tile gathers whole lines from the slab
+// while it is cached.  On the 8x4 J1-J2 sector: 101.6 ms in one pass, 89.8 ms
+// as k_hv_canon without the high group (62 ms), the copy (4.8 ms) and this
+// pass (24.6 ms at 16 ranks per tile).
+
+__device__ __forceinline__ bool hi_unit(const HiPlan &P, uint64_t u, int &c, uint32_t &R, uint32_t &grp) {
+  while (u >= P.wgp[c + 1]) c++;  // a workgroup's units only increase
+  const uint64_t loc = u - P.wgp[c];
+  R = (uint32_t)(loc / P.ngrp[c]);
+  grp = (uint32_t)(loc % P.ngrp[c]);
+  return true;
+}
+
+// v1 into tile-major order (the workgroups and units of k_hv_high)
+__global__ __launch_bounds__(256) void k_hi_copy(const double2 *__restrict__ v1, const uint64_t *__restrict__ l22,
+                                                 const HiPlan P, const uint32_t *__restrict__ hb,
+                                                 double2 *__restrict__ v1t) {
+  const uint32_t tid = threadIdx.x;
+  const uint64_t U = P.wgp[P.ncls];
+  const uint32_t xcd = blockIdx.x & 7, nslot = gridDim.x >> 3;
+  const uint64_t u1 = U * (xcd + 1) / 8;
+  int c = 0;
+  for (uint64_t u = U * xcd / 8 + (blockIdx.x >> 3); u < u1; u += nslot) {
+    uint32_t R, grp;
+    hi_unit(P, u, c, R, grp);
+    const uint32_t p = grp * HI_BPG + tid / HI_RT, rl = tid % HI_RT, r = R * HI_RT + rl;
+    if (p < P.nblk[c] && r < P.size[c]) {
+      const uint32_t ib = hb[P.start[c] + p];
+      v1t[P.base[c] + ((uint64_t)R * P.nblk[c] + p) * HI_RT + rl] = v1[l22[ib] - 1 + r];
+    }
+  }
+}
+
+template <bool REALC, int UNR>
+__global__ __launch_bounds__(256) void k_hv_high(const HvArgs A, const HiPlan P, const uint32_t *__restrict__ hb,
+                                                 const uint32_t *__restrict__ blkpos,
+                                                 const double2 *__restrict__ v1t, double2 *__restrict__ partial) {
+  const uint32_t tid = threadIdx.x;
+  double2 *__restrict__ v0 = (double2 *)A.v0;
+  const int nlow = A.nlow;
+  const CTerm th = (CTerm)A.terms + A.n_low;
+  const uint64_t U = P.wgp[P.ncls];
+  const uint32_t xcd = blockIdx.x & 7, nslot = gridDim.x >> 3;
+  const uint64_t u1 = U * (xcd + 1) / 8;
+  double pre = 0.0, pim = 0.0;
+  int c = 0;
+  for (uint64_t u = U * xcd / 8 + (blockIdx.x >> 3); u < u1; u += nslot) {
+    uint32_t R, grp;
+    hi_unit(P, u, c, R, grp);
+    const uint32_t nb = P.nblk[c];
+    const uint32_t p = grp * HI_BPG + tid / HI_RT, rl = tid % HI_RT, r = R * HI_RT + rl;
+    const bool valid = p < nb && r < P.size[c];
+    const uint32_t ib = hb[P.start[c] + (p < nb ? p : 0u)];
+    const uint32_t xh = ib << nlow;
+    const double2 *__restrict__ tile = v1t + P.base[c] + (uint64_t)R * nb * HI_RT + rl;  // this tile's slab
+    double ar = 0.0, ai = 0.0;
+    for (int t = 0; t < A.e_high; t += UNR) {
+      uint32_t q[UNR];
+#pragma unroll
+      for (int u2 = 0; u2 < UNR; u2++) {
+        const GTerm T = load_term(th + t + u2);
+        const bool act = valid & pred32(T.kind, (uint32_t)T.tmask, (uint32_t)T.tval, xh);
+        q[u2] = act ? blkpos[(xh ^ (uint32_t)T.flip) >> nlow] : ~0u;
+      }
+      double2 g[UNR];
+#pragma unroll
+      for (int u2 = 0; u2 < UNR; u2++) {
+        g[u2] = make_double2(0.0, 0.0);
+        if (q[u2] < nb) g[u2] = tile[(uint64_t)q[u2] * HI_RT];
+      }
+#pragma unroll
+      for (int u2 = 0; u2 < UNR; u2++) {
+        const GTerm T = load_term(th + t + u2);
+        double2 gg = g[u2];
+        if ((uint32_t)T.sgn != 0u && (__popc(xh & (uint32_t)T.sgn) & 1)) {
+          gg.x = -gg.x;
+          gg.y = -gg.y;
+        }
+        cmac<REALC>(ar, ai, T.cre, T.cim, gg);
+      }
+    }
+    if (valid) {  // after k_hv_canon in stream order: accumulate onto its output
+      const uint64_t j = A.l22[ib] - 1 + r;
+      const double2 a = tile[(uint64_t)p * HI_RT];
+      const double2 o = v0[j];
+      v0[j] = make_double2(o.x + A.c1 * ar, o.y + A.c1 * ai);
+      pre += a.x * ar + a.y * ai;
+      pim += a.x * ai - a.y * ar;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    pre += __shfl_xor(pre, off, 64);
+    pim += __shfl_xor(pim, off, 64);
+  }
+  __shared__ double2 red[4];
+  if ((tid & 63) == 0) red[tid >> 6] = make_double2(pre, pim);
+  __syncthreads();
+  if (tid == 0) {
+    double2 s2 = red[0];
+    for (int w = 1; w < 4; w++) {
+      s2.x += red[w].x;
+      s2.y += red[w].y;
+    }
+    partial[blockIdx.x] = s2;
+  }
+}
+
+// sectors of at least this many states take the high-half pass by default
+// (J1-J2 7x4, 40M states: 5.19 ms either way; 6x4, 2.7M: 0.29 against 0.29 ms)
+#define HI_AUTO_DIM ((uint64_t)1 << 27)
+
+int hphi_hi_wanted(const hphi_engine *e) {
+  if (e->hi_split == 0 || e->hi_ok == 0 || e->model != HPHI_MODEL_SPIN || !e->c2_ok || !e->use_c2) return 0;
+  const int nhigh = e->nbits - e->nlow;
+  if (e->r_high == 0 || e->nlow > 16 || nhigh < 1 || nhigh > 16) return 0;
+  // every high term is an exchange of two high-half bits (the block changes,
+  // the class and the rank do not); padding terms have an empty mask
+  for (int t = e->n_low; t < e->n_low + e->n_high; t++) {
+    const GTerm &x = e->h_terms_dev[t];
+    if (x.kind == P_EQ && x.tmask == 0) continue;
+    if (x.kind != P_DIFF || x.tmask != x.flip || __builtin_popcountll(x.flip) != 2 || (x.flip & e->irght) ||
+        (x.sgn & e->irght))
+      return 0;
+  }
+  return e->hi_split == 1 || e->dim >= HI_AUTO_DIM;
+}
+
+// The plan: blocks of each class from list_2_2 (non-empty blocks), checked to
+// be laid out as the pass assumes (class c blocks hold C(nlow, c) states,
+// consecutive in block order); a sector that does not match keeps the single
+// pass.
+int hphi_hi_split_on(hphi_engine *e) {
+  if (!hphi_hi_wanted(e)) return 0;
+  if (e->d_v1t) return 1;
+  const int nhigh = e->nbits - e->nlow;
+  const uint32_t nb = 1u << nhigh;
+  if (e->size22 < nb) {
+    e->hi_ok = 0;
+    return 0;
+  }
+  std::vector<uint64_t> l22(nb);
+  uint64_t x0 = 0;
+  if (hipMemcpyAsync(l22.data(), e->d_l22, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+      hipMemcpyAsync(&x0, e->d_list1, sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+      hipStreamSynchronize(e->stream) != hipSuccess)
+    return hphi_set_error("high-half pass: reading list_2_2 failed");
+  const int n1 = __builtin_popcountll(x0);  // set bits of every state of the sector
+  auto binom = [](int n, int k) -> uint64_t {
+    if (k < 0 || k > n) return 0;
+    uint64_t r = 1;
+    for (int i = 1; i <= k; i++) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
+    return r;
+  };
+  uint64_t next = 1;
+  for (uint32_t ib = 0; ib < nb; ib++) {
+    if (!l22[ib]) continue;
+    const int c = n1 - __builtin_popcount(ib);
+    if (c < 0 || c > e->nlow || l22[ib] != next) {
+      e->hi_ok = 0;
+      return 0;
+    }
+    next += binom(e->nlow, c);
+  }
+  if (next != e->dim + 1) {
+    e->hi_ok = 0;
+    return 0;
+  }
+  HiPlan P{};
+  std::vector<uint32_t> hb, pos(nb, ~0u);
+  P.ncls = e->nlow + 1;
+  uint64_t w = 0, base = 0;
+  for (int c = 0; c < P.ncls; c++) {
+    P.start[c] = (uint32_t)hb.size();
+    for (uint32_t ib = 0; ib < nb; ib++)
+      if (l22[ib] && n1 - __builtin_popcount(ib) == c) {
+        pos[ib] = (uint32_t)hb.size() - P.start[c];
+        hb.push_back(ib);
+      }
+    P.nblk[c] = (uint32_t)hb.size() - P.start[c];
+    P.size[c] = (uint32_t)binom(e->nlow, c);
+    P.ngrp[c] = (P.nblk[c] + HI_BPG - 1) / HI_BPG;
+    P.wgp[c] = w;
+    P.base[c] = base;
+    const uint64_t nr = (P.size[c] + HI_RT - 1) / HI_RT;
+    w += nr * P.ngrp[c];
+    base += nr * P.nblk[c] * HI_RT;
+  }
+  P.wgp[P.ncls] = w;
+  P.base[P.ncls] = base;
+  if (hb.empty()) {
+    e->hi_ok = 0;
+    return 0;
+  }
+  if (hipMalloc(&e->d_hb, sizeof(uint32_t) * hb.size()) != hipSuccess ||
+      hipMalloc(&e->d_blkpos, sizeof(uint32_t) * nb) != hipSuccess ||
+      hipMalloc(&e->d_v1t, sizeof(double2) * base) != hipSuccess ||
+      hipMemcpy(e->d_hb, hb.data(), sizeof(uint32_t) * hb.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_blkpos, pos.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(e->d_hb);
+    (void)hipFree(e->d_blkpos);
+    (void)hipFree(e->d_v1t);
+    e->d_hb = nullptr;
+    e->d_blkpos = nullptr;
+    e->d_v1t = nullptr;
+    return hphi_set_error("high-half pass: allocation of its tables and tile-major vector failed");
+  }
+  e->bytes += sizeof(uint32_t) * (hb.size() + nb) + sizeof(double2) * base;
+  e->hi_plan = P;
+  e->hi_ok = 1;
+  return 1;
+}
+
+int hphi_hv_launch_high(hphi_engine *e, const HvArgs &A, double2 *partial, int *grid) {
+  uint64_t g = (uint64_t)e->hi_wgs_per_cu * std::max(e->num_cu, 1);
+  g = std::max<uint64_t>((g + 7) / 8 * 8, 8);
+  if (g > HV_MAX_GRID) g = HV_MAX_GRID;
+  hipLaunchKernelGGL(k_hi_copy, dim3((unsigned)g), dim3(256), 0, e->stream, (const double2 *)A.v1, A.l22,
+                     e->hi_plan, (const uint32_t *)e->d_hb, e->d_v1t);
+  HvArgs B = A;
+  B.e_high = std::min(A.n_high, (e->r_high + 7) / 8 * 8);
+  static_assert(HV_TERM_PAD % 8 == 0, "a batch must not cross a term group");
+  if (e->c2_realc)
+    hipLaunchKernelGGL((k_hv_high<true, 8>), dim3((unsigned)g), dim3(256), 0, e->stream, B, e->hi_plan,
+                       (const uint32_t *)e->d_hb, (const uint32_t *)e->d_blkpos, (const double2 *)e->d_v1t, partial);
+  else
+    hipLaunchKernelGGL((k_hv_high<false, 8>), dim3((unsigned)g), dim3(256), 0, e->stream, B, e->hi_plan,
+                       (const uint32_t *)e->d_hb, (const uint32_t *)e->d_blkpos, (const double2 *)e->d_v1t, partial);
+  HIPCHK(hipGetLastError());
+  *grid = (int)g;
+  return 0;
+}
+
 // Which terms are real and whether the list_2_1 table fits LDS as uint16.
 int hphi_canon_config(hphi_engine *e) {
   e->c2_realc = 1;
@@ -338,7 +585,7 @@ void hphi_hrank_tables(uint16_t *t) {
 }
 
 template <int TPB, bool REALC, int RK, int UNR>
-static int launch_one(hphi_engine *e, const HvArgs &A, int g) {
+static int launch_one(hphi_engine *e, const HvArgs &A, int g, int skip_high) {
   const size_t lds = RK == 1 ? 2 * (size_t)e->size21 : RK == 2 ? 2 * (size_t)CRANK_N : RK == 3 ? 2 * (size_t)HRANK_N : 0;
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void *)k_hv_canon<TPB, REALC, RK, UNR>,
@@ -348,7 +595,7 @@ static int launch_one(hphi_engine *e, const HvArgs &A, int g) {
   static_assert(HV_TERM_PAD % UNR == 0, "a batch must not cross a term group");
   HvArgs B = A;
   B.e_low = std::min(A.n_low, (e->r_low + UNR - 1) / UNR * UNR);
-  B.e_high = std::min(A.n_high, (e->r_high + UNR - 1) / UNR * UNR);
+  B.e_high = skip_high ? 0 : std::min(A.n_high, (e->r_high + UNR - 1) / UNR * UNR);
   B.e_mixed = std::min(A.n_mixed, (e->r_mixed + UNR - 1) / UNR * UNR);
   hipLaunchKernelGGL((k_hv_canon<TPB, REALC, RK, UNR>), dim3(g), dim3(TPB), lds, e->stream, B);
   HIPCHK(hipGetLastError());
@@ -356,25 +603,25 @@ static int launch_one(hphi_engine *e, const HvArgs &A, int g) {
 }
 
 template <int TPB, bool REALC>
-static int launch_rk(hphi_engine *e, const HvArgs &A, int g) {
+static int launch_rk(hphi_engine *e, const HvArgs &A, int g, int sk) {
   const int rk = e->c2_crank ? (e->model == HPHI_MODEL_HUBBARD ? 3 : 2) : (e->c2_lds21 ? 1 : 0);
   if (e->c2_unroll == 16) {
-    if (rk == 3) return launch_one<TPB, REALC, 3, 16>(e, A, g);
-    if (rk == 2) return launch_one<TPB, REALC, 2, 16>(e, A, g);
-    if (rk == 1) return launch_one<TPB, REALC, 1, 16>(e, A, g);
-    return launch_one<TPB, REALC, 0, 16>(e, A, g);
+    if (rk == 3) return launch_one<TPB, REALC, 3, 16>(e, A, g, sk);
+    if (rk == 2) return launch_one<TPB, REALC, 2, 16>(e, A, g, sk);
+    if (rk == 1) return launch_one<TPB, REALC, 1, 16>(e, A, g, sk);
+    return launch_one<TPB, REALC, 0, 16>(e, A, g, sk);
   }
-  if (rk == 3) return launch_one<TPB, REALC, 3, 8>(e, A, g);
-  if (rk == 2) return launch_one<TPB, REALC, 2, 8>(e, A, g);
-  if (rk == 1) return launch_one<TPB, REALC, 1, 8>(e, A, g);
-  return launch_one<TPB, REALC, 0, 8>(e, A, g);
+  if (rk == 3) return launch_one<TPB, REALC, 3, 8>(e, A, g, sk);
+  if (rk == 2) return launch_one<TPB, REALC, 2, 8>(e, A, g, sk);
+  if (rk == 1) return launch_one<TPB, REALC, 1, 8>(e, A, g, sk);
+  return launch_one<TPB, REALC, 0, 8>(e, A, g, sk);
 }
 
-int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g) {
+int hphi_hv_launch_canon(hphi_engine *e, const HvArgs &A, int g, int skip_high) {
   const int tpb = e->c2_tpb;
   if (e->c2_crank && !A.crank) return hphi_set_error("rank tables are not built");
 #define SEL(TPB)                                                                   \
-  if (tpb == TPB) return e->c2_realc ? launch_rk<TPB, true>(e, A, g) : launch_rk<TPB, false>(e, A, g);
+  if (tpb == TPB) return e->c2_realc ? launch_rk<TPB, true>(e, A, g, skip_high) : launch_rk<TPB, false>(e, A, g, skip_high);
   SEL(256)
   SEL(512)
   SEL(1024)

@@ -219,7 +219,10 @@ TUNINGS = [dict(use_c2=0), dict(use_c2=1, lds21=0, tpb=256, wgs_per_cu=4),
            dict(use_c2=1, lds21=1, c2_unroll=16, tpb=1024, wgs_per_cu=1),
            dict(use_c2=1, lds21=0, c2_unroll=16, tpb=256, wgs_per_cu=4),
            dict(use_c2=1, c2_crank=1, tpb=256, wgs_per_cu=8),           # spin only: rank tables
-           dict(use_c2=1, c2_crank=1, c2_unroll=16, tpb=512, wgs_per_cu=2, xcd_map=1)]
+           dict(use_c2=1, c2_crank=1, c2_unroll=16, tpb=512, wgs_per_cu=2, xcd_map=1),
+           # canonical Spin: the high-half terms in their own pass over a tile-major copy of v1
+           dict(use_c2=1, c2_crank=1, hi_split=1), dict(use_c2=1, lds21=1, tpb=1024, wgs_per_cu=1, hi_split=1,
+                                                        hi_wgs_per_cu=2)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
@@ -240,6 +243,49 @@ def test_every_tuning_matches_oracle(A, oracle_lib, name, tuning):
     assert rel_inf(hv, ohv) <= HV_TOL
     assert prdct_ok(pr, opr, v, ohv)
     eng.close()
+
+
+@pytest.mark.parametrize("spec", [L.j1j2_square(4, 4, 1.0, 0.5), L.j1j2_ladder(9, 1.0, 0.3),
+                                  L.heisenberg_chain(18, 1.0, 0.4, two_sz=2), L.heisenberg_chain(17, 1.0, 0.2, two_sz=1)],
+                         ids=lambda s: s.name)
+def test_high_half_pass_equals_single_pass(A, spec):
+    """The two-pass H·v (high-half exchange terms over the tile-major copy of
+    v1) against the single pass on the same vector: 1e-14 relative (only the
+    order of the sums differs), prdct likewise, in accumulate and in overwrite
+    form, and the output of a second call on the same vectors unchanged."""
+    print(f"[gpu] high-half pass {spec.name}", flush=True)
+    eng = A.Engine.from_spec(spec)
+    rng = np.random.default_rng(11)
+    v = rng.standard_normal(eng.dim) + 1j * rng.standard_normal(eng.dim)
+    w = rng.standard_normal(eng.dim) + 1j * rng.standard_normal(eng.dim)
+    out = {}
+    for hs in (0, 1):
+        eng.tune(hi_split=hs)
+        assert eng.get_tune("hi_split") == hs
+        a, b = A.DeviceArray(eng.dim).upload(v), A.DeviceArray(eng.dim).upload(w)
+        p1 = eng.mltply_dev(b, a)
+        p2 = eng.mltply_dev(b, a)  # w + 2 H v
+        acc = b.download()
+        eng.mltply_dev_scaled(b, a, 0.0, 2.0, overwrite=True)  # 2 H v
+        out[hs] = (acc, b.download(), p1, p2)
+        a.free(); b.free()
+    (y0, z0, p0, q0), (y1, z1, p1, q1) = out[0], out[1]
+    assert rel_inf(y1, y0) <= 1e-14 and rel_inf(z1, z0) <= 1e-14
+    assert rel_inf(y1 - w, z1) <= 1e-13  # accumulate and overwrite forms agree
+    assert abs(p1 - p0) <= 1e-13 * abs(p0) and abs(q1 - q0) <= 1e-13 * abs(q0)
+    eng.close()
+
+
+def test_high_half_pass_is_the_default_at_full_size(A):
+    """The 8x4 J1-J2 sector takes the two-pass H·v by default (its full-size
+    checksums against the reference run through it); a small sector and the
+    Hubbard workload do not."""
+    big = A.Engine.from_spec(L.baseline_config(3))
+    small = A.Engine.from_spec(L.j1j2_square(4, 4, 1.0, 0.5))
+    hub = A.Engine.from_spec(L.baseline_config(2))
+    assert big.get_tune("hi_split") == 1 and small.get_tune("hi_split") == 0 and hub.get_tune("hi_split") == 0
+    for e in (big, small, hub):
+        e.close()
 
 
 def _extreme_eigenvalues(orc):
