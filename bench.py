@@ -259,8 +259,15 @@ def pmc_probe(args):
     eng.close()
 
 
+# kernels of one plain H·v: k_hv_canon, and for large canonical Spin sectors
+# also the tile-major copy of v1 and the high-half pass (one dispatch each)
+HV_KERNELS = ("k_hv_canon", "k_hi_copy", "k_hv_high")
+
+
 def _pmc_means(directory: str, counter: str):
-    """Mean counter value per kernel from rocprofv3 --output-format csv."""
+    """Mean counter value per dispatch of each H·v kernel and of the
+    calibration kernel from rocprofv3 --output-format csv; "k_hv" sums the H·v
+    kernels (the counter per H·v)."""
     import csv
     import glob
 
@@ -270,30 +277,34 @@ def _pmc_means(directory: str, counter: str):
             if row.get("Counter_Name") != counter:
                 continue
             name = row["Kernel_Name"]
-            key = "k_hv_canon" if "k_hv_canon" in name else ("k_axpby" if "k_axpby" in name else None)
+            key = next((k for k in HV_KERNELS + ("k_axpby",) if k in name), None)
             if key:
                 a = sums.setdefault(key, [0.0, 0])
                 a[0] += float(row["Counter_Value"])
                 a[1] += 1
-    return {k: v[0] / v[1] for k, v in sums.items() if v[1]}
+    means = {k: v[0] / v[1] for k, v in sums.items() if v[1]}
+    means["k_hv"] = sum(means.get(k, 0.0) for k in HV_KERNELS)
+    return means
 
 
 def summarize_stats(csv_path: str, out: str, title: str, cmd: str, kernel: str, balg: float):
     """Short text summary of a rocprofv3 --stats kernel_stats.csv: per kernel
-    calls and average / min / max duration; the kernel named `kernel` also gets
-    its algorithmic rate balg / average."""
+    calls and average / min / max duration.  `kernel` names the kernel (or a
+    comma-separated list of the kernels launched once each per operation,
+    whose averages add up) that gets the algorithmic rate balg / average."""
     import csv
     rows = list(csv.DictReader(open(csv_path)))
     lines = [f"# rocprofv3 kernel statistics: {title}", f"# command: {cmd}"]
-    for r in rows:
-        if kernel and kernel in r["Name"]:
-            avg = float(r["AverageNs"]) * 1e-9
-            if balg > 0:
-                lines.append(f"# {r['Name']}: average {avg * 1e3:.3f} ms -> {balg / avg / 1e12:.3f} TB/s algorithmic "
-                             f"({balg:.6e} B per launch) = {100 * balg / avg / (HBM_PEAK_GBS * 1e9):.1f} % of 8 TB/s")
-            else:  # launches of varying size (packed served slabs): the average only
-                lines.append(f"# {r['Name']}: average {avg * 1e3:.3f} ms over launches of varying size")
-            break
+    names = [k for k in kernel.split(",") if k] if kernel else []
+    hit = [(k, r) for k in names for r in rows if k in r["Name"]]
+    if hit:
+        avg = sum(float(r["AverageNs"]) for _, r in hit) * 1e-9
+        label = " + ".join(r["Name"] for _, r in hit)
+        if balg > 0:
+            lines.append(f"# {label}: average {avg * 1e3:.3f} ms -> {balg / avg / 1e12:.3f} TB/s algorithmic "
+                         f"({balg:.6e} B per launch) = {100 * balg / avg / (HBM_PEAK_GBS * 1e9):.1f} % of 8 TB/s")
+        else:  # launches of varying size (packed served slabs): the average only
+            lines.append(f"# {label}: average {avg * 1e3:.3f} ms over launches of varying size")
     lines.append("")
     lines.append(f"{'kernel':<72} {'calls':>6} {'avg_ms':>10} {'min_ms':>10} {'max_ms':>10} {'pct':>6}")
     for r in rows:
@@ -316,16 +327,17 @@ def reduce_pmc(fetch_dir: str, write_dir: str, out: str, cmd: str, cfg: int = 3)
     f, w = _pmc_means(fetch_dir, "FETCH_SIZE"), _pmc_means(write_dir, "WRITE_SIZE")
     ff = (32 * dim / 1024) / f["k_axpby"]
     fw = (16 * dim / 1024) / w["k_axpby"]
-    fetch_b = f["k_hv_canon"] * 1024 * ff
-    write_b = w["k_hv_canon"] * 1024 * fw
+    fetch_b = f["k_hv"] * 1024 * ff
+    write_b = w["k_hv"] * 1024 * fw
     nnz = plain_nnz(spec)
+    used = [k for k in HV_KERNELS if k in f]
     lines = [
-        f"# HBM traffic of one plain H·v launch (k_hv_canon), {PMC_WORKLOADS[cfg][1]}",
+        f"# HBM traffic of one plain H·v ({' + '.join(used)}), {PMC_WORKLOADS[cfg][1]}",
         f"# command: {cmd}",
-        f"FETCH_SIZE k_hv_canon (KiB/dispatch, raw): {f['k_hv_canon']:.1f}",
+        *[f"FETCH_SIZE {k} (KiB/dispatch, raw): {f[k]:.1f}" for k in used],
         f"FETCH_SIZE k_axpby calibration (KiB/dispatch, raw): {f['k_axpby']:.1f}  known: {32 * dim / 1024:.1f}"
         f"  factor: {ff:.4f}",
-        f"WRITE_SIZE k_hv_canon (KiB/dispatch, raw): {w['k_hv_canon']:.1f}",
+        *[f"WRITE_SIZE {k} (KiB/dispatch, raw): {w[k]:.1f}" for k in used if k in w],
         f"WRITE_SIZE k_axpby calibration (KiB/dispatch, raw): {w['k_axpby']:.1f}  known: {16 * dim / 1024:.1f}"
         f"  factor: {fw:.4f}",
         f"fetch_bytes_per_launch: {fetch_b:.6e}",
@@ -690,7 +702,7 @@ def main():
                     help="write a text summary of a rocprofv3 --stats run (with --stats-cmd, --stats-title)")
     ap.add_argument("--stats-cmd", type=str, default="")
     ap.add_argument("--stats-title", type=str, default="plain H·v on the J1-J2 8x4 Sz=0 sector (dim 601,080,390)")
-    ap.add_argument("--stats-kernel", type=str, default="k_hv_canon")
+    ap.add_argument("--stats-kernel", type=str, default=",".join(HV_KERNELS))
     ap.add_argument("--stats-balg", type=float, default=float(b_alg(601080390, 39710085120)),
                     help="algorithmic bytes per launch of --stats-kernel (0: report the average only)")
     ap.add_argument("--serve", action="store_true", help="served-stream benchmark instead of plain H·v")
