@@ -117,12 +117,13 @@ __device__ __forceinline__ int64_t hrank_l21(const uint16_t *t, uint32_t ia) {
          2;
 }
 
-// Canonical Spin, high-half terms by rank tiles (hv_canon.hip): for each class
-// c (low-half popcount) its blocks (high-half values, ascending), the class
-// size C(nlow, c), the block groups of HI_BPG blocks, the workgroups before
-// the class, and where the class's rank tiles start in the tile-major copy of
-// v1 (tile R of block position p at base[c] + (R nblk[c] + p) HI_RT)
-#define HI_MAXC 33
+// Canonical Spin and Hubbard, high-half terms by rank tiles (hv_canon.hip):
+// for each class c (blocks whose high half holds the same number of up
+// spins, or of up and of down electrons) its blocks (high-half values,
+// ascending), the block size, the block groups of HI_BPG blocks, the
+// workgroups before the class, and where the class's rank tiles start in the
+// tile-major copy of v1 (tile R of block position p at base[c] + (R nblk[c] + p) HI_RT)
+#define HI_MAXC 96
 #define HI_RT 32              // ranks per tile (8x4 J1-J2: 89.8 ms at 32, 90.5 at 16, 92.8 at 8)
 #define HI_BPG (256 / HI_RT)  // blocks per workgroup
 struct HiPlan {
@@ -169,7 +170,7 @@ struct hphi_engine {
   int bk_tpb = 0, bk_wgs_per_cu = 0, use_blk = 1;
   int device = 0;  // the device current when the engine was created
   int lb_mfma = 1;  // LOBPCG block-8 Gram on the FP64 matrix cores
-  // canonical Spin: high-half terms in their own pass over a tile-major copy of v1
+  // canonical Spin and Hubbard: high-half terms in their own pass over a tile-major copy of v1
   int hi_split = -1;      // -1: by sector size, 0: off, 1: on when the sector allows it
   int hi_ok = -1;         // -1: not checked yet, 0: the sector does not allow it
   int hi_wgs_per_cu = 6;  // workgroups per CU of that pass (8x4 J1-J2: 6 best of 4-16)

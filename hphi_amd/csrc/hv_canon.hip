@@ -14,6 +14,8 @@
 //  * state words are 32-bit; real couplings skip the imaginary products.
 #include <hip/hip_runtime.h>
 
+#include <map>
+
 #include "engine_internal.h"
 
 #define HIPCHK(x)                                                                  \
@@ -387,26 +389,34 @@ __global__ __launch_bounds__(256) void k_hv_high(const HvArgs A, const HiPlan P,
 // (J1-J2 7x4, 40M states: 5.19 ms either way; 6x4, 2.7M: 0.29 against 0.29 ms)
 #define HI_AUTO_DIM ((uint64_t)1 << 27)
 
+// Hubbard orbitals 2i + sigma: up electrons at even bit positions
+#define HI_EVEN 0x5555555555555555ull
+
 int hphi_hi_wanted(const hphi_engine *e) {
-  if (e->hi_split == 0 || e->hi_ok == 0 || e->model != HPHI_MODEL_SPIN || !e->c2_ok || !e->use_c2) return 0;
+  const bool hub = e->model == HPHI_MODEL_HUBBARD;
+  if (e->hi_split == 0 || e->hi_ok == 0 || !(e->model == HPHI_MODEL_SPIN || hub) || !e->c2_ok || !e->use_c2)
+    return 0;
   const int nhigh = e->nbits - e->nlow;
   if (e->r_high == 0 || e->nlow > 16 || nhigh < 1 || nhigh > 16) return 0;
-  // every high term is an exchange of two high-half bits (the block changes,
-  // the class and the rank do not); padding terms have an empty mask
+  // every high term moves one particle (spin) between two high-half bits of
+  // one species, so the block changes and its class and the rank do not:
+  // an exchange or hop pair (P_DIFF) or one hop (P_EQ: exactly one of the two
+  // bits set); padding terms have an empty mask
   for (int t = e->n_low; t < e->n_low + e->n_high; t++) {
     const GTerm &x = e->h_terms_dev[t];
     if (x.kind == P_EQ && x.tmask == 0) continue;
-    if (x.kind != P_DIFF || x.tmask != x.flip || __builtin_popcountll(x.flip) != 2 || (x.flip & e->irght) ||
-        (x.sgn & e->irght))
-      return 0;
+    const bool two = x.tmask == x.flip && __builtin_popcountll(x.flip) == 2;
+    const bool kind_ok = x.kind == P_DIFF || (x.kind == P_EQ && __builtin_popcountll(x.tval) == 1);
+    const bool species = !hub || __builtin_popcountll(x.flip & HI_EVEN) != 1;
+    if (!two || !kind_ok || !species || (x.flip & e->irght) || (x.sgn & e->irght)) return 0;
   }
-  return e->hi_split == 1 || e->dim >= HI_AUTO_DIM;
+  // by default Spin only: on the 4x4 Hubbard sector the two passes took 14.9 ms against 14.0
+  return e->hi_split == 1 || (!hub && e->dim >= HI_AUTO_DIM);
 }
 
-// The plan: blocks of each class from list_2_2 (non-empty blocks), checked to
-// be laid out as the pass assumes (class c blocks hold C(nlow, c) states,
-// consecutive in block order); a sector that does not match keeps the single
-// pass.
+// The plan: the non-empty blocks from list_2_2 grouped by class, checked to be
+// laid out as the pass assumes (consecutive in block order, every block of a
+// class the same size); a sector that does not match keeps the single pass.
 int hphi_hi_split_on(hphi_engine *e) {
   if (!hphi_hi_wanted(e)) return 0;
   if (e->d_v1t) return 1;
@@ -417,58 +427,65 @@ int hphi_hi_split_on(hphi_engine *e) {
     return 0;
   }
   std::vector<uint64_t> l22(nb);
-  uint64_t x0 = 0;
   if (hipMemcpyAsync(l22.data(), e->d_l22, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
-      hipMemcpyAsync(&x0, e->d_list1, sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
       hipStreamSynchronize(e->stream) != hipSuccess)
     return hphi_set_error("high-half pass: reading list_2_2 failed");
-  const int n1 = __builtin_popcountll(x0);  // set bits of every state of the sector
-  auto binom = [](int n, int k) -> uint64_t {
-    if (k < 0 || k > n) return 0;
-    uint64_t r = 1;
-    for (int i = 1; i <= k; i++) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
-    return r;
+  const bool hub = e->model == HPHI_MODEL_HUBBARD;
+  const uint64_t even = HI_EVEN >> (e->nlow & 1);  // up positions of the high half's own bits
+  auto key = [&](uint32_t ib) -> uint32_t {
+    return hub ? (uint32_t)(__builtin_popcountll(ib & even) * 64 + __builtin_popcountll(ib & ~even & (nb - 1)))
+               : (uint32_t)__builtin_popcount(ib);
   };
-  uint64_t next = 1;
-  for (uint32_t ib = 0; ib < nb; ib++) {
-    if (!l22[ib]) continue;
-    const int c = n1 - __builtin_popcount(ib);
-    if (c < 0 || c > e->nlow || l22[ib] != next) {
+  // block sizes from consecutive offsets (non-empty blocks are consecutive in block order)
+  std::vector<uint32_t> ne;
+  for (uint32_t ib = 0; ib < nb; ib++)
+    if (l22[ib]) ne.push_back(ib);
+  if (ne.empty() || l22[ne[0]] != 1) {
+    e->hi_ok = 0;
+    return 0;
+  }
+  std::map<uint32_t, uint64_t> csize;  // class key -> block size
+  for (size_t i = 0; i < ne.size(); i++) {
+    const uint64_t end = i + 1 < ne.size() ? l22[ne[i + 1]] : e->dim + 1;
+    if (end <= l22[ne[i]]) {
       e->hi_ok = 0;
       return 0;
     }
-    next += binom(e->nlow, c);
+    const uint64_t sz = end - l22[ne[i]];
+    auto it = csize.emplace(key(ne[i]), sz).first;
+    if (it->second != sz || sz > 0xffffffffull) {
+      e->hi_ok = 0;
+      return 0;
+    }
   }
-  if (next != e->dim + 1) {
+  if (csize.size() > HI_MAXC) {
     e->hi_ok = 0;
     return 0;
   }
   HiPlan P{};
   std::vector<uint32_t> hb, pos(nb, ~0u);
-  P.ncls = e->nlow + 1;
+  P.ncls = (int)csize.size();
   uint64_t w = 0, base = 0;
-  for (int c = 0; c < P.ncls; c++) {
+  int c = 0;
+  for (const auto &kv : csize) {
     P.start[c] = (uint32_t)hb.size();
-    for (uint32_t ib = 0; ib < nb; ib++)
-      if (l22[ib] && n1 - __builtin_popcount(ib) == c) {
+    for (uint32_t ib : ne)
+      if (key(ib) == kv.first) {
         pos[ib] = (uint32_t)hb.size() - P.start[c];
         hb.push_back(ib);
       }
     P.nblk[c] = (uint32_t)hb.size() - P.start[c];
-    P.size[c] = (uint32_t)binom(e->nlow, c);
+    P.size[c] = (uint32_t)kv.second;
     P.ngrp[c] = (P.nblk[c] + HI_BPG - 1) / HI_BPG;
     P.wgp[c] = w;
     P.base[c] = base;
     const uint64_t nr = (P.size[c] + HI_RT - 1) / HI_RT;
     w += nr * P.ngrp[c];
     base += nr * P.nblk[c] * HI_RT;
+    c++;
   }
   P.wgp[P.ncls] = w;
   P.base[P.ncls] = base;
-  if (hb.empty()) {
-    e->hi_ok = 0;
-    return 0;
-  }
   if (hipMalloc(&e->d_hb, sizeof(uint32_t) * hb.size()) != hipSuccess ||
       hipMalloc(&e->d_blkpos, sizeof(uint32_t) * nb) != hipSuccess ||
       hipMalloc(&e->d_v1t, sizeof(double2) * base) != hipSuccess ||

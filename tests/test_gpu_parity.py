@@ -246,13 +246,15 @@ def test_every_tuning_matches_oracle(A, oracle_lib, name, tuning):
 
 
 @pytest.mark.parametrize("spec", [L.j1j2_square(4, 4, 1.0, 0.5), L.j1j2_ladder(9, 1.0, 0.3),
-                                  L.heisenberg_chain(18, 1.0, 0.4, two_sz=2), L.heisenberg_chain(17, 1.0, 0.2, two_sz=1)],
+                                  L.heisenberg_chain(18, 1.0, 0.4, two_sz=2), L.heisenberg_chain(17, 1.0, 0.2, two_sz=1),
+                                  L.hubbard_square(4, 2, 1.0, 4.0), L.hubbard_chain(10, 1.0, 2.0, ncond=9),
+                                  L.hubbard_square(3, 3, 1.0, 8.0, two_sz=1)],
                          ids=lambda s: s.name)
 def test_high_half_pass_equals_single_pass(A, spec):
-    """The two-pass H·v (high-half exchange terms over the tile-major copy of
-    v1) against the single pass on the same vector: 1e-14 relative (only the
-    order of the sums differs), prdct likewise, in accumulate and in overwrite
-    form, and the output of a second call on the same vectors unchanged."""
+    """The two-pass H·v (high-half exchange terms or hops over the tile-major
+    copy of v1) against the single pass on the same vector: 1e-14 relative
+    (only the order of the sums differs), prdct likewise, in accumulate and in
+    overwrite form."""
     print(f"[gpu] high-half pass {spec.name}", flush=True)
     eng = A.Engine.from_spec(spec)
     rng = np.random.default_rng(11)
@@ -279,7 +281,7 @@ def test_high_half_pass_equals_single_pass(A, spec):
 def test_high_half_pass_is_the_default_at_full_size(A):
     """The 8x4 J1-J2 sector takes the two-pass H·v by default (its full-size
     checksums against the reference run through it); a small sector and the
-    Hubbard workload do not."""
+    4x4 Hubbard sector (slower in two passes) do not."""
     big = A.Engine.from_spec(L.baseline_config(3))
     small = A.Engine.from_spec(L.j1j2_square(4, 4, 1.0, 0.5))
     hub = A.Engine.from_spec(L.baseline_config(2))
