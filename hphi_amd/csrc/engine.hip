@@ -922,6 +922,7 @@ extern "C" void hphi_engine_destroy(hphi_engine *e) {
   (void)hipFree(e->d_hb);
   (void)hipFree(e->d_blkpos);
   (void)hipFree(e->d_v1t);
+  (void)hipFree(e->d_tofs);
   (void)hipFree(e->d_jb);
   (void)hipFree(e->d_diag);
   (void)hipFree(e->d_terms);
@@ -988,6 +989,8 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
   } else if (k == "hi_split") {
     if (value < -1 || value > 1) return hphi_set_error("hi_split must be -1, 0 or 1");
     e->hi_split = value;
+  } else if (k == "hi_fuse_copy") {
+    e->hi_fuse_copy = value ? 1 : 0;
   } else if (k == "hi_wgs_per_cu") {
     if (value < 1 || value > 64) return hphi_set_error("hi_wgs_per_cu out of range");
     e->hi_wgs_per_cu = value;
@@ -1032,6 +1035,7 @@ extern "C" int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int 
   else if (k == "xcd_map") *value = e->xcd_map;
   else if (k == "hi_split") *value = hphi_hi_wanted(e);
   else if (k == "hi_wgs_per_cu") *value = e->hi_wgs_per_cu;
+  else if (k == "hi_fuse_copy") *value = e->hi_fuse_copy;
   else if (k == "use_c2") *value = e->use_c2 && e->c2_ok;
   else if (k == "grid_per_cu") *value = e->grid_per_cu;
   else if (k == "use_blk") *value = e->use_blk && hphi_blk_ok(e);
@@ -1238,6 +1242,10 @@ static int hv_launch_args(hphi_engine *e, int k, HvArgs &A, double2 *prdct_out) 
     // both grids)
     const int split = A.partial == (double2 *)e->d_partial ? hphi_hi_split_on(e) : 0;
     if (split < 0) return -1;
+    if (split && e->hi_fuse_copy) {  // the first pass also writes the tile-major copy of v1
+      A.v1t = e->d_v1t;
+      A.tofs = e->d_tofs;
+    }
     if (hphi_hv_launch_canon(e, A, g, split)) return -1;
     if (split) {
       int g2 = 0;

@@ -95,6 +95,10 @@ struct HvArgs {
   // canonical Spin: list_2_1[ia] = rank of ia among the values of its popcount
   // + 2, from two byte tables (hphi_crank_tables)
   const uint16_t *crank;
+  // k_hv_canon also writes the tile-major copy of v1 for the high-half pass
+  // when v1t is set: state (block ib, rank r) at tofs[ib].x + (r / HI_RT) tofs[ib].y + r % HI_RT
+  double2 *v1t;
+  const ulonglong2 *tofs;
 };
 
 // byte tables of the combinatorial rank (spin, nlow <= 16): rank(ia) =
@@ -174,9 +178,11 @@ struct hphi_engine {
   int hi_split = -1;      // -1: by sector size, 0: off, 1: on when the sector allows it
   int hi_ok = -1;         // -1: not checked yet, 0: the sector does not allow it
   int hi_wgs_per_cu = 6;  // workgroups per CU of that pass (8x4 J1-J2: 6 best of 4-16)
+  int hi_fuse_copy = 1;   // the first pass writes the tile-major copy (0: a copy kernel does)
   uint32_t *d_hb = nullptr;      // blocks of each class (hi_plan)
   uint32_t *d_blkpos = nullptr;  // position of every block within its class (~0u: empty block)
   double2 *d_v1t = nullptr;      // tile-major copy of v1, hi_plan.base[ncls] elements
+  ulonglong2 *d_tofs = nullptr;  // per block: its first tile-major element, and the stride between its tiles
   HiPlan hi_plan{};
 };
 

@@ -100,6 +100,11 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
     const double d = A.diag[jj];
     double ar = d * a.x, ai = d * a.y;
     const int64_t b21 = r21<RK>(A, s21, ia);
+    if (A.v1t && valid) {  // tile-major copy for the high-half pass (block ib, rank b21 - 2)
+      const ulonglong2 ts = A.tofs[ib];
+      const uint64_t r = (uint64_t)(b21 - 2);
+      A.v1t[ts.x + (r / HI_RT) * ts.y + r % HI_RT] = a;
+    }
 
     // ---- low-half terms: target in the same block, per-lane rank lookup
     for (int t = 0; t < A.e_low; t += UNR) {
@@ -464,6 +469,7 @@ int hphi_hi_split_on(hphi_engine *e) {
   }
   HiPlan P{};
   std::vector<uint32_t> hb, pos(nb, ~0u);
+  std::vector<ulonglong2> tofs(nb, make_ulonglong2(0, 0));
   P.ncls = (int)csize.size();
   uint64_t w = 0, base = 0;
   int c = 0;
@@ -476,6 +482,8 @@ int hphi_hi_split_on(hphi_engine *e) {
       }
     P.nblk[c] = (uint32_t)hb.size() - P.start[c];
     P.size[c] = (uint32_t)kv.second;
+    for (uint32_t q = 0; q < P.nblk[c]; q++)
+      tofs[hb[P.start[c] + q]] = make_ulonglong2(base + (uint64_t)q * HI_RT, (uint64_t)P.nblk[c] * HI_RT);
     P.ngrp[c] = (P.nblk[c] + HI_BPG - 1) / HI_BPG;
     P.wgp[c] = w;
     P.base[c] = base;
@@ -489,18 +497,22 @@ int hphi_hi_split_on(hphi_engine *e) {
   if (hipMalloc(&e->d_hb, sizeof(uint32_t) * hb.size()) != hipSuccess ||
       hipMalloc(&e->d_blkpos, sizeof(uint32_t) * nb) != hipSuccess ||
       hipMalloc(&e->d_v1t, sizeof(double2) * base) != hipSuccess ||
+      hipMalloc(&e->d_tofs, sizeof(ulonglong2) * nb) != hipSuccess ||
+      hipMemcpy(e->d_tofs, tofs.data(), sizeof(ulonglong2) * nb, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->d_hb, hb.data(), sizeof(uint32_t) * hb.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->d_blkpos, pos.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipFree(e->d_hb);
     (void)hipFree(e->d_blkpos);
     (void)hipFree(e->d_v1t);
+    (void)hipFree(e->d_tofs);
     e->d_hb = nullptr;
     e->d_blkpos = nullptr;
     e->d_v1t = nullptr;
+    e->d_tofs = nullptr;
     return hphi_set_error("high-half pass: allocation of its tables and tile-major vector failed");
   }
-  e->bytes += sizeof(uint32_t) * (hb.size() + nb) + sizeof(double2) * base;
+  e->bytes += sizeof(uint32_t) * (hb.size() + nb) + sizeof(double2) * base + sizeof(ulonglong2) * nb;
   e->hi_plan = P;
   e->hi_ok = 1;
   return 1;
@@ -510,8 +522,9 @@ int hphi_hv_launch_high(hphi_engine *e, const HvArgs &A, double2 *partial, int *
   uint64_t g = (uint64_t)e->hi_wgs_per_cu * std::max(e->num_cu, 1);
   g = std::max<uint64_t>((g + 7) / 8 * 8, 8);
   if (g > HV_MAX_GRID) g = HV_MAX_GRID;
-  hipLaunchKernelGGL(k_hi_copy, dim3((unsigned)g), dim3(256), 0, e->stream, (const double2 *)A.v1, A.l22,
-                     e->hi_plan, (const uint32_t *)e->d_hb, e->d_v1t);
+  if (!A.v1t)  // the first pass did not write the tile-major copy
+    hipLaunchKernelGGL(k_hi_copy, dim3((unsigned)g), dim3(256), 0, e->stream, (const double2 *)A.v1, A.l22,
+                       e->hi_plan, (const uint32_t *)e->d_hb, e->d_v1t);
   HvArgs B = A;
   B.e_high = std::min(A.n_high, (e->r_high + 7) / 8 * 8);
   static_assert(HV_TERM_PAD % 8 == 0, "a batch must not cross a term group");

@@ -221,8 +221,8 @@ TUNINGS = [dict(use_c2=0), dict(use_c2=1, lds21=0, tpb=256, wgs_per_cu=4),
            dict(use_c2=1, c2_crank=1, tpb=256, wgs_per_cu=8),           # spin only: rank tables
            dict(use_c2=1, c2_crank=1, c2_unroll=16, tpb=512, wgs_per_cu=2, xcd_map=1),
            # canonical Spin: the high-half terms in their own pass over a tile-major copy of v1
-           dict(use_c2=1, c2_crank=1, hi_split=1), dict(use_c2=1, lds21=1, tpb=1024, wgs_per_cu=1, hi_split=1,
-                                                        hi_wgs_per_cu=2)]
+           dict(use_c2=1, c2_crank=1, hi_split=1), dict(use_c2=1, c2_crank=1, hi_split=1, hi_fuse_copy=0),
+           dict(use_c2=1, lds21=1, tpb=1024, wgs_per_cu=1, hi_split=1, hi_wgs_per_cu=2)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
