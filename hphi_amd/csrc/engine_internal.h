@@ -128,7 +128,7 @@ __device__ __forceinline__ int64_t hrank_l21(const uint16_t *t, uint32_t ia) {
 // workgroups before the class, and where the class's rank tiles start in the
 // tile-major copy of v1 (tile R of block position p at base[c] + (R nblk[c] + p) HI_RT)
 #define HI_MAXC 96
-#define HI_RT 32              // ranks per tile (8x4 J1-J2: 89.8 ms at 32, 90.5 at 16, 92.8 at 8)
+#define HI_RT 32              // ranks per tile (8x4 J1-J2: 89.8 ms at 32, 90.5 at 16, 92.8 at 8; 64 no better)
 #define HI_BPG (256 / HI_RT)  // blocks per workgroup
 struct HiPlan {
   int ncls;

@@ -390,6 +390,8 @@ __global__ __launch_bounds__(256) void k_hv_high(const HvArgs A, const HiPlan P,
   }
 }
 
+#define HI_UNR 8  // high terms per batch of the high-half pass (16: 95-101 ms against 93.5 on one box)
+
 // sectors of at least this many states take the high-half pass by default
 // (J1-J2 7x4, 40M states: 5.19 ms either way; 6x4, 2.7M: 0.29 against 0.29 ms)
 #define HI_AUTO_DIM ((uint64_t)1 << 27)
@@ -526,13 +528,13 @@ int hphi_hv_launch_high(hphi_engine *e, const HvArgs &A, double2 *partial, int *
     hipLaunchKernelGGL(k_hi_copy, dim3((unsigned)g), dim3(256), 0, e->stream, (const double2 *)A.v1, A.l22,
                        e->hi_plan, (const uint32_t *)e->d_hb, e->d_v1t);
   HvArgs B = A;
-  B.e_high = std::min(A.n_high, (e->r_high + 7) / 8 * 8);
-  static_assert(HV_TERM_PAD % 8 == 0, "a batch must not cross a term group");
+  B.e_high = std::min(A.n_high, (e->r_high + HI_UNR - 1) / HI_UNR * HI_UNR);
+  static_assert(HV_TERM_PAD % HI_UNR == 0, "a batch must not cross a term group");
   if (e->c2_realc)
-    hipLaunchKernelGGL((k_hv_high<true, 8>), dim3((unsigned)g), dim3(256), 0, e->stream, B, e->hi_plan,
+    hipLaunchKernelGGL((k_hv_high<true, HI_UNR>), dim3((unsigned)g), dim3(256), 0, e->stream, B, e->hi_plan,
                        (const uint32_t *)e->d_hb, (const uint32_t *)e->d_blkpos, (const double2 *)e->d_v1t, partial);
   else
-    hipLaunchKernelGGL((k_hv_high<false, 8>), dim3((unsigned)g), dim3(256), 0, e->stream, B, e->hi_plan,
+    hipLaunchKernelGGL((k_hv_high<false, HI_UNR>), dim3((unsigned)g), dim3(256), 0, e->stream, B, e->hi_plan,
                        (const uint32_t *)e->d_hb, (const uint32_t *)e->d_blkpos, (const double2 *)e->d_v1t, partial);
   HIPCHK(hipGetLastError());
   *grid = (int)g;
