@@ -966,7 +966,7 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
     if (value != 256 && value != 512 && value != 1024) return hphi_set_error("tpb must be 256, 512 or 1024");
     e->c2_tpb = value;
   } else if (k == "wgs_per_cu") {
-    if (value < 1 || value > 64) return hphi_set_error("wgs_per_cu out of range");
+    if (value < 1 || value > 256) return hphi_set_error("wgs_per_cu out of range");  // grids cap at HV_MAX_GRID
     e->c2_wgs_per_cu = value;
   } else if (k == "lds21") {
     if (value) {
@@ -989,6 +989,9 @@ extern "C" int hphi_engine_tune(hphi_engine *e, const char *knob, int value) {
   } else if (k == "hi_split") {
     if (value < -1 || value > 1) return hphi_set_error("hi_split must be -1, 0 or 1");
     e->hi_split = value;
+  } else if (k == "hi_p1_wgs_per_cu") {
+    if (value < 1 || value > 256) return hphi_set_error("hi_p1_wgs_per_cu out of range");
+    e->hi_p1_wgs_per_cu = value;
   } else if (k == "hi_fuse_copy") {
     e->hi_fuse_copy = value ? 1 : 0;
   } else if (k == "hi_wgs_per_cu") {
@@ -1036,6 +1039,7 @@ extern "C" int hphi_engine_get_tune(const hphi_engine *e, const char *knob, int 
   else if (k == "hi_split") *value = hphi_hi_wanted(e);
   else if (k == "hi_wgs_per_cu") *value = e->hi_wgs_per_cu;
   else if (k == "hi_fuse_copy") *value = e->hi_fuse_copy;
+  else if (k == "hi_p1_wgs_per_cu") *value = e->hi_p1_wgs_per_cu;
   else if (k == "use_c2") *value = e->use_c2 && e->c2_ok;
   else if (k == "grid_per_cu") *value = e->grid_per_cu;
   else if (k == "use_blk") *value = e->use_blk && hphi_blk_ok(e);
@@ -1193,9 +1197,10 @@ int hphi_hv_grid(const hphi_engine *e, int k) {
 }
 
 // launches H·v and the final reduction on e->stream; result in e->d_prdct[0..k)
-static int canon_grid(const hphi_engine *e) {
+static int canon_grid(const hphi_engine *e, int wgs_per_cu = 0) {
   uint64_t ntiles = (e->dim + e->c2_tpb - 1) / e->c2_tpb;
-  uint64_t g = std::min<uint64_t>(ntiles, (uint64_t)e->c2_wgs_per_cu * std::max(e->num_cu, 1));
+  const uint64_t w = (uint64_t)(wgs_per_cu > 0 ? wgs_per_cu : e->c2_wgs_per_cu);
+  uint64_t g = std::min<uint64_t>(ntiles, w * std::max(e->num_cu, 1));
   g = std::max<uint64_t>(g, 8);
   g = (g + 7) / 8 * 8;
   if (g > HV_MAX_GRID) g = HV_MAX_GRID;
@@ -1246,6 +1251,7 @@ static int hv_launch_args(hphi_engine *e, int k, HvArgs &A, double2 *prdct_out) 
       A.v1t = e->d_v1t;
       A.tofs = e->d_tofs;
     }
+    if (split) g = canon_grid(e, e->hi_p1_wgs_per_cu);  // the first pass without the high terms
     if (hphi_hv_launch_canon(e, A, g, split)) return -1;
     if (split) {
       int g2 = 0;

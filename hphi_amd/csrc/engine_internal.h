@@ -179,6 +179,7 @@ struct hphi_engine {
   int hi_ok = -1;         // -1: not checked yet, 0: the sector does not allow it
   int hi_wgs_per_cu = 6;  // workgroups per CU of that pass (8x4 J1-J2: 6 best of 4-16)
   int hi_fuse_copy = 1;   // the first pass writes the tile-major copy (0: a copy kernel does)
+  int hi_p1_wgs_per_cu = 256;  // workgroups per CU of that first pass (8x4 J1-J2: 90.4 ms at 256, 93.6 at 48)
   uint32_t *d_hb = nullptr;      // blocks of each class (hi_plan)
   uint32_t *d_blkpos = nullptr;  // position of every block within its class (~0u: empty block)
   double2 *d_v1t = nullptr;      // tile-major copy of v1, hi_plan.base[ncls] elements
