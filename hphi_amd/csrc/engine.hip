@@ -893,6 +893,7 @@ extern "C" int hphi_engine_create(const hphi_model_desc *d, hphi_engine **out) {
     e->c2_tpb = 256;
     e->c2_wgs_per_cu = 48;  // 8x4 J1-J2: 107.3 ms vs 138.9 ms with the LDS table, grids of 40-64 workgroups
                             // per CU within 1 %; 4x4 Hubbard: 20.1 ms vs 27.3 ms
+    if (e->model == HPHI_MODEL_HUBBARD) e->c2_wgs_per_cu = 160;  // 4x4 Hubbard: 14.42 ms against 14.98 at 48
     e->c2_unroll = 8;
     e->xcd_map = e->model == HPHI_MODEL_SPIN ? 1 : 0;
   } else if (e->c2_lds21) {  // list_2_1 in LDS: one 1024-thread workgroup per CU
