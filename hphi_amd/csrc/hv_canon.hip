@@ -269,23 +269,25 @@ __global__ __launch_bounds__(TPB) void k_hv_canon(const HvArgs A) {
   }
 }
 
-// ---- canonical Spin: the high-half terms over a tile-major copy of v1 -----
-// A state of a canonical Spin sector is (block ib = its high-half bits, rank r
-// of its low-half bits among the low-half values of the same popcount), at
-// index list_2_2[ib] - 1 + r.  An exchange term that flips two high-half bits
-// maps (ib, r) to (ib ^ m, r): a block of the same class (low-half popcount)
-// at the same rank.  k_hv_canon visits all ranks of a block before the next,
-// so each high term streams a whole other block and every block's rows come
-// from HBM again for each block that gathers from them.  Here the high group
-// runs in its own pass ordered by (class, tile of HI_RT ranks, group of HI_BPG
-// blocks), after a copy of v1 into tile-major order (tile R of the class's
-// block at position p at base[c] + (R nblk[c] + p) HI_RT): one rank tile of a
+// ---- canonical Spin (and Hubbard): the high-half terms over a tile-major copy of v1
+// A state of a canonical sector is (block ib = its high-half bits, rank r of
+// its low-half bits among the low-half values of the same class), at index
+// list_2_2[ib] - 1 + r.  A term that moves one spin (or one electron) between
+// two high-half bits maps (ib, r) to (ib ^ m, r): a block of the same class at
+// the same rank.  k_hv_canon visits all ranks of a block before the next, so
+// each high term streams a whole other block and every block's rows come from
+// HBM again for each block that gathers from them.  Here the high group runs
+// in its own pass ordered by (class, tile of HI_RT ranks, group of HI_BPG
+// blocks), over a copy of v1 in tile-major order (tile R of the class's block
+// at position p at base[c] + (R nblk[c] + p) HI_RT; k_hv_canon writes it as
+// it reads each state's own entry, or k_hi_copy does): one rank tile of a
 // class is then one contiguous slab (HI_RT x 16 B per block: 6.6 MB for the
 // largest class of the 8x4 sector), each XCD takes a contiguous range of
 // tiles, and every high term of the tile gathers whole lines from the slab
-// while it is cached.  On the 8x4 J1-J2 sector: 101.6 ms in one pass, 89.8 ms
+// while it is cached.  On the 8x4 J1-J2 sector: 101.6 ms in one pass; 89.8 ms
 // as k_hv_canon without the high group (62 ms), the copy (4.8 ms) and this
-// pass (24.6 ms at 16 ranks per tile).
+// pass (23 ms); 90.0 ms with the copy fused and the first pass on a larger
+// grid, on a box where the single pass takes 107.4 ms.
 
 __device__ __forceinline__ bool hi_unit(const HiPlan &P, uint64_t u, int &c, uint32_t &R, uint32_t &grp) {
   while (u >= P.wgp[c + 1]) c++;  // a workgroup's units only increase
