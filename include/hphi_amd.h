@@ -106,9 +106,13 @@ void hphi_model_desc_free(hphi_model_desc *desc);
 int hphi_engine_info(const hphi_engine *e, uint64_t *dim, uint64_t *size21, uint64_t *size22,
                      uint64_t *irght, uint64_t *ilft, uint64_t *ihfbit);
 /* Launch-shape knobs of the H·v kernels (set by the autotuner or by hand):
- *   "tpb" 256|512|1024, "wgs_per_cu" 1..64, "lds21" 0|1 (list_2_1 staged in
+ *   "tpb" 256|512|1024, "wgs_per_cu" 1..256, "lds21" 0|1 (list_2_1 staged in
  *   LDS), "xcd_map" 0|1 (contiguous state ranges per XCD), "use_c2" 0|1
  *   (32-bit canonical kernel), "grid_per_cu" 1..64 (generic kernel);
+ *   two-pass H·v of canonical sectors (high-half terms over a tile-major copy
+ *   of v1, one extra vector of device memory): "hi_split" -1|0|1 (-1: Spin
+ *   sectors of at least 2^27 states), "hi_wgs_per_cu" 1..64 (high pass),
+ *   "hi_p1_wgs_per_cu" 1..256 (first pass), "hi_fuse_copy" 0|1;
  *   block kernel (k > 1): "use_blk" 0|1, "blk_cpl2|4|8|16" columns per lane
  *   (1, 4 or k; 0 = default), "blk_tpb" 0|256|1024, "blk_wgs_per_cu" 0..64;
  *   LOBPCG: "lb_mfma" 0|1 (block-8 Gram matrices on the FP64 matrix cores). */
