@@ -425,7 +425,8 @@ int hphi_hi_wanted(const hphi_engine *e) {
 
 // The plan: the non-empty blocks from list_2_2 grouped by class, checked to be
 // laid out as the pass assumes (consecutive in block order, every block of a
-// class the same size); a sector that does not match keeps the single pass.
+// class the same size); a sector that does not match, or a device without
+// room for the tile-major copy, keeps the single pass.
 int hphi_hi_split_on(hphi_engine *e) {
   if (!hphi_hi_wanted(e)) return 0;
   if (e->d_v1t) return 1;
@@ -514,7 +515,8 @@ int hphi_hi_split_on(hphi_engine *e) {
     e->d_blkpos = nullptr;
     e->d_v1t = nullptr;
     e->d_tofs = nullptr;
-    return hphi_set_error("high-half pass: allocation of its tables and tile-major vector failed");
+    e->hi_ok = 0;  // no room for the tile-major copy: the single pass serves this engine
+    return 0;
   }
   e->bytes += sizeof(uint32_t) * (hb.size() + nb) + sizeof(double2) * base + sizeof(ulonglong2) * nb;
   e->hi_plan = P;
